@@ -1,0 +1,190 @@
+"""ctypes loader for the CPU oracle (oracle/lib/liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker.  The product
+(autom2_amd) never imports this module.
+
+Array conventions follow TiltRec-mpi after RotateX
+(src/TiltRec/src/mrcmx/mrcstack.cpp:566-579): projections [P][rows][X],
+volume [rows][T][X].  Origins are (X/2, oy, T/2); TiltRec-mpi uses oy = T/2
+(main-mpi.cpp:743-745), the CUDA and Z binaries use oy = Y/2.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "liborc.so")
+_lib = None
+
+_F = ctypes.POINTER(ctypes.c_float)
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.c_int
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc only)."""
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE, "lib/liborc.so"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_coeffs_mpi.argtypes = [_F, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _D]
+        L.orc_coeffs_cuda.argtypes = [_F, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _F]
+        for n in ("orc_bpt_f32",):
+            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _F, _F]
+        for n in ("orc_sirt_f32", "orc_sart_f32"):
+            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _F, _I, ctypes.c_float, _F]
+        L.orc_reproject_f32.argtypes = [_D, _I, _I, _I, _F, _F, _F, _F]
+        L.orc_reproject_f64.argtypes = [_D, _I, _I, _I, _F, _D, _D, _D]
+        L.orc_backproject_sums_f64.argtypes = [_D, _I, _I, _I, _F, _D, _D, _D]
+        L.orc_bpt_f64.argtypes = [_D, _I, _I, _I, _I, _F, _D, _D]
+        for n in ("orc_sirt_f64", "orc_sart_f64"):
+            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, ctypes.c_double, _D]
+        L.orc_rweight_f64.argtypes = [_D, _I, _I]
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(_F)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _org(org):
+    return (ctypes.c_float * 3)(*[float(v) for v in org])
+
+
+def coeffs_mpi(angles, offset=0.0, pitch=0.0, zshift=0.0):
+    """[P][8] double coefficients (a0..a3, b0..b3), main-mpi.cpp:85-183."""
+    ang = np.ascontiguousarray(angles, dtype=np.float32)
+    out = np.zeros((len(ang), 8), dtype=np.float64)
+    lib().orc_coeffs_mpi(_fp(ang), len(ang), offset, pitch, zshift, _dp(out))
+    return out
+
+
+def coeffs_cuda(angles, offset=0.0, pitch=0.0, zshift=0.0):
+    """[P][8] float coefficients, main-cuda.cpp:111-168."""
+    ang = np.ascontiguousarray(angles, dtype=np.float32)
+    out = np.zeros((len(ang), 8), dtype=np.float32)
+    lib().orc_coeffs_cuda(_fp(ang), len(ang), offset, pitch, zshift, _fp(out))
+    return out
+
+
+def origin(X, T, oy):
+    return (np.float32(X * 0.5), np.float32(oy), np.float32(T * 0.5))
+
+
+def _dims(proj, T):
+    P, rows, X = proj.shape
+    return P, X, rows, T
+
+
+def reproject_f32(coef1, vol, org):
+    """One tilt's Reproject (main-mpi.cpp:286-344): returns (val, wt) [rows][X]."""
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    rows, T, X = vol.shape
+    rv = np.zeros((rows, X), np.float32)
+    rw = np.zeros((rows, X), np.float32)
+    c = np.ascontiguousarray(coef1, dtype=np.float64)
+    lib().orc_reproject_f32(_dp(c), X, rows, T, _org(org), _fp(vol), _fp(rv), _fp(rw))
+    return rv, rw
+
+
+def reproject_f64(coef1, vol, org):
+    vol = np.ascontiguousarray(vol, dtype=np.float64)
+    rows, T, X = vol.shape
+    rv = np.zeros((rows, X), np.float64)
+    rw = np.zeros((rows, X), np.float64)
+    c = np.ascontiguousarray(coef1, dtype=np.float64)
+    lib().orc_reproject_f64(_dp(c), X, rows, T, _org(org), _dp(vol), _dp(rv), _dp(rw))
+    return rv, rw
+
+
+def backproject_sums_f64(coef1, img, T, org):
+    """Per-voxel BilinearValue sums (s, c) of one tilt, [rows][T][X]."""
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    rows, X = img.shape
+    s = np.zeros((rows, T, X), np.float64)
+    c = np.zeros((rows, T, X), np.float64)
+    cc = np.ascontiguousarray(coef1, dtype=np.float64)
+    lib().orc_backproject_sums_f64(_dp(cc), X, rows, T, _org(org), _dp(img), _dp(s), _dp(c))
+    return s, c
+
+
+def bpt_f32(coef, proj, T, org, vol=None):
+    proj = np.ascontiguousarray(proj, dtype=np.float32)
+    P, X, rows, _ = _dims(proj, T)
+    out = np.zeros((rows, T, X), np.float32) if vol is None else np.array(vol, np.float32)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    lib().orc_bpt_f32(_dp(c), P, X, rows, T, _org(org), _fp(proj), _fp(out))
+    return out
+
+
+def bpt_f64(coef, proj, T, org, vol=None):
+    proj = np.ascontiguousarray(proj, dtype=np.float64)
+    P, X, rows, _ = _dims(proj, T)
+    out = np.zeros((rows, T, X), np.float64) if vol is None else np.array(vol, np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    lib().orc_bpt_f64(_dp(c), P, X, rows, T, _org(org), _dp(proj), _dp(out))
+    return out
+
+
+def _iter_f32(name, coef, proj, T, org, iters, gamma, vol):
+    proj = np.ascontiguousarray(proj, dtype=np.float32)
+    P, X, rows, _ = _dims(proj, T)
+    out = np.zeros((rows, T, X), np.float32) if vol is None else np.array(vol, np.float32)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    getattr(lib(), name)(_dp(c), P, X, rows, T, _org(org), _fp(proj), iters, gamma, _fp(out))
+    return out
+
+
+def _iter_f64(name, coef, proj, T, org, iters, gamma, vol):
+    proj = np.ascontiguousarray(proj, dtype=np.float64)
+    P, X, rows, _ = _dims(proj, T)
+    out = np.zeros((rows, T, X), np.float64) if vol is None else np.array(vol, np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    getattr(lib(), name)(_dp(c), P, X, rows, T, _org(org), _dp(proj), iters, gamma, _dp(out))
+    return out
+
+
+def sirt_f32(coef, proj, T, org, iters, gamma=1.0, vol=None):
+    return _iter_f32("orc_sirt_f32", coef, proj, T, org, iters, gamma, vol)
+
+
+def sart_f32(coef, proj, T, org, iters, gamma=1.0, vol=None):
+    return _iter_f32("orc_sart_f32", coef, proj, T, org, iters, gamma, vol)
+
+
+def sirt_f64(coef, proj, T, org, iters, gamma=1.0, vol=None):
+    return _iter_f64("orc_sirt_f64", coef, proj, T, org, iters, gamma, vol)
+
+
+def sart_f64(coef, proj, T, org, iters, gamma=1.0, vol=None):
+    return _iter_f64("orc_sart_f64", coef, proj, T, org, iters, gamma, vol)
+
+
+def rweight_f64(rows):
+    """ApplyFilterInplace mode 2 on [..., X] rows, FP64 (filter_prj.cpp:652-736)."""
+    a = np.array(rows, dtype=np.float64, copy=True)
+    shp = a.shape
+    a = np.ascontiguousarray(a.reshape(-1, shp[-1]))
+    lib().orc_rweight_f64(_dp(a), shp[-1], a.shape[0])
+    return a.reshape(shp)
+
+
+def rel_l2(a, b):
+    """||a - b|| / ||b|| in float64."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b.ravel())
+    return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))

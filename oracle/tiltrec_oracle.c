@@ -1,0 +1,583 @@
+/*
+ * tiltrec_oracle.c -- CPU restatement of TiltRec's reconstruction arithmetic.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load the library built from it.  The HIP library in autom2_amd/ never
+ * links, loads or falls back to it.
+ *
+ * What it restates (paths relative to the AuTom2 source tree):
+ *   - coefficients:  src/TiltRec/src/main/main-mpi.cpp:85-183 (double storage,
+ *                    float trig) and main-cuda.cpp:111-168 (float storage)
+ *   - ValCoef:       main-mpi.cpp:204-226
+ *   - Reproject:     main-mpi.cpp:286-344 (the y-major variant SIRT/SART use)
+ *   - BilinearValue: main-mpi.cpp:378-410
+ *   - BackProject:   main-mpi.cpp:411-450 (BPT, and WBP after filtering)
+ *   - SART:          main-mpi.cpp:492-524, 578-612
+ *   - SIRT:          main-mpi.cpp:526-576, 614-664
+ *   - R-weighting:   src/TiltRec/src/filter/filter_prj.cpp:636-736 (mode 2 of
+ *                    ApplyFilterInplace, :738-802), evaluated in double with an
+ *                    explicit DFT instead of FFTW/cuFFT.
+ *
+ * Two arithmetic forms are provided for every method:
+ *   *_f32 -- the reference's own types: double coordinates (ValCoef), float
+ *            weights and float accumulators, in the reference's loop order.
+ *            Built with -ffp-contract=off so no FMA changes the rounding.
+ *   *_f64 -- the same loops with every weight and accumulator in double: the
+ *            FP64 oracle that states the error of a float result.
+ *
+ * Data layouts follow the TiltRec-mpi y path after RotateX
+ * (mrcstack.cpp:566-579): projections are [P][rows][X] and the volume is
+ * [rows][T][X] (one XZ section per row, the order WriteBlock('z') writes).
+ *
+ * Parity of each function is pinned in tests/test_oracle.py against closed
+ * forms, a brute-force FP64 system matrix, adjointness and numpy's FFT.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+
+#define ORC_PI_180 0.01745329252f /* PI_180, main-mpi.cpp:9 */
+
+/* ------------------------------------------------------------------ */
+/* coefficients                                                         */
+/* ------------------------------------------------------------------ */
+
+/* main-mpi.cpp:85-106 then :110-183, linear terms only (the quadratic
+ * coefficients a4..a9/b4..b9 start at zero and ValCoef never reads them).
+ * coef[8*i + 0..3] = a0..a3, coef[8*i + 4..7] = b0..b3.
+ * cos()/sin() of a float argument resolve to the float overloads in the
+ * reference (mrcstack.h includes <math.h>, which brings std::cos(float) into
+ * scope), so the tilt trig is cosf/sinf widened to double. */
+void orc_coeffs_mpi(const float *angles, int P, float offset, float pitch,
+                    float zshift, double *coef)
+{
+    double alpha = -(pitch * ORC_PI_180);
+    double beta = offset * ORC_PI_180;
+    double t = -zshift;
+    double ca = cos(alpha), sa = sin(alpha), cb = cos(beta), sb = sin(beta);
+    for (int i = 0; i < P; i++) {
+        float r = angles[i] * ORC_PI_180;
+        double a[4] = {0.0, (double)cosf(r), 0.0, (double)(-sinf(r))};
+        double b[4] = {0.0, 0.0, 1.0, 0.0};
+        double na[4], nb[4];
+        na[0] = a[0];
+        na[1] = (a[2] * sa * sb + a[3] * ca * sb + a[1] * cb);
+        na[2] = (a[2] * ca - a[3] * sa);
+        na[3] = (a[3] * ca * cb + a[2] * cb * sa - a[1] * sb);
+        nb[0] = b[0];
+        nb[1] = (b[2] * sa * sb + b[3] * ca * sb + b[1] * cb);
+        nb[2] = (b[2] * ca - b[3] * sa);
+        nb[3] = (b[3] * ca * cb + b[2] * cb * sa - b[1] * sb);
+        /* z shift, main-mpi.cpp:167-182 (a9 = b9 = 0, a5 = a6 = 0) */
+        double *c = coef + 8 * (size_t)i;
+        c[0] = na[0] + na[3] * t;
+        c[1] = na[1];
+        c[2] = na[2];
+        c[3] = na[3];
+        c[4] = nb[0] + nb[3] * t;
+        c[5] = nb[1];
+        c[6] = nb[2];
+        c[7] = nb[3];
+    }
+}
+
+/* main-cuda.cpp:111-168: the same map held in float (SimCoeff). */
+void orc_coeffs_cuda(const float *angles, int P, float offset, float pitch,
+                     float zshift, float *coef)
+{
+    float alpha = -(pitch * ORC_PI_180), beta = offset * ORC_PI_180, t = -zshift;
+    float ca = cosf(alpha), sa = sinf(alpha), cb = cosf(beta), sb = sinf(beta);
+    for (int i = 0; i < P; i++) {
+        float r = angles[i] * ORC_PI_180;
+        float a[4] = {0.f, cosf(r), 0.f, -sinf(r)};
+        float b[4] = {0.f, 0.f, 1.f, 0.f};
+        float *c = coef + 8 * (size_t)i;
+        float a3 = (a[3] * ca * cb + a[2] * cb * sa - a[1] * sb);
+        float b3 = (b[3] * ca * cb + b[2] * cb * sa - b[1] * sb);
+        c[0] = a[0] + a3 * t;
+        c[1] = (a[2] * sa * sb + a[3] * ca * sb + a[1] * cb);
+        c[2] = (a[2] * ca - a[3] * sa);
+        c[3] = a3;
+        c[4] = b[0] + b3 * t;
+        c[5] = (b[2] * sa * sb + b[3] * ca * sb + b[1] * cb);
+        c[6] = (b[2] * ca - b[3] * sa);
+        c[7] = b3;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* ValCoef / taps                                                        */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int xm, ym;
+    float xd, yd;   /* Weight::x_min_del / y_min_del are float (mrcstack.h:291-298) */
+    double xdd, ydd; /* the same fractions before the float rounding (FP64 oracle) */
+} orc_w;
+
+/* main-mpi.cpp:204-226.  coord - origin is int - float, evaluated in float. */
+static inline void valcoef(const float org[3], int cx, int cy, int cz,
+                           const double *c, orc_w *w)
+{
+    double X = (float)((float)cx - org[0]);
+    double Y = (float)((float)cy - org[1]);
+    double Z = (float)((float)cz - org[2]);
+    double n0 = c[0] + c[1] * X + c[2] * Y + c[3] * Z;
+    double n1 = c[4] + c[5] * X + c[6] * Y + c[7] * Z;
+    double x = n0 + (double)org[0];
+    double y = n1 + (double)org[1];
+    w->xm = (int)floor(x);
+    w->ym = (int)floor(y);
+    w->xdd = x - (double)w->xm;
+    w->ydd = y - (double)w->ym;
+    w->xd = (float)w->xdd;
+    w->yd = (float)w->ydd;
+}
+
+/* ------------------------------------------------------------------ */
+/* FP32 form                                                             */
+/* ------------------------------------------------------------------ */
+
+/* Reproject, main-mpi.cpp:286-344.  vol [len][T][W]; rv/rw [len][W], added to. */
+static void reproject_f32(const float org[3], const float *vol, const double *c,
+                          float *rv, float *rw, int T, int len, int W)
+{
+    for (int y = 0; y < len; y++) {
+        const float *vy = vol + (size_t)y * W * T;
+        for (int z = 0; z < T; z++) {
+            const float *vz = vy + (size_t)z * W;
+            for (int x = 0; x < W; x++) {
+                float v = vz[x];
+                orc_w w;
+                valcoef(org, x, y, z, c, &w);
+                int n;
+                if (w.xm >= 0 && w.xm < W && w.ym >= 0 && w.ym < len) {
+                    n = w.xm + w.ym * W;
+                    rv[n] += (1 - w.xd) * (1 - w.yd) * v;
+                    rw[n] += (1 - w.xd) * (1 - w.yd);
+                }
+                if ((w.xm + 1) >= 0 && (w.xm + 1) < W && w.ym >= 0 && w.ym < len) {
+                    n = w.xm + 1 + w.ym * W;
+                    rv[n] += w.xd * (1 - w.yd) * v;
+                    rw[n] += w.xd * (1 - w.yd);
+                }
+                if (w.xm >= 0 && w.xm < W && (w.ym + 1) >= 0 && (w.ym + 1) < len) {
+                    n = w.xm + (w.ym + 1) * W;
+                    rv[n] += (1 - w.xd) * w.yd * v;
+                    rw[n] += (1 - w.xd) * w.yd;
+                }
+                if ((w.xm + 1) >= 0 && (w.xm + 1) < W && (w.ym + 1) >= 0 && (w.ym + 1) < len) {
+                    n = (w.xm + 1) + (w.ym + 1) * W;
+                    rv[n] += w.xd * w.yd * v;
+                    rw[n] += w.xd * w.yd;
+                }
+            }
+        }
+    }
+}
+
+/* BilinearValue, main-mpi.cpp:378-410 */
+static inline void bilinear_f32(int W, int len, const float *img, const orc_w *w,
+                                float *val, float *vwt)
+{
+    int n;
+    if (w->xm >= 0 && w->xm < W && w->ym >= 0 && w->ym < len) {
+        n = w->xm + w->ym * W;
+        *val += (1 - w->xd) * (1 - w->yd) * img[n];
+        *vwt += (1 - w->xd) * (1 - w->yd);
+    }
+    if ((w->xm + 1) >= 0 && (w->xm + 1) < W && w->ym >= 0 && w->ym < len) {
+        n = w->xm + 1 + w->ym * W;
+        *val += w->xd * (1 - w->yd) * img[n];
+        *vwt += w->xd * (1 - w->yd);
+    }
+    if (w->xm >= 0 && w->xm < W && (w->ym + 1) >= 0 && (w->ym + 1) < len) {
+        n = w->xm + (w->ym + 1) * W;
+        *val += (1 - w->xd) * w->yd * img[n];
+        *vwt += (1 - w->xd) * w->yd;
+    }
+    if ((w->xm + 1) >= 0 && (w->xm + 1) < W && (w->ym + 1) >= 0 && (w->ym + 1) < len) {
+        n = w->xm + 1 + (w->ym + 1) * W;
+        *val += w->xd * w->yd * img[n];
+        *vwt += w->xd * w->yd;
+    }
+}
+
+/* One forward projection of one tilt: rv/rw [len][W] are overwritten. */
+void orc_reproject_f32(const double *coef, int W, int len, int T,
+                       const float org[3], const float *vol, float *rv, float *rw)
+{
+    memset(rv, 0, sizeof(float) * (size_t)W * len);
+    memset(rw, 0, sizeof(float) * (size_t)W * len);
+    reproject_f32(org, vol, coef, rv, rw, T, len, W);
+}
+
+/* BackProject, main-mpi.cpp:411-450 (BPT).  proj [P][len][W], vol [len][T][W]
+ * is added to (the reference starts from zero or from --initial). */
+void orc_bpt_f32(const double *coef, int P, int W, int len, int T,
+                 const float org[3], const float *proj, float *vol)
+{
+    for (int idx = 0; idx < P; idx++) {
+        const float *one = proj + (size_t)W * len * idx;
+        for (int y = 0; y < len; y++) {
+            float *vy = vol + (size_t)y * W * T;
+            for (int z = 0; z < T; z++) {
+                float *vz = vy + (size_t)z * W;
+                for (int x = 0; x < W; x++) {
+                    orc_w w;
+                    float s = 0, c = 0;
+                    valcoef(org, x, y, z, coef + 8 * (size_t)idx, &w);
+                    bilinear_f32(W, len, one, &w, &s, &c);
+                    if (c) vz[x] += (float)(s / c);
+                }
+            }
+        }
+    }
+}
+
+/* SIRT, main-mpi.cpp:614-664 with UpdateWeightsByProjDiff (:526-562). */
+void orc_sirt_f32(const double *coef, int P, int W, int len, int T,
+                  const float org[3], const float *proj, int iters, float gamma,
+                  float *vol)
+{
+    size_t px = (size_t)len * W, size = (size_t)len * W * T;
+    float *rv = (float *)malloc(sizeof(float) * px);
+    float *rw = (float *)malloc(sizeof(float) * px);
+    float *valvol = (float *)malloc(sizeof(float) * size);
+    float *wtvol = (float *)malloc(sizeof(float) * size);
+    for (int it = 0; it < iters; it++) {
+        memset(valvol, 0, sizeof(float) * size);
+        memset(wtvol, 0, sizeof(float) * size);
+        for (int idx = 0; idx < P; idx++) {
+            const double *c = coef + 8 * (size_t)idx;
+            memset(rv, 0, sizeof(float) * px);
+            memset(rw, 0, sizeof(float) * px);
+            reproject_f32(org, vol, c, rv, rw, T, len, W);
+            const float *pd = proj + (size_t)W * len * idx;
+            for (size_t n = 0; n < px; n++) {
+                if (rw[n]) rv[n] /= rw[n];
+                rv[n] = pd[n] - rv[n];
+            }
+            for (int y = 0; y < len; y++) {
+                float *vy = valvol + (size_t)y * W * T;
+                float *wy = wtvol + (size_t)y * W * T;
+                for (int z = 0; z < T; z++) {
+                    for (int x = 0; x < W; x++) {
+                        orc_w w;
+                        float s = 0, cc = 0;
+                        valcoef(org, x, y, z, c, &w);
+                        bilinear_f32(W, len, rv, &w, &s, &cc);
+                        vy[(size_t)z * W + x] += s;
+                        wy[(size_t)z * W + x] += cc;
+                    }
+                }
+            }
+        }
+        for (size_t i = size; i--;) {
+            if (wtvol[i]) vol[i] += valvol[i] / wtvol[i] * gamma;
+        }
+    }
+    free(rv);
+    free(rw);
+    free(valvol);
+    free(wtvol);
+}
+
+/* SART, main-mpi.cpp:578-612 with UpdateVolumeByProjDiff (:492-524). */
+void orc_sart_f32(const double *coef, int P, int W, int len, int T,
+                  const float org[3], const float *proj, int iters, float gamma,
+                  float *vol)
+{
+    size_t px = (size_t)len * W;
+    float *rv = (float *)malloc(sizeof(float) * px);
+    float *rw = (float *)malloc(sizeof(float) * px);
+    for (int it = 0; it < iters; it++) {
+        for (int idx = 0; idx < P; idx++) {
+            const double *c = coef + 8 * (size_t)idx;
+            memset(rv, 0, sizeof(float) * px);
+            memset(rw, 0, sizeof(float) * px);
+            reproject_f32(org, vol, c, rv, rw, T, len, W);
+            const float *pd = proj + (size_t)W * len * idx;
+            for (size_t n = 0; n < px; n++) {
+                if (rw[n] != 0) rv[n] /= rw[n];
+                rv[n] = pd[n] - rv[n];
+            }
+            for (int y = 0; y < len; y++) {
+                float *vy = vol + (size_t)y * W * T;
+                for (int z = 0; z < T; z++) {
+                    for (int x = 0; x < W; x++) {
+                        orc_w w;
+                        float s = 0, cc = 0;
+                        valcoef(org, x, y, z, c, &w);
+                        bilinear_f32(W, len, rv, &w, &s, &cc);
+                        if (cc) vy[(size_t)z * W + x] += (float)(s / cc) * gamma;
+                    }
+                }
+            }
+        }
+    }
+    free(rv);
+    free(rw);
+}
+
+/* ------------------------------------------------------------------ */
+/* FP64 form: identical loops, double weights and accumulators          */
+/* ------------------------------------------------------------------ */
+
+static void reproject_f64(const float org[3], const double *vol, const double *c,
+                          double *rv, double *rw, int T, int len, int W)
+{
+    for (int y = 0; y < len; y++) {
+        const double *vy = vol + (size_t)y * W * T;
+        for (int z = 0; z < T; z++) {
+            const double *vz = vy + (size_t)z * W;
+            for (int x = 0; x < W; x++) {
+                double v = vz[x];
+                orc_w w;
+                valcoef(org, x, y, z, c, &w);
+                double xd = w.xdd, yd = w.ydd;
+                int n;
+                if (w.xm >= 0 && w.xm < W && w.ym >= 0 && w.ym < len) {
+                    n = w.xm + w.ym * W;
+                    rv[n] += (1 - xd) * (1 - yd) * v;
+                    rw[n] += (1 - xd) * (1 - yd);
+                }
+                if ((w.xm + 1) >= 0 && (w.xm + 1) < W && w.ym >= 0 && w.ym < len) {
+                    n = w.xm + 1 + w.ym * W;
+                    rv[n] += xd * (1 - yd) * v;
+                    rw[n] += xd * (1 - yd);
+                }
+                if (w.xm >= 0 && w.xm < W && (w.ym + 1) >= 0 && (w.ym + 1) < len) {
+                    n = w.xm + (w.ym + 1) * W;
+                    rv[n] += (1 - xd) * yd * v;
+                    rw[n] += (1 - xd) * yd;
+                }
+                if ((w.xm + 1) >= 0 && (w.xm + 1) < W && (w.ym + 1) >= 0 && (w.ym + 1) < len) {
+                    n = (w.xm + 1) + (w.ym + 1) * W;
+                    rv[n] += xd * yd * v;
+                    rw[n] += xd * yd;
+                }
+            }
+        }
+    }
+}
+
+static inline void bilinear_f64(int W, int len, const double *img, const orc_w *w,
+                                double *val, double *vwt)
+{
+    double xd = w->xdd, yd = w->ydd;
+    int n;
+    if (w->xm >= 0 && w->xm < W && w->ym >= 0 && w->ym < len) {
+        n = w->xm + w->ym * W;
+        *val += (1 - xd) * (1 - yd) * img[n];
+        *vwt += (1 - xd) * (1 - yd);
+    }
+    if ((w->xm + 1) >= 0 && (w->xm + 1) < W && w->ym >= 0 && w->ym < len) {
+        n = w->xm + 1 + w->ym * W;
+        *val += xd * (1 - yd) * img[n];
+        *vwt += xd * (1 - yd);
+    }
+    if (w->xm >= 0 && w->xm < W && (w->ym + 1) >= 0 && (w->ym + 1) < len) {
+        n = w->xm + (w->ym + 1) * W;
+        *val += (1 - xd) * yd * img[n];
+        *vwt += (1 - xd) * yd;
+    }
+    if ((w->xm + 1) >= 0 && (w->xm + 1) < W && (w->ym + 1) >= 0 && (w->ym + 1) < len) {
+        n = w->xm + 1 + (w->ym + 1) * W;
+        *val += xd * yd * img[n];
+        *vwt += xd * yd;
+    }
+}
+
+void orc_reproject_f64(const double *coef, int W, int len, int T,
+                       const float org[3], const double *vol, double *rv, double *rw)
+{
+    memset(rv, 0, sizeof(double) * (size_t)W * len);
+    memset(rw, 0, sizeof(double) * (size_t)W * len);
+    reproject_f64(org, vol, coef, rv, rw, T, len, W);
+}
+
+/* The per-voxel sums of one tilt's back-projection (BilinearValue over a
+ * whole slab): s[len][T][W] and c[len][T][W] are overwritten. */
+void orc_backproject_sums_f64(const double *coef, int W, int len, int T,
+                              const float org[3], const double *img,
+                              double *s, double *c)
+{
+    for (int y = 0; y < len; y++)
+        for (int z = 0; z < T; z++)
+            for (int x = 0; x < W; x++) {
+                orc_w w;
+                double sv = 0, cv = 0;
+                valcoef(org, x, y, z, coef, &w);
+                bilinear_f64(W, len, img, &w, &sv, &cv);
+                size_t i = ((size_t)y * T + z) * W + x;
+                s[i] = sv;
+                c[i] = cv;
+            }
+}
+
+void orc_bpt_f64(const double *coef, int P, int W, int len, int T,
+                 const float org[3], const double *proj, double *vol)
+{
+    for (int idx = 0; idx < P; idx++) {
+        const double *one = proj + (size_t)W * len * idx;
+        for (int y = 0; y < len; y++)
+            for (int z = 0; z < T; z++)
+                for (int x = 0; x < W; x++) {
+                    orc_w w;
+                    double s = 0, c = 0;
+                    valcoef(org, x, y, z, coef + 8 * (size_t)idx, &w);
+                    bilinear_f64(W, len, one, &w, &s, &c);
+                    if (c) vol[((size_t)y * T + z) * W + x] += s / c;
+                }
+    }
+}
+
+void orc_sirt_f64(const double *coef, int P, int W, int len, int T,
+                  const float org[3], const double *proj, int iters, double gamma,
+                  double *vol)
+{
+    size_t px = (size_t)len * W, size = (size_t)len * W * T;
+    double *rv = (double *)malloc(sizeof(double) * px);
+    double *rw = (double *)malloc(sizeof(double) * px);
+    double *valvol = (double *)malloc(sizeof(double) * size);
+    double *wtvol = (double *)malloc(sizeof(double) * size);
+    for (int it = 0; it < iters; it++) {
+        memset(valvol, 0, sizeof(double) * size);
+        memset(wtvol, 0, sizeof(double) * size);
+        for (int idx = 0; idx < P; idx++) {
+            const double *c = coef + 8 * (size_t)idx;
+            memset(rv, 0, sizeof(double) * px);
+            memset(rw, 0, sizeof(double) * px);
+            reproject_f64(org, vol, c, rv, rw, T, len, W);
+            const double *pd = proj + (size_t)W * len * idx;
+            for (size_t n = 0; n < px; n++) {
+                if (rw[n]) rv[n] /= rw[n];
+                rv[n] = pd[n] - rv[n];
+            }
+            for (int y = 0; y < len; y++)
+                for (int z = 0; z < T; z++)
+                    for (int x = 0; x < W; x++) {
+                        orc_w w;
+                        double s = 0, cc = 0;
+                        valcoef(org, x, y, z, c, &w);
+                        bilinear_f64(W, len, rv, &w, &s, &cc);
+                        size_t i = ((size_t)y * T + z) * W + x;
+                        valvol[i] += s;
+                        wtvol[i] += cc;
+                    }
+        }
+        for (size_t i = size; i--;) {
+            if (wtvol[i]) vol[i] += valvol[i] / wtvol[i] * gamma;
+        }
+    }
+    free(rv);
+    free(rw);
+    free(valvol);
+    free(wtvol);
+}
+
+void orc_sart_f64(const double *coef, int P, int W, int len, int T,
+                  const float org[3], const double *proj, int iters, double gamma,
+                  double *vol)
+{
+    size_t px = (size_t)len * W;
+    double *rv = (double *)malloc(sizeof(double) * px);
+    double *rw = (double *)malloc(sizeof(double) * px);
+    for (int it = 0; it < iters; it++) {
+        for (int idx = 0; idx < P; idx++) {
+            const double *c = coef + 8 * (size_t)idx;
+            memset(rv, 0, sizeof(double) * px);
+            memset(rw, 0, sizeof(double) * px);
+            reproject_f64(org, vol, c, rv, rw, T, len, W);
+            const double *pd = proj + (size_t)W * len * idx;
+            for (size_t n = 0; n < px; n++) {
+                if (rw[n] != 0) rv[n] /= rw[n];
+                rv[n] = pd[n] - rv[n];
+            }
+            for (int y = 0; y < len; y++)
+                for (int z = 0; z < T; z++)
+                    for (int x = 0; x < W; x++) {
+                        orc_w w;
+                        double s = 0, cc = 0;
+                        valcoef(org, x, y, z, c, &w);
+                        bilinear_f64(W, len, rv, &w, &s, &cc);
+                        if (cc) vol[((size_t)y * T + z) * W + x] += s / cc * gamma;
+                    }
+        }
+    }
+    free(rv);
+    free(rw);
+}
+
+/* ------------------------------------------------------------------ */
+/* R-weighting filter (ApplyFilterInplace mode 2), FP64                 */
+/* ------------------------------------------------------------------ */
+
+/* filter_prj.cpp:652-736 for `rows` rows of length W, in place, in double:
+ * extend each row to N = W + W/10 with the linear ramp of buf2fft_padding_1D,
+ * forward real DFT, weight (0.2 at DC, k up to K = floor(0.05 N), a Gaussian
+ * fall-off of width 0.5 N beyond), unnormalised inverse real DFT, keep the
+ * first W samples and divide by W.  The DFT is evaluated directly (O(N^2)),
+ * which is exact up to double rounding; numpy's FFT pins it in the tests. */
+void orc_rweight_f64(double *data, int W, int rows)
+{
+    int N = W + W / 10;
+    int nxp = N - W + 1;
+    int K = (int)floorf((float)N * 0.05f); /* int(floor(Nx_final * radial)), radial = 0.05f */
+    double sig = (double)((float)N * 0.5f); /* sigma_Nx = float(Nx_final) * 0.5f */
+    int nb = N / 2 + 1; /* bins the r2c transform produces */
+    double *e = (double *)malloc(sizeof(double) * N);
+    double *re = (double *)malloc(sizeof(double) * nb);
+    double *im = (double *)malloc(sizeof(double) * nb);
+    double *ct = (double *)malloc(sizeof(double) * N);
+    double *st = (double *)malloc(sizeof(double) * N);
+    for (int m = 0; m < N; m++) {
+        ct[m] = cos(2.0 * M_PI * m / N);
+        st[m] = sin(2.0 * M_PI * m / N);
+    }
+    for (int r = 0; r < rows; r++) {
+        double *row = data + (size_t)r * W;
+        for (int j = 0; j < W; j++) e[j] = row[j];
+        for (int j = W; j < N; j++)
+            e[j] = ((double)(j - W + 1) / nxp) * row[0] + ((double)(N - j) / nxp) * row[W - 1];
+        for (int k = 0; k < nb; k++) {
+            double sr = 0, si = 0;
+            for (int j = 0; j < N; j++) {
+                int m = (int)(((long long)k * j) % N);
+                sr += e[j] * ct[m];
+                si -= e[j] * st[m];
+            }
+            double wgt;
+            if (k == 0)
+                wgt = 0.2;
+            else if (k <= K)
+                wgt = k;
+            else
+                wgt = (double)K * exp(-((double)(k - K) * (k - K)) / (sig * sig));
+            re[k] = sr * wgt;
+            im[k] = si * wgt;
+        }
+        for (int j = 0; j < W; j++) {
+            /* unnormalised c2r: Hermitian extension of the half spectrum;
+             * the imaginary parts of DC and (even N) Nyquist are dropped */
+            double acc = re[0];
+            for (int k = 1; k < nb; k++) {
+                int m = (int)(((long long)k * j) % N);
+                double term = re[k] * ct[m] - im[k] * st[m];
+                if ((N % 2 == 0) && k == N / 2)
+                    acc += re[k] * ct[m];
+                else
+                    acc += 2.0 * term;
+            }
+            row[j] = acc / W;
+        }
+    }
+    free(e);
+    free(re);
+    free(im);
+    free(ct);
+    free(st);
+}
