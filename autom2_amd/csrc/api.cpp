@@ -1,0 +1,1127 @@
+// api.cpp -- host side of libtiltrec_hip.so: the C ABI of include/tiltrec.h.
+//
+// A call validates its arguments, builds each job's per-tilt constants on
+// the host (the coefficient arithmetic of main-mpi.cpp:85-183), carves the
+// job's buffers out of the context's cached device arena, uploads the small
+// descriptors through pinned staging, and issues the kernels of kernels.hip.
+// Work is organised in stages (one SIRT iteration, one SART tilt, one
+// back-projection pass); jobs of a batch run stage by stage, every stage as
+// one multi-job launch per kernel, and the whole schedule is captured in a
+// HIP graph that is cached on the context and replayed while nothing
+// changes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "trec_internal.h"
+#include "trec_host.h"
+
+using namespace trec;
+
+namespace {
+
+constexpr float PI_180 = 0.01745329252f;  // main-mpi.cpp:9
+constexpr size_t ALIGN = 256;
+
+size_t align_up(size_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
+
+std::mutex g_tune_mu;
+std::map<std::string, int> g_tuned_nzc;  // process-wide autotune results
+
+std::string tune_key(int X, int T, int P, int storage)
+{
+    char b[96];
+    snprintf(b, sizeof b, "%d:%d:%d:%d", X, T, P, storage);
+    return b;
+}
+
+}  // namespace
+
+namespace trec {
+
+// Event timing of launch groups (trec_profile_*).
+enum ProfKind { PROF_PREP = 0, PROF_FORWARD, PROF_COMBINE, PROF_BACK, PROF_FINISH, PROF_N };
+const char *const prof_names[PROF_N] = {"prep", "forward", "combine", "back", "finish"};
+
+struct Prof {
+    bool on = false;
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[PROF_N] = {};
+    long long count[PROF_N] = {};
+    hipEvent_t get()
+    {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+
+thread_local Prof *tl_prof = nullptr;
+
+// Brackets a launch group with events when profiling is on.
+struct ProfScope {
+    int kind;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    ProfScope(int k, hipStream_t s) : kind(k), st(s)
+    {
+        if (tl_prof && tl_prof->on) {
+            a = tl_prof->get();
+            if (a && hipEventRecord(a, st) != hipSuccess) a = nullptr;
+        }
+    }
+    ~ProfScope()
+    {
+        if (!a) return;
+        hipEvent_t b = tl_prof->get();
+        if (b && hipEventRecord(b, st) == hipSuccess) tl_prof->pending.push_back({kind, a, b});
+    }
+};
+
+}  // namespace trec
+
+// ------------------------------------------------------------------ context
+
+struct trec_context {
+    int device = 0;
+    std::string err;
+    char *arena = nullptr;
+    size_t arena_size = 0;
+    char *staging = nullptr;  // pinned host memory for descriptor uploads
+    size_t staging_size = 0;
+    hipEvent_t staging_ev = nullptr;
+    hipStream_t cap = nullptr;  // capture stream
+    hipGraphExec_t gexec = nullptr;
+    std::string gkey;
+    std::map<std::string, int> tuned;  // nzc per geometry
+    trec::Prof prof;
+};
+
+namespace trec {
+
+trec_status fail(trec_context *ctx, trec_status s, const char *fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return s;
+}
+
+}  // namespace trec
+
+#define HIPCHK(ctx, x)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            return trec::fail((ctx), e_ == hipErrorOutOfMemory ? TREC_ERR_NOMEM : TREC_ERR_DEVICE, \
+                              "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+// ------------------------------------------------------------------ geometry
+
+namespace trec {
+
+// TranslateAngleToCoefficients + DecorateCoefficients, main-mpi.cpp:85-183,
+// linear terms: float trig of the float angle, double everything after.
+void coefficients(const trec_geom &g, const float *angles, double *coef)
+{
+    const double alpha = -(g.pitch * PI_180);
+    const double beta = g.offset * PI_180;
+    const double t = -g.zshift;
+    const double ca = cos(alpha), sa = sin(alpha), cb = cos(beta), sb = sin(beta);
+    for (int i = 0; i < g.nproj; i++) {
+        const float r = angles[i] * PI_180;
+        const double a[4] = {0.0, (double)cosf(r), 0.0, (double)(-sinf(r))};
+        const double b[4] = {0.0, 0.0, 1.0, 0.0};
+        double na[4], nb[4];
+        na[0] = a[0];
+        na[1] = (a[2] * sa * sb + a[3] * ca * sb + a[1] * cb);
+        na[2] = (a[2] * ca - a[3] * sa);
+        na[3] = (a[3] * ca * cb + a[2] * cb * sa - a[1] * sb);
+        nb[0] = b[0];
+        nb[1] = (b[2] * sa * sb + b[3] * ca * sb + b[1] * cb);
+        nb[2] = (b[2] * ca - b[3] * sa);
+        nb[3] = (b[3] * ca * cb + b[2] * cb * sa - b[1] * sb);
+        double *c = coef + 8 * (size_t)i;
+        c[0] = na[0] + na[3] * t;
+        c[1] = na[1];
+        c[2] = na[2];
+        c[3] = na[3];
+        c[4] = nb[0] + nb[3] * t;
+        c[5] = nb[1];
+        c[6] = nb[2];
+        c[7] = nb[3];
+    }
+}
+
+// Origins of the reference binaries: x = X/2, z = T/2; y = T/2 in TiltRec-mpi
+// (main-mpi.cpp:743-745, mrcvol.Y() is the thickness there) and Y/2 in the
+// CUDA and Z binaries (main-cuda.cpp:259-261).
+void origins(const trec_geom &g, double &ox, double &oy, double &oz)
+{
+    ox = (double)(float)(g.nx * .5);
+    oz = (double)(float)(g.thickness * .5);
+    oy = g.rules == TREC_RULES_MPI ? (double)(float)(g.thickness * .5) : (double)(float)(g.ny * .5);
+}
+
+// R-weighting as a circular convolution kernel of length N = X + X/10
+// (filter_prj.cpp:674-736): hk[d] = (1/X) sum_k W_k exp(2 pi i d k / N) with
+// W_0 = 0.2, W_k = k for k <= K = floor(0.05 N), K exp(-(k-K)^2 / (N/2)^2)
+// beyond, mirrored for the negative frequencies.
+std::vector<float> rweight_kernel(int X, int &N)
+{
+    N = X + X / 10;
+    const int K = (int)floorf((float)N * 0.05f);
+    const double sig = (double)((float)N * 0.5f);
+    const int nb = N / 2 + 1;
+    std::vector<double> W(nb);
+    for (int k = 0; k < nb; ++k)
+        W[k] = k == 0 ? 0.2 : (k <= K ? (double)k : (double)K * exp(-((double)(k - K) * (k - K)) / (sig * sig)));
+    std::vector<double> ct(N);
+    for (int m = 0; m < N; ++m) ct[m] = cos(2.0 * M_PI * m / N);
+    std::vector<float> hk(N);
+    for (int d = 0; d < N; ++d) {
+        double acc = W[0];
+        for (int k = 1; k <= (N - 1) / 2; ++k) acc += 2.0 * W[k] * ct[(int)(((long long)d * k) % N)];
+        if (N % 2 == 0) acc += W[N / 2] * ((d % 2) ? -1.0 : 1.0);
+        hk[d] = (float)(acc / X);
+    }
+    return hk;
+}
+
+}  // namespace trec
+
+// ------------------------------------------------------------------ plans
+
+namespace trec {
+
+bool is_iterative(int k) { return k == TREC_SIRT || k == TREC_SART; }
+
+trec_status validate(trec_context *ctx, const trec_geom *g, const trec_method *m, const float *angles)
+{
+    if (!g || !m || !angles) return fail(ctx, TREC_ERR_INVALID, "null geometry, method or angles");
+    if (g->nx < 2 || g->ny < 1 || g->nproj < 1 || g->thickness < 1)
+        return fail(ctx, TREC_ERR_INVALID, "bad sizes nx=%d ny=%d nproj=%d thickness=%d", g->nx, g->ny, g->nproj,
+                    g->thickness);
+    if (g->nproj > 32767 || g->nx > (1 << 20) || g->ny > (1 << 20) || g->thickness > (1 << 20))
+        return fail(ctx, TREC_ERR_INVALID, "sizes out of range");
+    if (!std::isfinite(g->offset) || !std::isfinite(g->pitch) || !std::isfinite(g->zshift))
+        return fail(ctx, TREC_ERR_INVALID, "non-finite geometry");
+    if (g->rules != TREC_RULES_MPI && g->rules != TREC_RULES_CUDA) return fail(ctx, TREC_ERR_INVALID, "bad rules");
+    if (m->method == TREC_FBP || m->method == TREC_ADMM)
+        return fail(ctx, TREC_ERR_UNSUPPORTED, "method %s is not provided", m->method == TREC_FBP ? "FBP" : "ADMM");
+    if (m->method != TREC_BPT && m->method != TREC_WBP && m->method != TREC_SIRT && m->method != TREC_SART)
+        return fail(ctx, TREC_ERR_INVALID, "unknown method %d", m->method);
+    if (is_iterative(m->method) && (m->iterations < 0 || m->iterations > 100000))
+        return fail(ctx, TREC_ERR_INVALID, "bad iteration count %d", m->iterations);
+    if (!std::isfinite(m->gamma)) return fail(ctx, TREC_ERR_INVALID, "non-finite gamma");
+    if (m->storage < TREC_STORE_F32 || m->storage > TREC_STORE_BF16) return fail(ctx, TREC_ERR_INVALID, "bad storage");
+    if (m->layout != TREC_LAYOUT_Y && m->layout != TREC_LAYOUT_Z) return fail(ctx, TREC_ERR_INVALID, "bad layout");
+    for (int i = 0; i < g->nproj; ++i)
+        if (!std::isfinite(angles[i])) return fail(ctx, TREC_ERR_INVALID, "non-finite tilt angle %d", i);
+    return TREC_OK;
+}
+
+int pick_nzc(trec_context *ctx, const trec_geom &g, int storage)
+{
+    if (const char *e = getenv("TREC_NZC")) {
+        int v = atoi(e);
+        if (v >= 1) return std::min(v, g.thickness);
+    }
+    const std::string key = tune_key(g.nx, g.thickness, g.nproj, storage);
+    if (ctx) {
+        auto it = ctx->tuned.find(key);
+        if (it != ctx->tuned.end()) return std::min(it->second, g.thickness);
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned_nzc.find(key);
+        if (it != g_tuned_nzc.end()) return std::min(it->second, g.thickness);
+    }
+    // Keep one z chunk of one row block (T/nzc * X * RB floats) near 2 MiB,
+    // so a chunk and its tilts stay in one XCD's L2.
+    const double slab = (double)g.thickness * g.nx * RB * 4.0;
+    int nzc = (int)std::ceil(slab / (2.0 * 1024 * 1024));
+    return std::max(1, std::min(nzc, g.thickness));
+}
+
+// Fills the host side of a plan (no device memory yet).
+trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const trec_method &m, const float *angles,
+                     const float *stack, const float *initial, float *volume)
+{
+    p.g = g;
+    p.m = m;
+    p.stack = stack;
+    p.initial = initial;
+    p.volume = volume;
+    if (m.method == TREC_BPT || m.method == TREC_WBP) p.m.storage = TREC_STORE_F32;
+    p.coef.resize(8 * (size_t)g.nproj);
+    coefficients(g, angles, p.coef.data());
+    double ox, oy, oz;
+    origins(g, ox, oy, oz);
+
+    // Row-blocked path: pitch 0 (b = (0,0,1,0), a2 = 0 exactly) and every
+    // tilt's a1 >= 0.2 (|angle + offset| <= ~78 degrees).
+    bool rows = !getenv("TREC_FORCE_GENERAL");
+    for (int i = 0; i < g.nproj && rows; ++i) {
+        const double *c = &p.coef[8 * (size_t)i];
+        if (!(c[2] == 0.0 && c[4] == 0.0 && c[5] == 0.0 && c[6] == 1.0 && c[7] == 0.0 && c[1] >= 0.2)) rows = false;
+    }
+    p.path = rows ? PATH_ROWS : PATH_GENERAL;
+
+    // numeric rules (see trec_rules)
+    float thr_upd = 0.f, thr_res = 0.f;
+    if (g.rules == TREC_RULES_CUDA) {
+        if (m.method == TREC_SIRT) thr_upd = 1e-5f;
+        if (m.method == TREC_BPT || m.method == TREC_WBP) thr_upd = 1e-5f;
+        if (m.method == TREC_SART) {
+            thr_upd = 1e-6f;
+            thr_res = 1e-7f;
+        }
+    }
+    const int iters = is_iterative(m.method) ? m.iterations : 0;
+    if (m.method == TREC_SIRT) p.stages = iters;
+    else if (m.method == TREC_SART) p.stages = iters * g.nproj;
+    else p.stages = 1;
+
+    if (p.path == PATH_ROWS) {
+        JobDesc &h = p.h;
+        memset(&h, 0, sizeof h);
+        h.X = g.nx;
+        h.Y = g.ny;
+        h.T = g.thickness;
+        h.P = g.nproj;
+        h.nyb = (g.ny + RB - 1) / RB;
+        h.nzc = 1;
+        if (m.method == TREC_SIRT) h.nzc = pick_nzc(ctx, g, p.m.storage);
+        h.zc_len = (g.thickness + h.nzc - 1) / h.nzc;
+        h.nzc = (g.thickness + h.zc_len - 1) / h.zc_len;
+        h.storage = p.m.storage;
+        h.gamma = m.gamma;
+        h.thr_upd = thr_upd;
+        h.thr_res = thr_res;
+        h.scale = 1.0f;
+        p.tc.resize(g.nproj);
+        for (int i = 0; i < g.nproj; ++i) {
+            const double *c = &p.coef[8 * (size_t)i];
+            TiltConst &t = p.tc[i];
+            t.a0 = c[0];
+            t.a1 = c[1];
+            t.a3 = c[3];
+            t.ox = ox;
+            t.oz = oz;
+            t.c0 = c[0] + ox - c[1] * ox - c[3] * oz;
+            t.inv_a1 = 1.0 / c[1];
+            t.slope = -c[3] / c[1];
+            t.a1f = (float)c[1];
+            t.ncand = (int)std::ceil(2.0 / c[1]);
+        }
+        if (m.method == TREC_WBP) p.hk_host = rweight_kernel(g.nx, p.N);
+    } else {
+        GeneralJob &q = p.gj;
+        q.X = g.nx;
+        q.Y = g.ny;
+        q.T = g.thickness;
+        q.P = g.nproj;
+        q.gamma = m.gamma;
+        q.thr_upd = thr_upd;
+        q.thr_res = thr_res;
+        q.layout = m.layout;
+        q.stack = stack;
+        q.vol = volume;
+        p.tg.resize(g.nproj);
+        for (int i = 0; i < g.nproj; ++i) {
+            const double *c = &p.coef[8 * (size_t)i];
+            TiltGeneral &t = p.tg[i];
+            for (int k = 0; k < 4; ++k) {
+                t.a[k] = c[k];
+                t.b[k] = c[4 + k];
+            }
+            t.ox = ox;
+            t.oy = oy;
+            t.oz = oz;
+        }
+        if (m.method == TREC_WBP) p.hk_host = rweight_kernel(g.nx, p.N);
+    }
+    return TREC_OK;
+}
+
+// Assigns device pointers from `base` (nullptr: sizing only); returns bytes.
+size_t carve(JobPlan &p, char *base)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char * {
+        char *r = base ? base + off : nullptr;
+        off += align_up(bytes);
+        return r;
+    };
+    const size_t X = p.g.nx, Y = p.g.ny, T = p.g.thickness, P = p.g.nproj;
+    const int k = p.m.method;
+    if (p.path == PATH_ROWS) {
+        JobDesc &h = p.h;
+        const size_t es = p.m.storage == TREC_STORE_F32 ? 4 : 2;
+        const size_t nyb = h.nyb;
+        p.d = (JobDesc *)take(sizeof(JobDesc));
+        p.d_ones = (JobDesc *)take(sizeof(JobDesc));
+        h.tc = (TiltConst *)take(sizeof(TiltConst) * P);
+        h.vol = (float *)take(4 * nyb * T * X * RB);
+        h.proj = take(es * nyb * P * X * RB);
+        h.res = is_iterative(k) ? take(es * nyb * P * X * RB) : nullptr;
+        h.part = (k == TREC_SIRT && h.nzc > 1) ? (float *)take(4 * (size_t)h.nzc * nyb * P * X * RB) : nullptr;
+        h.inv_rwt = is_iterative(k) ? (float *)take(4 * P * X) : nullptr;
+        h.inv_c = k == TREC_SIRT ? (float *)take(4 * T * X) : nullptr;
+        h.rwt = p.want_rwt ? (float *)take(4 * P * X) : nullptr;
+        p.sumsq = p.m.storage != TREC_STORE_F32 ? (double *)take(sizeof(double)) : nullptr;
+        p.hk = k == TREC_WBP ? (float *)take(4 * (size_t)p.N) : nullptr;
+    } else {
+        GeneralJob &q = p.gj;
+        p.d_tg = (TiltGeneral *)take(sizeof(TiltGeneral) * P);
+        q.tg = p.d_tg;
+        q.rval = is_iterative(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
+        q.rwt = is_iterative(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
+        p.stack_copy = k == TREC_WBP ? (float *)take(4 * P * Y * X) : nullptr;
+        p.hk = k == TREC_WBP ? (float *)take(4 * (size_t)p.N) : nullptr;
+    }
+    p.bytes = off;
+    return off;
+}
+
+trec_status ensure_arena(trec_context *ctx, size_t bytes)
+{
+    if (ctx->arena_size >= bytes) return TREC_OK;
+    HIPCHK(ctx, hipDeviceSynchronize());
+    if (ctx->gexec) {
+        (void)hipGraphExecDestroy(ctx->gexec);
+        ctx->gexec = nullptr;
+        ctx->gkey.clear();
+    }
+    if (ctx->arena) (void)hipFree(ctx->arena);
+    ctx->arena = nullptr;
+    ctx->arena_size = 0;
+    const size_t want = align_up(bytes + bytes / 16);
+    hipError_t e = hipMalloc((void **)&ctx->arena, want);
+    if (e != hipSuccess) {
+        ctx->arena = nullptr;
+        (void)hipGetLastError();
+        return fail(ctx, TREC_ERR_NOMEM, "device workspace of %zu bytes: %s", want, hipGetErrorString(e));
+    }
+    ctx->arena_size = want;
+    return TREC_OK;
+}
+
+// Copies the host-side descriptors of all plans to the device in one pass
+// through the pinned staging buffer.
+trec_status upload(trec_context *ctx, const std::vector<JobPlan *> &plans, hipStream_t st)
+{
+    struct Piece {
+        void *dst;
+        const void *src;
+        size_t n;
+    };
+    std::vector<Piece> pcs;
+    std::vector<JobDesc> ones(plans.size());
+    for (size_t i = 0; i < plans.size(); ++i) {
+        JobPlan &p = *plans[i];
+        if (p.path == PATH_ROWS) {
+            ones[i] = p.h;
+            ones[i].nyb = 1;
+            ones[i].nzc = 1;
+            ones[i].zc_len = p.h.T;
+            pcs.push_back({p.d, &p.h, sizeof(JobDesc)});
+            pcs.push_back({p.d_ones, &ones[i], sizeof(JobDesc)});
+            pcs.push_back({const_cast<TiltConst *>(p.h.tc), p.tc.data(), sizeof(TiltConst) * p.tc.size()});
+        } else {
+            pcs.push_back({p.d_tg, p.tg.data(), sizeof(TiltGeneral) * p.tg.size()});
+        }
+        if (p.hk) pcs.push_back({p.hk, p.hk_host.data(), sizeof(float) * p.hk_host.size()});
+    }
+    size_t total = 0;
+    for (auto &c : pcs) total += align_up(c.n);
+    HIPCHK(ctx, hipEventSynchronize(ctx->staging_ev));
+    if (ctx->staging_size < total) {
+        if (ctx->staging) (void)hipHostFree(ctx->staging);
+        ctx->staging = nullptr;
+        ctx->staging_size = 0;
+        HIPCHK(ctx, hipHostMalloc((void **)&ctx->staging, total, hipHostMallocDefault));
+        ctx->staging_size = total;
+    }
+    size_t off = 0;
+    for (auto &c : pcs) {
+        memcpy(ctx->staging + off, c.src, c.n);
+        HIPCHK(ctx, hipMemcpyAsync(c.dst, ctx->staging + off, c.n, hipMemcpyHostToDevice, st));
+        off += align_up(c.n);
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->staging_ev, st));
+    return TREC_OK;
+}
+
+// ------------------------------------------------------------------ issue
+
+#define LCHK(x)                                  \
+    do {                                         \
+        hipError_t e_ = (x);                     \
+        if (e_ != hipSuccess) return e_;         \
+    } while (0)
+
+hipError_t issue_prep(JobPlan &p, hipStream_t st)
+{
+    ProfScope ps(PROF_PREP, st);
+    const size_t X = p.g.nx, Y = p.g.ny, T = p.g.thickness, P = p.g.nproj;
+    const int k = p.m.method;
+    if (p.path == PATH_ROWS) {
+        JobDesc &h = p.h;
+        if (p.sumsq) {
+            LCHK(launch_zero((float *)p.sumsq, 2, st));
+            LCHK(launch_sumsq(p.stack, TREC_STORE_F32, P * Y * X, p.sumsq, st));
+            LCHK(launch_set_scale(p.d, p.sumsq, (double)(P * Y * X), st));
+        }
+        LCHK(launch_block_stack(h, p.d, p.stack, st));
+        if (p.initial) LCHK(launch_block_volume(h, p.d, p.initial, p.m.layout, st));
+        else LCHK(launch_zero(h.vol, (size_t)h.nyb * T * X * RB, st));
+        if (k == TREC_WBP)
+            LCHK(launch_rweight((float *)h.proj, p.hk, p.N, (int)X, (long long)h.nyb * P * RB, 1, RB, RB,
+                                (long long)X * RB, st));
+        if (is_iterative(k)) {
+            LaunchList L{};
+            L.n = 1;
+            L.off[0] = 0;
+            JobDesc o = h;
+            o.nyb = 1;
+            o.nzc = 1;
+            o.zc_len = (int)T;
+            L.off[1] = forward_blocks(o, (int)P);
+            L.job[0] = p.d_ones;
+            L.theta0[0] = 0;
+            L.ntheta[0] = 0;
+            LCHK(launch_forward(L, L.off[1], -1, false, st));
+        }
+        if (k == TREC_SIRT) LCHK(launch_weights(h, p.d, st));
+    } else {
+        GeneralJob &q = p.gj;
+        const size_t nv = X * Y * T;
+        if (p.initial && p.initial != p.volume) LCHK(launch_copy(p.volume, p.initial, nv, st));
+        else if (!p.initial) LCHK(launch_zero(p.volume, nv, st));
+        if (k == TREC_WBP) {
+            LCHK(launch_copy(p.stack_copy, p.stack, P * Y * X, st));
+            LCHK(launch_rweight(p.stack_copy, p.hk, p.N, (int)X, (long long)P * Y, 0, 1, 1, (long long)X, st));
+        }
+        if (is_iterative(k)) {
+            LCHK(launch_zero(q.rwt, P * Y * X, st));
+            LCHK(launch_general_forward(q, 0, (int)P, true, st));
+        }
+    }
+    return hipSuccess;
+}
+
+hipError_t issue_finish(JobPlan &p, hipStream_t st)
+{
+    ProfScope ps(PROF_FINISH, st);
+    if (p.path == PATH_ROWS) return launch_unblock_volume(p.h, p.d, p.volume, p.m.layout, st);
+    return hipSuccess;
+}
+
+// One stage of a general-path job (issued on its own).
+hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
+{
+    GeneralJob &q = p.gj;
+    const size_t X = p.g.nx, Y = p.g.ny, P = p.g.nproj;
+    switch (p.m.method) {
+    case TREC_SIRT:
+        LCHK(launch_zero(q.rval, P * Y * X, st));
+        LCHK(launch_general_forward(q, 0, (int)P, false, st));
+        LCHK(launch_general_residual(q, 0, (int)P, st));
+        return launch_general_back(q, BACK_SIRT, 0, (int)P, q.rval, st);
+    case TREC_SART: {
+        const int th = s % (int)P;
+        LCHK(launch_zero(q.rval + (size_t)th * Y * X, Y * X, st));
+        LCHK(launch_general_forward(q, th, 1, false, st));
+        LCHK(launch_general_residual(q, th, 1, st));
+        return launch_general_back(q, BACK_SART, th, 1, q.rval, st);
+    }
+    default:
+        return launch_general_back(q, BACK_BPT, 0, (int)P, p.m.method == TREC_WBP ? p.stack_copy : p.stack, st);
+    }
+}
+
+// Collects jobs into LaunchLists (at most MAX_JOBS each) and launches them.
+struct Batcher {
+    struct Item {
+        const JobDesc *d;
+        unsigned blocks;
+        short theta0, ntheta;
+    };
+    std::vector<Item> items;
+    void add(const JobDesc *d, unsigned blocks, int theta0, int ntheta)
+    {
+        if (blocks) items.push_back({d, blocks, (short)theta0, (short)ntheta});
+    }
+    template <typename F>
+    hipError_t flush(F launch)
+    {
+        size_t i = 0;
+        while (i < items.size()) {
+            LaunchList L{};
+            unsigned total = 0;
+            int n = 0;
+            while (i < items.size() && n < MAX_JOBS) {
+                L.off[n] = total;
+                L.job[n] = items[i].d;
+                L.theta0[n] = items[i].theta0;
+                L.ntheta[n] = items[i].ntheta;
+                total += items[i].blocks;
+                ++n;
+                ++i;
+            }
+            L.off[n] = total;
+            L.n = n;
+            LCHK(launch(L, total));
+        }
+        items.clear();
+        return hipSuccess;
+    }
+};
+
+hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &stage, hipStream_t st)
+{
+    Batcher fp_fused[3], fp_part, comb[3], back[3][3];
+    for (size_t j = 0; j < plans.size(); ++j) {
+        JobPlan &p = *plans[j];
+        const int s = stage[j];
+        if (s >= p.stages) continue;
+        stage[j] = s + 1;
+        if (p.path == PATH_GENERAL) {
+            LCHK(issue_general_stage(p, s, st));
+            continue;
+        }
+        const JobDesc &h = p.h;
+        const int sto = h.storage;
+        switch (p.m.method) {
+        case TREC_SIRT:
+            if (h.nzc == 1) {
+                fp_fused[sto].add(p.d, forward_blocks(h, h.P), 0, 0);
+            } else {
+                fp_part.add(p.d, forward_blocks(h, h.P), 0, 0);
+                comb[sto].add(p.d, combine_blocks(h, h.P), 0, 0);
+            }
+            back[BACK_SIRT][sto].add(p.d, back_blocks(h), 0, 0);
+            break;
+        case TREC_SART: {
+            const int th = s % h.P;
+            fp_fused[sto].add(p.d, forward_blocks(h, 1), th, 1);
+            back[BACK_SART][sto].add(p.d, back_blocks(h), th, 1);
+            break;
+        }
+        default:
+            back[BACK_BPT][sto].add(p.d, back_blocks(h), 0, 0);
+            break;
+        }
+    }
+    {
+        ProfScope ps(PROF_FORWARD, st);
+        for (int sto = 0; sto < 3; ++sto)
+            LCHK(fp_fused[sto].flush(
+                [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, true, st); }));
+        LCHK(fp_part.flush([&](const LaunchList &L, unsigned n) { return launch_forward(L, n, 0, false, st); }));
+    }
+    {
+        ProfScope ps(PROF_COMBINE, st);
+        for (int sto = 0; sto < 3; ++sto)
+            LCHK(comb[sto].flush([&](const LaunchList &L, unsigned n) { return launch_combine(L, n, sto, st); }));
+    }
+    {
+        ProfScope ps(PROF_BACK, st);
+        for (int mode = 0; mode < 3; ++mode)
+            for (int sto = 0; sto < 3; ++sto)
+                LCHK(back[mode][sto].flush(
+                    [&](const LaunchList &L, unsigned n) { return launch_back(L, n, sto, mode, st); }));
+    }
+    return hipSuccess;
+}
+
+hipError_t issue_all(const std::vector<JobPlan *> &plans, hipStream_t st)
+{
+    for (auto *p : plans) LCHK(issue_prep(*p, st));
+    std::vector<int> stage(plans.size(), 0);
+    int S = 0;
+    for (auto *p : plans) S = std::max(S, p->stages);
+    for (int s = 0; s < S; ++s) LCHK(issue_step(plans, stage, st));
+    for (auto *p : plans) LCHK(issue_finish(*p, st));
+    return hipSuccess;
+}
+
+std::string graph_key(trec_context *ctx, const std::vector<JobPlan *> &plans)
+{
+    std::string k;
+    char b[512];
+    snprintf(b, sizeof b, "%p|", (void *)ctx->arena);
+    k += b;
+    for (auto *pp : plans) {
+        const JobPlan &p = *pp;
+        snprintf(b, sizeof b, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%a,%p,%p,%p,%d,%d|", p.g.nx, p.g.ny, p.g.nproj,
+                 p.g.thickness, p.g.rules, p.m.method, p.m.iterations, p.m.storage, p.m.layout, p.path,
+                 (double)p.m.gamma, (const void *)p.stack, (const void *)p.initial, (void *)p.volume,
+                 p.path == PATH_ROWS ? p.h.nzc : 0, (int)p.want_rwt);
+        k += b;
+    }
+    return k;
+}
+
+// Allocates, uploads and issues a list of planned jobs on `st`.
+trec_status run_jobs(trec_context *ctx, std::vector<JobPlan> &plan_store, hipStream_t st, bool use_graph)
+{
+    std::vector<JobPlan *> plans;
+    for (auto &p : plan_store) plans.push_back(&p);
+    size_t total = 0;
+    for (auto *p : plans) total += carve(*p, nullptr);
+    trec_status s = ensure_arena(ctx, total);
+    if (s != TREC_OK) return s;
+    size_t off = 0;
+    for (auto *p : plans) {
+        carve(*p, ctx->arena + off);
+        off += p->bytes;
+    }
+    s = upload(ctx, plans, st);
+    if (s != TREC_OK) return s;
+    if (getenv("TREC_NO_GRAPH") || ctx->prof.on) use_graph = false;
+    if (!use_graph) {
+        tl_prof = &ctx->prof;
+        hipError_t e = issue_all(plans, st);
+        tl_prof = nullptr;
+        HIPCHK(ctx, e);
+        return TREC_OK;
+    }
+    const std::string key = graph_key(ctx, plans);
+    if (!ctx->gexec || key != ctx->gkey) {
+        if (ctx->gexec) {
+            HIPCHK(ctx, hipGraphExecDestroy(ctx->gexec));
+            ctx->gexec = nullptr;
+            ctx->gkey.clear();
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->cap, hipStreamCaptureModeThreadLocal));
+        hipError_t e = issue_all(plans, ctx->cap);
+        hipError_t e2 = hipStreamEndCapture(ctx->cap, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(ctx, TREC_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        e = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            ctx->gexec = nullptr;
+            return fail(ctx, TREC_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e));
+        }
+        ctx->gkey = key;
+    }
+    HIPCHK(ctx, hipGraphLaunch(ctx->gexec, st));
+    return TREC_OK;
+}
+
+size_t request_bytes(const trec_geom &g, const trec_method &m)
+{
+    std::vector<float> ang(std::max(g.nproj, 1), 0.f);
+    JobPlan p;
+    if (validate(nullptr, &g, &m, ang.data()) != TREC_OK) return 0;
+    if (plan_job(nullptr, p, g, m, ang.data(), nullptr, nullptr, nullptr) != TREC_OK) return 0;
+    const size_t X = g.nx, Y = g.ny, T = g.thickness, P = g.nproj;
+    return carve(p, nullptr) + align_up(4 * P * Y * X) + align_up(4 * X * Y * T);
+}
+
+}  // namespace trec
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char *trec_status_string(trec_status s)
+{
+    switch (s) {
+    case TREC_OK: return "ok";
+    case TREC_ERR_INVALID: return "invalid argument";
+    case TREC_ERR_UNSUPPORTED: return "unsupported";
+    case TREC_ERR_NOMEM: return "out of memory";
+    case TREC_ERR_DEVICE: return "device error";
+    case TREC_ERR_IO: return "i/o error";
+    case TREC_ERR_BUSY: return "busy";
+    case TREC_ERR_NOT_READY: return "not ready";
+    }
+    return "unknown status";
+}
+
+const char *trec_version(void) { return "tiltrec-hip 1.0 gfx950"; }
+
+trec_status trec_context_create(int device, trec_context **out)
+{
+    if (!out) return TREC_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return TREC_ERR_DEVICE;
+    }
+    if (device < 0 || device >= n) return TREC_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return TREC_ERR_DEVICE;
+    trec_context *c = new (std::nothrow) trec_context;
+    if (!c) return TREC_ERR_NOMEM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->staging_ev, hipEventDisableTiming) != hipSuccess) {
+        trec_context_destroy(c);
+        return TREC_ERR_DEVICE;
+    }
+    *out = c;
+    return TREC_OK;
+}
+
+void trec_context_destroy(trec_context *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    if (ctx->arena) (void)hipFree(ctx->arena);
+    if (ctx->staging) (void)hipHostFree(ctx->staging);
+    if (ctx->staging_ev) (void)hipEventDestroy(ctx->staging_ev);
+    if (ctx->cap) (void)hipStreamDestroy(ctx->cap);
+    for (auto &r : ctx->prof.pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ctx->prof.pool) (void)hipEventDestroy(e);
+    delete ctx;
+}
+
+trec_status trec_context_trim(trec_context *ctx)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipDeviceSynchronize());
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->gexec = nullptr;
+    ctx->gkey.clear();
+    if (ctx->arena) (void)hipFree(ctx->arena);
+    ctx->arena = nullptr;
+    ctx->arena_size = 0;
+    return TREC_OK;
+}
+
+const char *trec_last_error(const trec_context *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+trec_status trec_coefficients(const trec_geom *geom, const float *angles_deg, double *coef_out)
+{
+    if (!geom || !angles_deg || !coef_out || geom->nproj < 1) return TREC_ERR_INVALID;
+    coefficients(*geom, angles_deg, coef_out);
+    return TREC_OK;
+}
+
+trec_status trec_workspace_size(const trec_geom *geom, const trec_method *method, size_t *bytes)
+{
+    if (!geom || !method || !bytes) return TREC_ERR_INVALID;
+    std::vector<float> ang(std::max(geom->nproj, 1), 0.f);
+    trec_status s = validate(nullptr, geom, method, ang.data());
+    if (s != TREC_OK) return s;
+    JobPlan p;
+    s = plan_job(nullptr, p, *geom, *method, ang.data(), nullptr, nullptr, nullptr);
+    if (s != TREC_OK) return s;
+    *bytes = carve(p, nullptr);
+    return TREC_OK;
+}
+
+trec_status trec_reconstruct(trec_context *ctx, const trec_geom *geom, const float *angles_deg, const float *stack,
+                             const float *initial, const trec_method *method, float *volume, void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    trec_status s = validate(ctx, geom, method, angles_deg);
+    if (s != TREC_OK) return s;
+    if (!stack || !volume) return fail(ctx, TREC_ERR_INVALID, "null stack or volume");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<JobPlan> plans(1);
+    s = plan_job(ctx, plans[0], *geom, *method, angles_deg, stack, initial, volume);
+    if (s != TREC_OK) return s;
+    // a single job is issued directly unless it is made of many short stages
+    const bool graph = plans[0].stages > 64;
+    return run_jobs(ctx, plans, (hipStream_t)stream, graph);
+}
+
+trec_status trec_reconstruct_batch(trec_context *ctx, int njobs, const trec_job *jobs, void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    if (njobs < 0 || (njobs > 0 && !jobs)) return fail(ctx, TREC_ERR_INVALID, "bad job list");
+    if (njobs == 0) return TREC_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<JobPlan> plans(njobs);
+    for (int i = 0; i < njobs; ++i) {
+        const trec_job &j = jobs[i];
+        trec_status s = validate(ctx, &j.geom, &j.method, j.angles_deg);
+        if (s != TREC_OK) return s;
+        if (!j.stack || !j.volume) return fail(ctx, TREC_ERR_INVALID, "job %d: null stack or volume", i);
+        s = plan_job(ctx, plans[i], j.geom, j.method, j.angles_deg, j.stack, j.initial, j.volume);
+        if (s != TREC_OK) return s;
+    }
+    return run_jobs(ctx, plans, (hipStream_t)stream, true);
+}
+
+trec_status trec_filter_rweight(trec_context *ctx, float *rows, int nx, long long nrows, void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    if (!rows || nx < 2 || nrows < 0) return fail(ctx, TREC_ERR_INVALID, "bad filter arguments");
+    if (nrows == 0) return TREC_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int N = 0;
+    std::vector<float> hk = rweight_kernel(nx, N);
+    trec_status s = ensure_arena(ctx, align_up(sizeof(float) * N));
+    if (s != TREC_OK) return s;
+    HIPCHK(ctx, hipEventSynchronize(ctx->staging_ev));
+    if (ctx->staging_size < sizeof(float) * N) {
+        if (ctx->staging) (void)hipHostFree(ctx->staging);
+        ctx->staging = nullptr;
+        ctx->staging_size = 0;
+        HIPCHK(ctx, hipHostMalloc((void **)&ctx->staging, sizeof(float) * N, hipHostMallocDefault));
+        ctx->staging_size = sizeof(float) * N;
+    }
+    memcpy(ctx->staging, hk.data(), sizeof(float) * N);
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->arena, ctx->staging, sizeof(float) * N, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipEventRecord(ctx->staging_ev, st));
+    HIPCHK(ctx, launch_rweight(rows, (const float *)ctx->arena, N, nx, nrows, 0, 1, 1, nx, st));
+    return TREC_OK;
+}
+
+trec_status trec_forward_project(trec_context *ctx, const trec_geom *geom, const float *angles_deg,
+                                 const float *volume, float *rval, float *rwt, void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    trec_method m{TREC_SIRT, 0, 1.0f, TREC_STORE_F32, TREC_LAYOUT_Y};
+    trec_status s = validate(ctx, geom, &m, angles_deg);
+    if (s != TREC_OK) return s;
+    if (!volume || !rval) return fail(ctx, TREC_ERR_INVALID, "null volume or rval");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<JobPlan> plans(1);
+    JobPlan &p = plans[0];
+    p.want_rwt = true;
+    s = plan_job(ctx, p, *geom, m, angles_deg, volume, nullptr, nullptr);
+    if (s != TREC_OK) return s;
+    const size_t X = geom->nx, Y = geom->ny, T = geom->thickness, P = geom->nproj;
+    if (p.path == PATH_ROWS) {
+        p.h.nzc = 1;
+        p.h.zc_len = (int)T;
+        p.m.method = TREC_SIRT;
+        // the partial buffer of one z chunk holds the raw projection
+        size_t bytes = carve(p, nullptr) + align_up(4 * (size_t)p.h.nyb * P * X * RB);
+        s = ensure_arena(ctx, bytes);
+        if (s != TREC_OK) return s;
+        carve(p, ctx->arena);
+        p.h.part = (float *)(ctx->arena + p.bytes);
+        s = upload(ctx, {&p}, st);
+        if (s != TREC_OK) return s;
+        JobDesc &h = p.h;
+        HIPCHK(ctx, launch_block_volume(h, p.d, volume, TREC_LAYOUT_Y, st));
+        LaunchList L{};
+        L.n = 1;
+        L.job[0] = p.d;
+        L.off[0] = 0;
+        L.off[1] = forward_blocks(h, (int)P);
+        HIPCHK(ctx, launch_forward(L, L.off[1], TREC_STORE_F32, false, st));
+        HIPCHK(ctx, launch_unblock_proj(h.part, (int)P, (int)X, (int)Y, h.nyb, rval, st));
+        if (rwt) {
+            JobDesc o = h;
+            o.nyb = 1;
+            L.job[0] = p.d_ones;
+            L.off[1] = forward_blocks(o, (int)P);
+            HIPCHK(ctx, launch_forward(L, L.off[1], -1, false, st));
+            HIPCHK(ctx, launch_broadcast_rows(h.rwt, (int)P, (int)X, (int)Y, rwt, st));
+        }
+        return TREC_OK;
+    }
+    s = ensure_arena(ctx, carve(p, nullptr));
+    if (s != TREC_OK) return s;
+    carve(p, ctx->arena);
+    s = upload(ctx, {&p}, st);
+    if (s != TREC_OK) return s;
+    GeneralJob q = p.gj;
+    q.vol = const_cast<float *>(volume);
+    q.layout = TREC_LAYOUT_Y;
+    q.rval = rval;
+    HIPCHK(ctx, launch_zero(rval, P * Y * X, st));
+    HIPCHK(ctx, launch_general_forward(q, 0, (int)P, false, st));
+    if (rwt) {
+        q.rwt = rwt;
+        HIPCHK(ctx, launch_zero(rwt, P * Y * X, st));
+        HIPCHK(ctx, launch_general_forward(q, 0, (int)P, true, st));
+    }
+    return TREC_OK;
+}
+
+trec_status trec_backproject(trec_context *ctx, const trec_geom *geom, const float *angles_deg, const float *stack,
+                             float *volume, void *stream)
+{
+    trec_method m{TREC_BPT, 0, 1.0f, TREC_STORE_F32, TREC_LAYOUT_Y};
+    return trec_reconstruct(ctx, geom, angles_deg, stack, volume, &m, volume, stream);
+}
+
+trec_status trec_sirt_iterate(trec_context *ctx, const trec_geom *geom, const float *angles_deg, const float *stack,
+                              float gamma, float *volume, void *stream)
+{
+    trec_method m{TREC_SIRT, 1, gamma, TREC_STORE_F32, TREC_LAYOUT_Y};
+    return trec_reconstruct(ctx, geom, angles_deg, stack, volume, &m, volume, stream);
+}
+
+trec_status trec_request_footprint(const trec_geom *geom, const trec_method *method, size_t *bytes)
+{
+    if (!geom || !method || !bytes) return TREC_ERR_INVALID;
+    std::vector<float> ang(std::max(geom->nproj, 1), 0.f);
+    trec_status s = validate(nullptr, geom, method, ang.data());
+    if (s != TREC_OK) return s;
+    *bytes = request_bytes(*geom, *method);
+    return TREC_OK;
+}
+
+trec_status trec_profile_enable(trec_context *ctx, int on)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    for (auto &r : ctx->prof.pending) {
+        (void)hipEventSynchronize(r.b);
+        ctx->prof.pool.push_back(r.a);
+        ctx->prof.pool.push_back(r.b);
+    }
+    ctx->prof.pending.clear();
+    for (int k = 0; k < PROF_N; ++k) {
+        ctx->prof.total_ms[k] = 0;
+        ctx->prof.count[k] = 0;
+    }
+    ctx->prof.on = on != 0;
+    return TREC_OK;
+}
+
+trec_status trec_profile_read(trec_context *ctx, const char *kernel, double *total_ms, long long *launches)
+{
+    if (!ctx || !kernel) return TREC_ERR_INVALID;
+    int k = 0;
+    while (k < PROF_N && strcmp(kernel, prof_names[k])) ++k;
+    if (k == PROF_N) return fail(ctx, TREC_ERR_INVALID, "unknown kernel name %s", kernel);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    for (auto &r : ctx->prof.pending) {
+        HIPCHK(ctx, hipEventSynchronize(r.b));
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, r.a, r.b));
+        ctx->prof.total_ms[r.kind] += ms;
+        ctx->prof.count[r.kind] += 1;
+        ctx->prof.pool.push_back(r.a);
+        ctx->prof.pool.push_back(r.b);
+    }
+    ctx->prof.pending.clear();
+    if (total_ms) *total_ms = ctx->prof.total_ms[k];
+    if (launches) *launches = ctx->prof.count[k];
+    return TREC_OK;
+}
+
+trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_method *method)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    if (!geom || !method) return fail(ctx, TREC_ERR_INVALID, "null geometry or method");
+    std::vector<float> ang(std::max(geom->nproj, 1));
+    for (int i = 0; i < geom->nproj; ++i) ang[i] = geom->nproj > 1 ? -60.f + 120.f * i / (geom->nproj - 1) : 0.f;
+    trec_status s = validate(ctx, geom, method, ang.data());
+    if (s != TREC_OK) return s;
+    if (method->method != TREC_SIRT) return TREC_OK;  // only SIRT has a tunable z split
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t X = geom->nx, Y = geom->ny, T = geom->thickness, P = geom->nproj;
+    float *stack = nullptr, *vol = nullptr;
+    HIPCHK(ctx, hipMalloc((void **)&stack, 4 * P * Y * X));
+    hipError_t e = hipMalloc((void **)&vol, 4 * X * Y * T);
+    if (e != hipSuccess) {
+        (void)hipFree(stack);
+        return fail(ctx, TREC_ERR_NOMEM, "autotune buffers: %s", hipGetErrorString(e));
+    }
+    hipStream_t st = ctx->cap;
+    if (launch_zero(stack, P * Y * X, st) != hipSuccess || launch_zero(vol, X * Y * T, st) != hipSuccess) {
+        (void)hipFree(stack);
+        (void)hipFree(vol);
+        return fail(ctx, TREC_ERR_DEVICE, "autotune: zero fill failed");
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    int best = 1;
+    float best_ms = 1e30f;
+    const int cands[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
+    trec_status res = TREC_OK;
+    for (int nzc : cands) {
+        if (nzc > (int)T) break;
+        JobPlan p;
+        trec_method m = *method;
+        m.iterations = 3;
+        if ((res = plan_job(ctx, p, *geom, m, ang.data(), stack, nullptr, vol)) != TREC_OK) break;
+        if (p.path != PATH_ROWS) break;
+        p.h.zc_len = (int)((T + nzc - 1) / nzc);
+        p.h.nzc = (int)((T + p.h.zc_len - 1) / p.h.zc_len);
+        if ((res = ensure_arena(ctx, carve(p, nullptr))) != TREC_OK) break;
+        carve(p, ctx->arena);
+        if ((res = upload(ctx, {&p}, st)) != TREC_OK) break;
+        std::vector<JobPlan *> pl{&p};
+        std::vector<int> stage{0};
+        if (issue_prep(p, st) != hipSuccess || issue_step(pl, stage, st) != hipSuccess) {
+            res = fail(ctx, TREC_ERR_DEVICE, "autotune launch failed");
+            break;
+        }
+        if (hipEventRecord(e0, st) != hipSuccess || issue_step(pl, stage, st) != hipSuccess ||
+            issue_step(pl, stage, st) != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess) {
+            res = fail(ctx, TREC_ERR_DEVICE, "autotune run failed");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = p.h.nzc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(stack);
+    (void)hipFree(vol);
+    if (res != TREC_OK) return res;
+    const std::string key = tune_key(geom->nx, geom->thickness, geom->nproj, method->storage);
+    ctx->tuned[key] = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned_nzc[key] = best;
+    }
+    return TREC_OK;
+}
+
+}  // extern "C"
+

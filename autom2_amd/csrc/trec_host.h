@@ -1,0 +1,69 @@
+// trec_host.h -- host-side job plans shared by api.cpp and server.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "trec_internal.h"
+
+struct trec_context;
+
+namespace trec {
+
+// One reconstruction: its arguments, the per-tilt constants built on the
+// host, and (once carved) the device buffers it works in.
+struct JobPlan {
+    trec_geom g{};
+    trec_method m{};
+    const float *stack = nullptr;
+    const float *initial = nullptr;
+    float *volume = nullptr;
+    int path = PATH_ROWS;
+    int stages = 0;         // SIRT: iterations; SART: iterations * tilts; BPT/WBP: 1
+    bool want_rwt = false;  // keep the raw ray weights (forward-projection API)
+    std::vector<double> coef;
+    // row-blocked path
+    JobDesc h{};
+    JobDesc *d = nullptr;
+    JobDesc *d_ones = nullptr;
+    std::vector<TiltConst> tc;
+    double *sumsq = nullptr;
+    float *hk = nullptr;
+    int N = 0;
+    std::vector<float> hk_host;
+    // general path
+    GeneralJob gj{};
+    std::vector<TiltGeneral> tg;
+    TiltGeneral *d_tg = nullptr;
+    float *stack_copy = nullptr;
+    size_t bytes = 0;
+};
+
+trec_status fail(trec_context *ctx, trec_status s, const char *fmt, ...);
+
+// Coefficients of main-mpi.cpp:85-183 (coef: 8 per tilt).
+void coefficients(const trec_geom &g, const float *angles, double *coef);
+// Circulant R-weighting kernel of length N = X + X/10.
+std::vector<float> rweight_kernel(int X, int &N);
+
+trec_status validate(trec_context *ctx, const trec_geom *g, const trec_method *m, const float *angles);
+trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const trec_method &m, const float *angles,
+                     const float *stack, const float *initial, float *volume);
+// Assigns the plan's device buffers from `base` (nullptr: sizing only).
+size_t carve(JobPlan &p, char *base);
+// Copies the plans' descriptors to the device through pinned staging.
+trec_status upload(trec_context *ctx, const std::vector<JobPlan *> &plans, hipStream_t st);
+
+hipError_t issue_prep(JobPlan &p, hipStream_t st);
+// Advances every job whose stage[j] < stages by one stage, merging the jobs'
+// kernels into multi-job launches; increments stage[j].
+hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &stage, hipStream_t st);
+hipError_t issue_finish(JobPlan &p, hipStream_t st);
+
+// Bytes a request holds on the device in the serving loop: workspace plus
+// its stack and volume.
+size_t request_bytes(const trec_geom &g, const trec_method &m);
+
+}  // namespace trec

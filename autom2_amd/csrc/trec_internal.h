@@ -1,0 +1,139 @@
+// trec_internal.h -- types shared by the HIP kernels and the host side of
+// libtiltrec_hip.so.  Not part of the public ABI (include/tiltrec.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tiltrec.h"
+
+namespace trec {
+
+// Rows of the volume that one thread carries.  At pitch 0 every row y is an
+// independent 2-D problem fed by row y of every projection (the detector y
+// coordinate equals y exactly), so a thread processes RB rows with one
+// evaluation of the geometry.  Volumes and projections are stored row-blocked:
+//   volume  [nyb][T][X][RB]      (nyb = ceil(Y / RB))
+//   stack   [nyb][P][X][RB]
+// so the RB rows of one (z, x) voxel column or one (tilt, pixel) are 64
+// contiguous bytes in float.
+constexpr int RB = 16;
+
+// Threads per work-group of the row-blocked kernels.
+constexpr int WG = 256;
+// Pixels per forward-projection work-group, voxels per back-projection wave.
+constexpr int FP_PIX = 256;
+constexpr int BP_X = 64;
+constexpr int BP_Z = WG / BP_X;
+
+// Largest number of jobs one multi-job launch carries.
+constexpr int MAX_JOBS = 64;
+
+// Per-tilt constants.  a0, a1, a3 are TiltRec-mpi's double coefficients
+// (main-mpi.cpp:85-183); the fast path needs a2 = 0 and b = (0, 0, 1, 0),
+// which holds exactly whenever the pitch is 0.  Voxel (x, z) maps to the
+// detector column
+//   u = ((a0 + a1 * (x - ox)) + a3 * (z - oz)) + ox      (ValCoef's order)
+// and, for the forward projection, u(x, z) = c0 + a1 * x + a3 * z.
+struct TiltConst {
+    double a0, a1, a3;
+    double ox, oz;
+    double c0;      // a0 + ox - a1 * ox - a3 * oz
+    double inv_a1;  // 1 / a1
+    double slope;   // -a3 / a1: shift of a pixel's voxel window per z step
+    float a1f;      // a1 rounded to float
+    int ncand;      // voxels a pixel can see in one z layer: ceil(2 / a1)
+};
+
+// General geometry (any pitch): u = a0 + a1 X + a2 Y + a3 Z + ox,
+// v = b0 + b1 X + b2 Y + b3 Z + oy with X = x - ox etc.
+struct TiltGeneral {
+    double a[4];
+    double b[4];
+    double ox, oy, oz;
+};
+
+enum PathKind { PATH_ROWS = 0, PATH_GENERAL = 1 };
+
+// One reconstruction job as the kernels see it (device memory).
+struct JobDesc {
+    int X, Y, T, P;
+    int nyb;       // row blocks
+    int nzc;       // z chunks of the forward projection (1 = fused residual)
+    int zc_len;    // layers per z chunk
+    int storage;   // trec_storage of proj / res
+    float gamma;
+    float thr_upd;  // SIRT: C > thr_upd; BPT/SART: c > thr_upd
+    float thr_res;  // residual: rwt > thr_res (0: rwt != 0)
+    float scale;    // FP16 form: stack divided by scale on the way in
+    const TiltConst *tc;     // [P]
+    float *vol;              // [nyb][T][X][RB]
+    void *proj;              // [nyb][P][X][RB] of the storage type
+    void *res;               // [nyb][P][X][RB] of the storage type
+    float *part;             // [nzc][nyb][P][X][RB] when nzc > 1
+    float *inv_rwt;          // [P][X]
+    float *inv_c;            // [T][X]
+    float *rwt;              // [P][X] raw ray weights, or NULL
+};
+
+// A multi-job launch: jobs[k] owns work-groups [off[k], off[k+1]).
+struct LaunchList {
+    int n;
+    unsigned off[MAX_JOBS + 1];
+    const JobDesc *job[MAX_JOBS];
+    short theta0[MAX_JOBS];  // first tilt of the job in this launch
+    short ntheta[MAX_JOBS];  // tilts of the job in this launch, 0 = all
+};
+
+// ---------------------------------------------------------------- launchers
+// (kernels.hip).  All enqueue on `st` and return the launch error.
+hipError_t launch_block_stack(const JobDesc &h, const JobDesc *d, const float *stack, hipStream_t st);
+hipError_t launch_block_volume(const JobDesc &h, const JobDesc *d, const float *src, int layout, hipStream_t st);
+hipError_t launch_zero(float *p, size_t n, hipStream_t st);
+hipError_t launch_unblock_volume(const JobDesc &h, const JobDesc *d, float *dst, int layout, hipStream_t st);
+hipError_t launch_weights(const JobDesc &h, const JobDesc *d, hipStream_t st);
+hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st);
+hipError_t launch_combine(const LaunchList &L, unsigned nblk, int storage, hipStream_t st);
+hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode, hipStream_t st);
+hipError_t launch_rweight(float *data, const float *hk, int N, int X, long long nrows,
+                          long long row_stride, int x_stride, int rows_per_group,
+                          long long group_stride, hipStream_t st);
+// out += sum of squares (in double) of n values of the storage type.
+hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipStream_t st);
+
+// Work-group counts of the multi-job kernels for one job.
+unsigned forward_blocks(const JobDesc &h, int ntheta);
+unsigned combine_blocks(const JobDesc &h, int ntheta);
+unsigned back_blocks(const JobDesc &h);
+
+// Back-projection modes.
+enum BackMode {
+    BACK_SIRT = 0,  // val += s over all tilts, then vol += gamma * val / C
+    BACK_BPT = 1,   // vol += s / c per tilt (c above the threshold)
+    BACK_SART = 2   // one tilt: vol += gamma * s / c
+};
+
+// General-geometry kernels (natural layouts, any pitch).
+struct GeneralJob {
+    int X, Y, T, P;
+    float gamma, thr_upd, thr_res;
+    int layout;             // trec_layout of vol
+    const TiltGeneral *tg;  // [P]
+    const float *stack;     // [P][Y][X]
+    float *vol;             // layout Y [Y][T][X] or Z [T][Y][X]
+    float *rval;            // [P][Y][X]
+    float *rwt;             // [P][Y][X]
+};
+hipError_t launch_general_forward(const GeneralJob &g, int theta0, int ntheta, bool ones, hipStream_t st);
+hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, hipStream_t st);
+hipError_t launch_general_back(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img,
+                               hipStream_t st);
+
+// Small helpers.
+hipError_t launch_set_scale(JobDesc *d, const double *sumsq, double n, hipStream_t st);
+hipError_t launch_unblock_proj(const float *src, int P, int X, int Y, int nyb, float *dst, hipStream_t st);
+hipError_t launch_broadcast_rows(const float *src, int P, int X, int Y, float *dst, hipStream_t st);
+hipError_t launch_copy(float *dst, const float *src, size_t n, hipStream_t st);
+
+}  // namespace trec

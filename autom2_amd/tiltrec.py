@@ -1,0 +1,293 @@
+"""Python binding of libtiltrec_hip.so (include/tiltrec.h) over ctypes.
+
+Mirrors TiltRec's own interface for the reconstruction path: the
+``--method NAME,iterations,gamma`` and ``--geometry offset,pitch,zshift,
+thickness`` option strings (src/TiltRec/src/opts/opts.cpp:232-335), the
+method names BPT/FBP/WBP/SIRT/SART/ADMM, and the two volume orientations
+(TiltRec-* writes [Y][T][X], TiltRecZ-* writes [T][Y][X]).
+
+Device memory and streams come from torch tensors (``tensor.data_ptr()`` and
+``torch.cuda.current_stream().cuda_stream``); the arithmetic is all in the
+HIP library.  If the library is missing the import of this module fails: there
+is no CPU fallback.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libtiltrec_hip.so")
+
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_NOMEM, ERR_DEVICE, ERR_IO, ERR_BUSY, ERR_NOT_READY = range(8)
+BPT, FBP, WBP, SIRT, SART, ADMM = range(6)
+STORE_F32, STORE_F16, STORE_BF16 = range(3)
+LAYOUT_Y, LAYOUT_Z = range(2)
+RULES_MPI, RULES_CUDA = range(2)
+
+METHOD_NAMES = {"BPT": BPT, "FBP": FBP, "WBP": WBP, "SIRT": SIRT, "SART": SART, "ADMM": ADMM}
+STORAGE_NAMES = {"f32": STORE_F32, "fp32": STORE_F32, "f16": STORE_F16, "fp16": STORE_F16,
+                 "bf16": STORE_BF16}
+
+
+class TrecError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+class Geom(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nproj", ctypes.c_int),
+                ("thickness", ctypes.c_int), ("offset", ctypes.c_float), ("pitch", ctypes.c_float),
+                ("zshift", ctypes.c_float), ("rules", ctypes.c_int)]
+
+
+class Method(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("iterations", ctypes.c_int), ("gamma", ctypes.c_float),
+                ("storage", ctypes.c_int), ("layout", ctypes.c_int)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("geom", Geom), ("method", Method), ("angles_deg", ctypes.POINTER(ctypes.c_float)),
+                ("stack", ctypes.c_void_p), ("initial", ctypes.c_void_p), ("volume", ctypes.c_void_p)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("geom", Geom), ("method", Method), ("angles_deg", ctypes.POINTER(ctypes.c_float)),
+                ("stack", ctypes.c_void_p), ("volume_out", ctypes.c_void_p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("footprint", ctypes.c_size_t)]
+
+
+# (name, restype, argtypes) of every function include/tiltrec.h declares
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_PF = ctypes.POINTER(ctypes.c_float)
+SIGNATURES = [
+    ("trec_context_create", _I, [_I, ctypes.POINTER(_P)]),
+    ("trec_context_destroy", None, [_P]),
+    ("trec_context_trim", _I, [_P]),
+    ("trec_status_string", ctypes.c_char_p, [_I]),
+    ("trec_last_error", ctypes.c_char_p, [_P]),
+    ("trec_version", ctypes.c_char_p, []),
+    ("trec_coefficients", _I, [ctypes.POINTER(Geom), _PF, ctypes.POINTER(ctypes.c_double)]),
+    ("trec_workspace_size", _I, [ctypes.POINTER(Geom), ctypes.POINTER(Method), ctypes.POINTER(ctypes.c_size_t)]),
+    ("trec_reconstruct", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, ctypes.POINTER(Method), _P, _P]),
+    ("trec_filter_rweight", _I, [_P, _P, _I, ctypes.c_longlong, _P]),
+    ("trec_forward_project", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, _P, _P]),
+    ("trec_backproject", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, _P]),
+    ("trec_sirt_iterate", _I, [_P, ctypes.POINTER(Geom), _PF, _P, ctypes.c_float, _P, _P]),
+    ("trec_reconstruct_batch", _I, [_P, _I, ctypes.POINTER(Job), _P]),
+    ("trec_autotune", _I, [_P, ctypes.POINTER(Geom), ctypes.POINTER(Method)]),
+    ("trec_profile_enable", _I, [_P, _I]),
+    ("trec_profile_read", _I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
+    ("trec_server_create", _I, [_I, ctypes.c_size_t, _I, ctypes.POINTER(_P)]),
+    ("trec_server_submit", _I, [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    ("trec_server_poll", _I, [_P, ctypes.c_uint64, ctypes.POINTER(_I)]),
+    ("trec_server_wait", _I, [_P, ctypes.c_uint64, ctypes.POINTER(Result)]),
+    ("trec_request_footprint", _I, [ctypes.POINTER(Geom), ctypes.POINTER(Method), ctypes.POINTER(ctypes.c_size_t)]),
+    ("trec_server_destroy", None, [_P]),
+]
+
+_lib = None
+
+
+def lib():
+    """Loads libtiltrec_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                              "or `make -C autom2_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+# ------------------------------------------------------------------ options
+
+def parse_method(text):
+    """--method value -> (kind, iterations, gamma), as opts.cpp:287-335 reads it.
+
+    "SIRT,20,1.0" and "SART,5,0.5" carry an iteration count and a relaxation;
+    "BPT", "FBP" and "WBP" carry nothing; "ADMM,n,cg,gamma,soft" is parsed and
+    refused later (not provided).  Unlike the reference, a bare "SIRT" is an
+    error instead of reading uninitialised values (opts.cpp:112-124).
+    """
+    parts = text.split(",")
+    name = parts[0]
+    if name not in METHOD_NAMES:
+        raise ValueError(f"unknown method {name!r}")
+    kind = METHOD_NAMES[name]
+    if kind in (SIRT, SART):
+        if len(parts) != 3:
+            raise ValueError(f"{name} needs NAME,iterations,gamma")
+        return kind, int(parts[1]), float(parts[2])
+    if kind == ADMM:
+        if len(parts) != 5:
+            raise ValueError("ADMM needs ADMM,iterations,cgiter,gamma,soft")
+        return kind, int(parts[1]), float(parts[3])
+    if len(parts) != 1:
+        raise ValueError(f"{name} takes no parameters")
+    return kind, 0, 1.0
+
+
+def parse_geometry(text):
+    """--geometry offset,pitch,zshift,thickness (opts.cpp:232-260)."""
+    parts = text.split(",")
+    if len(parts) != 4:
+        raise ValueError("geometry needs offset,pitch,zshift,thickness")
+    return float(parts[0]), float(parts[1]), float(parts[2]), int(parts[3])
+
+
+def make_geom(nx, ny, nproj, thickness, offset=0.0, pitch=0.0, zshift=0.0, rules=RULES_MPI):
+    return Geom(nx, ny, nproj, thickness, offset, pitch, zshift, rules)
+
+
+def make_method(kind, iterations=0, gamma=1.0, storage=STORE_F32, layout=LAYOUT_Y):
+    if isinstance(kind, str):
+        kind, iterations, gamma = parse_method(kind)
+    if isinstance(storage, str):
+        storage = STORAGE_NAMES[storage]
+    return Method(kind, iterations, gamma, storage, layout)
+
+
+def _angles(angles):
+    a = (ctypes.c_float * len(angles))(*[float(v) for v in angles])
+    return ctypes.cast(a, _PF), a
+
+
+def _check(status, ctx=None):
+    if status != OK:
+        msg = lib().trec_status_string(status).decode()
+        if ctx:
+            detail = lib().trec_last_error(ctx).decode()
+            if detail:
+                msg = f"{msg}: {detail}"
+        raise TrecError(status, msg)
+
+
+def coefficients(geom, angles):
+    import numpy as np
+    out = np.zeros((len(angles), 8), np.float64)
+    pa, _keep = _angles(angles)
+    _check(lib().trec_coefficients(ctypes.byref(geom), pa,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return out
+
+
+def workspace_size(geom, method):
+    n = ctypes.c_size_t()
+    _check(lib().trec_workspace_size(ctypes.byref(geom), ctypes.byref(method), ctypes.byref(n)))
+    return n.value
+
+
+def request_footprint(geom, method):
+    n = ctypes.c_size_t()
+    _check(lib().trec_request_footprint(ctypes.byref(geom), ctypes.byref(method), ctypes.byref(n)))
+    return n.value
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+@dataclass
+class Series:
+    """One tilt series for a batch: device tensors and its options."""
+    geom: Geom
+    method: Method
+    angles: list
+    stack: object
+    volume: object
+    initial: object = None
+
+
+class Context:
+    """A device context: cached workspace, cached HIP graph, autotune table."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().trec_context_create(device, ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().trec_context_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def reconstruct(self, geom, angles, stack, method, volume, initial=None, stream=None):
+        pa, _keep = _angles(angles)
+        _check(lib().trec_reconstruct(self._h, ctypes.byref(geom), pa, _ptr(stack), _ptr(initial),
+                                      ctypes.byref(method), _ptr(volume), _stream(stream)), self._h)
+
+    def reconstruct_batch(self, series, stream=None):
+        jobs = (Job * len(series))()
+        keep = []
+        for j, s in zip(jobs, series):
+            pa, k = _angles(s.angles)
+            keep.append(k)
+            j.geom = s.geom
+            j.method = s.method
+            j.angles_deg = pa
+            j.stack = s.stack.data_ptr()
+            j.initial = s.initial.data_ptr() if s.initial is not None else None
+            j.volume = s.volume.data_ptr()
+        _check(lib().trec_reconstruct_batch(self._h, len(series), jobs, _stream(stream)), self._h)
+
+    def filter_rweight(self, rows, stream=None):
+        nx = rows.shape[-1]
+        _check(lib().trec_filter_rweight(self._h, _ptr(rows), nx, rows.numel() // nx, _stream(stream)), self._h)
+
+    def forward_project(self, geom, angles, volume, rval, rwt=None, stream=None):
+        pa, _keep = _angles(angles)
+        _check(lib().trec_forward_project(self._h, ctypes.byref(geom), pa, _ptr(volume), _ptr(rval), _ptr(rwt),
+                                          _stream(stream)), self._h)
+
+    def backproject(self, geom, angles, stack, volume, stream=None):
+        pa, _keep = _angles(angles)
+        _check(lib().trec_backproject(self._h, ctypes.byref(geom), pa, _ptr(stack), _ptr(volume),
+                                      _stream(stream)), self._h)
+
+    def sirt_iterate(self, geom, angles, stack, gamma, volume, stream=None):
+        pa, _keep = _angles(angles)
+        _check(lib().trec_sirt_iterate(self._h, ctypes.byref(geom), pa, _ptr(stack), gamma, _ptr(volume),
+                                       _stream(stream)), self._h)
+
+    def autotune(self, geom, method):
+        _check(lib().trec_autotune(self._h, ctypes.byref(geom), ctypes.byref(method)), self._h)
+
+    def profile(self, on=True):
+        _check(lib().trec_profile_enable(self._h, 1 if on else 0), self._h)
+
+    def profile_read(self, kernel):
+        """(total milliseconds, launch groups) of one kernel since profile()."""
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        _check(lib().trec_profile_read(self._h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)), self._h)
+        return ms.value, n.value
+
+    def trim(self):
+        _check(lib().trec_context_trim(self._h), self._h)

@@ -1,0 +1,188 @@
+"""Benchmark of the TiltRec SIRT path on one MI355X.
+
+Workload (one step): SIRT x20, gamma 1.0, of a 61-tilt series of
+1024 x 1024 float32 projections (-60..+60 degrees in 2 degree steps, the
+ellipsoid-and-bead phantom scaled x4, noise and per-tilt shifts/rotations)
+into a 1024 x 1024 x 256 volume, TiltRec-mpi numeric rules, plain FP32.
+The stack is resident in HBM before timing starts; a step runs the whole
+reconstruction (layout transposes, ray weights, 20 iterations, output).
+
+Metric: voxel-projection updates/s = X * Y * T * P * iterations / seconds
+per step, with p50/p99 step latency.  The roofline of the dominant kernel
+is measured live with HIP events on its own stream; the CPU baseline times
+the CPU restatement of TiltRec-mpi's loops on a bounded sample of rows.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SIRT voxel·projection updates/s at 1 GPU vs plain FP32; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * q
+    lo = int(k)
+    hi = min(lo + 1, len(xs) - 1)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def time_form(trec, ctx, torch, geom, ang, stack_d, storage, iters, steps, warmup):
+    """Times `steps` reconstructions; returns per-step milliseconds."""
+    method = trec.make_method(trec.SIRT, iters, 1.0, storage=storage)
+    vol = torch.empty((geom.ny, geom.thickness, geom.nx), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        ctx.reconstruct(geom, ang, stack_d, method, vol, stream=stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record(stream)
+    for k in range(steps):
+        ctx.reconstruct(geom, ang, stack_d, method, vol, stream=stream)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    return [ev[k].elapsed_time(ev[k + 1]) for k in range(steps)], vol
+
+
+def cpu_baseline(ang, stack, T, iters, rows):
+    """The CPU restatement of TiltRec-mpi's SIRT (oracle, FP32, one thread)
+    on `rows` detector rows of the same series: rows are independent at
+    pitch 0, so this is the same per-row work as the full problem."""
+    from oracle import orc
+    P, Y, X = stack.shape
+    y0 = Y // 2 - rows // 2
+    sub = stack[:, y0:y0 + rows, :].copy()
+    coef = orc.coeffs_mpi(ang)
+    org = orc.origin(X, T, T / 2)
+    t = time.perf_counter()
+    orc.sirt_f32(coef, sub, T, org, iters, 1.0)
+    dt = time.perf_counter() - t
+    return rows * T * X * P * iters / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="sirt1024")
+    ap.add_argument("--iters", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=2)
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP16/BF16 form lines")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from autom2_amd import synth, tiltrec as trec
+
+    c = synth.CONFIGS[args.config]
+    X, Y, T, P = c["X"], c["Y"], c["T"], c["P"]
+    log(f"[bench] generating {args.config}: {P} x {Y} x {X} -> {Y} x {T} x {X}")
+    ang, stack, _xf, _ = synth.config_series(args.config)
+    stack_d = torch.from_numpy(stack).cuda()
+    geom = trec.make_geom(X, Y, P, T)
+    ctx = trec.Context(0)
+    t0 = time.perf_counter()
+    ctx.autotune(geom, trec.make_method(trec.SIRT, args.iters, 1.0))
+    log(f"[bench] autotune {time.perf_counter() - t0:.2f} s")
+
+    updates = float(X) * Y * T * P * args.iters
+    log(f"[bench] timing FP32 SIRT x{args.iters}: warmup {args.warmup}, steps {args.steps}")
+    ms, vol = time_form(trec, ctx, torch, geom, ang, stack_d, trec.STORE_F32, args.iters, args.steps, args.warmup)
+    ms_mean = sum(ms) / len(ms)
+    value = updates / (ms_mean / 1e3)
+    log(f"[bench] FP32: {ms_mean:.2f} ms/step, {value:.4e} updates/s")
+
+    # live per-kernel durations (HIP events on the launch stream)
+    ctx.profile(True)
+    method = trec.make_method(trec.SIRT, args.iters, 1.0)
+    for _ in range(2):
+        ctx.reconstruct(geom, ang, stack_d, method, vol)
+    torch.cuda.synchronize()
+    kern = {}
+    for k in ("prep", "forward", "combine", "back", "finish"):
+        tot, n = ctx.profile_read(k)
+        kern[k] = {"ms_total": tot, "launches": n, "ms_avg": tot / n if n else 0.0}
+    ctx.profile(False)
+    log(f"[bench] kernels: " + ", ".join(f"{k} {v['ms_avg']:.3f} ms" for k, v in kern.items()))
+
+    # algorithmic bytes per launch (compulsory traffic, float storage)
+    V = float(X) * Y * T
+    PA = float(P) * X * Y
+    alg = {"forward": 4 * V, "combine": 8 * PA, "back": 8 * V + 4 * PA}
+    if kern["combine"]["launches"] == 0:
+        alg["forward"] = 4 * V + 8 * PA
+    dom = max(("forward", "back"), key=lambda k: kern[k]["ms_total"])
+    achieved = alg[dom] / (kern[dom]["ms_avg"] / 1e3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "bytes_per_launch": alg[dom], "ms_per_launch": round(kern[dom]["ms_avg"], 4),
+                "kernels_ms": {k: round(v["ms_avg"], 4) for k, v in kern.items()}}
+
+    forms = {}
+    if not args.no_forms:
+        for name, sto in (("f16", trec.STORE_F16), ("bf16", trec.STORE_BF16)):
+            fms, _ = time_form(trec, ctx, torch, geom, ang, stack_d, sto, args.iters, max(3, args.steps // 2), 1)
+            fm = sum(fms) / len(fms)
+            forms[name] = {"ms_per_step": round(fm, 3), "value": updates / (fm / 1e3),
+                           "speedup_vs_f32": round(ms_mean / fm, 3)}
+            log(f"[bench] {name}: {fm:.2f} ms/step")
+
+    cpu = None
+    if not args.no_cpu:
+        rows = args.cpu_rows
+        log(f"[bench] CPU baseline: oracle SIRT x{args.iters} on {rows} rows")
+        v, dt = cpu_baseline(ang, stack, T, args.iters, rows)
+        cpu = {"value": v, "unit": "voxel-projection updates/s", "cores": 1, "kind": "port",
+               "sample": f"{rows} of {Y} rows of the same SIRT x{args.iters} workload ({dt:.1f} s); "
+                         "rows are independent at pitch 0",
+               "gpu_over_cpu": round(value / v, 1)}
+        log(f"[bench] CPU: {v:.4e} updates/s ({dt:.1f} s)")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-projection updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mean, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"SIRT x{args.iters} gamma 1.0, {P} tilts of {X}x{Y} float32 "
+                               f"(-60..+60 deg) -> {X}x{Y}x{T} volume, TiltRec-mpi rules",
+                   "global_batch": 1, "iterations": args.iters, "tilts": P, "detector": [X, Y],
+                   "volume": [X, Y, T], "parallelism": "single GPU"},
+        "latency_ms": {"p50": round(percentile(ms, 0.5), 3), "p99": round(percentile(ms, 0.99), 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,253 @@
+/*
+ * tiltrec.h -- C ABI of the MI355X TiltRec reconstruction library
+ * (libtiltrec_hip.so).
+ *
+ * The reference has no library boundary: its GPU drivers are C++ functions
+ * taking references to MPI-backed classes, and they write the output file
+ * themselves.  Each entry point below replaces one of them with plain
+ * pointers and sizes; paths are relative to the AuTom2 source tree.
+ *
+ *   reference                                          replaced by
+ *   -------------------------------------------------  -------------------------
+ *   TranslateAngleToCoefficients + Decorate*           trec_coefficients
+ *     (src/TiltRec/src/main/main-mpi.cpp:85-183,
+ *      main-cuda.cpp:111-168)
+ *   ExeRecY -> CuBackProject / CuFBP / CuSIRT / CuSART trec_reconstruct
+ *     (main-cuda.cpp:170-195,
+ *      TiltRec_cuda_y/recon_y.cuh:25-40)
+ *   CuBackProjectZ / CuSIRTZ / CuSARTZ / CuFBPZ         trec_reconstruct with
+ *     (TiltRec_cuda_z/recon_z.cuh:17-39)                 layout TREC_LAYOUT_Z
+ *   ApplyFilterInplace(mode 2)                          trec_filter_rweight
+ *     (src/TiltRec/src/filter/filter_prj.h:24)
+ *   CuReprojectKernel + CuReWeightKernel                trec_forward_project
+ *     (TiltRec_cuda_y/basic.cu:237-264,357-382)
+ *   CuBackProjKernel (basic.cu:383-415)                 trec_backproject
+ *   one iteration of CuSIRT's loop (recon_y.cu:347-382) trec_sirt_iterate
+ *   (none: AuTom2 launches one process per series)      trec_reconstruct_batch,
+ *                                                        trec_server_*
+ *
+ * Conventions
+ *   - Every array argument is a device pointer (hipMalloc'd memory on the
+ *     context's device) unless the name says host.  The caller owns all of
+ *     them; the library owns only its workspace, which a context caches.
+ *   - stack is [nproj][ny][nx] float (an MRC tilt stack, section = tilt).
+ *   - volume layout TREC_LAYOUT_Y is [ny][thickness][nx] (TiltRec-*: MRC
+ *     (nx, ny, nz) = (X, thickness, Y)); TREC_LAYOUT_Z is
+ *     [thickness][ny][nx] (TiltRecZ-*: MRC (X, Y, thickness)).
+ *   - stream is a hipStream_t passed as void* (NULL = the null stream).
+ *     Work is enqueued on it; calls return before it finishes unless the
+ *     description says otherwise.
+ *   - A context is used by one host thread at a time.  Different contexts
+ *     are independent; a server handle is thread-safe.
+ */
+#ifndef TILTREC_H
+#define TILTREC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TREC_OK = 0,
+    TREC_ERR_INVALID = 1,     /* an argument is out of range or inconsistent */
+    TREC_ERR_UNSUPPORTED = 2, /* valid for the reference, not provided here */
+    TREC_ERR_NOMEM = 3,       /* device or host allocation failed */
+    TREC_ERR_DEVICE = 4,      /* the HIP runtime reported an error */
+    TREC_ERR_IO = 5,          /* file could not be read or written */
+    TREC_ERR_BUSY = 6,        /* server: over the memory budget or queue full */
+    TREC_ERR_NOT_READY = 7    /* server: the request has not finished */
+} trec_status;
+
+typedef enum {
+    TREC_BPT = 0,  /* plain back-projection */
+    TREC_FBP = 1,  /* Shepp-Logan filtered back-projection (not provided) */
+    TREC_WBP = 2,  /* R-weighted back-projection */
+    TREC_SIRT = 3,
+    TREC_SART = 4,
+    TREC_ADMM = 5  /* not provided */
+} trec_method_kind;
+
+typedef enum {
+    TREC_STORE_F32 = 0,  /* projections and residuals in float */
+    TREC_STORE_F16 = 1,  /* stored as IEEE half, accumulated in float */
+    TREC_STORE_BF16 = 2  /* stored as bfloat16, accumulated in float */
+} trec_storage;
+
+typedef enum {
+    TREC_LAYOUT_Y = 0, /* [ny][thickness][nx] */
+    TREC_LAYOUT_Z = 1  /* [thickness][ny][nx] */
+} trec_layout;
+
+/* Which binary's numeric rules to follow where the reference binaries
+ * differ: the y origin (T/2 for TiltRec-mpi, Y/2 otherwise; it matters only
+ * with a pitch) and the weight thresholds (exact zero tests in TiltRec-mpi,
+ * main-mpi.cpp:442,571,600; 1e-5 / 1e-6 / 1e-7 in TiltRec-cuda,
+ * basic.cu:170,184,409,434). */
+typedef enum {
+    TREC_RULES_MPI = 0,
+    TREC_RULES_CUDA = 1
+} trec_rules;
+
+/* Geometry of one tilt series: the stack size, the volume thickness and the
+ * --geometry offset,pitch,zshift,thickness option (opts.cpp:232-260). */
+typedef struct {
+    int nx;          /* X: detector width = volume width */
+    int ny;          /* Y: detector height = number of volume rows */
+    int nproj;       /* P: number of tilts */
+    int thickness;   /* T: volume thickness in voxels */
+    float offset;    /* tilt-angle offset, degrees */
+    float pitch;     /* tilt of the specimen about x, degrees */
+    float zshift;    /* shift along z, voxels */
+    int rules;       /* trec_rules */
+} trec_geom;
+
+/* --method NAME,iterations,gamma (opts.cpp:287-335) plus the forms. */
+typedef struct {
+    int method;      /* trec_method_kind */
+    int iterations;  /* SIRT/SART iteration count */
+    float gamma;     /* relaxation */
+    int storage;     /* trec_storage (SIRT/SART); WBP/BPT use float */
+    int layout;      /* trec_layout of `volume` and `initial` */
+} trec_method;
+
+typedef struct trec_context trec_context;
+
+/* Opens a context on HIP device `device`. */
+trec_status trec_context_create(int device, trec_context **out);
+void trec_context_destroy(trec_context *ctx);
+
+/* Releases the cached workspace of a context (it is re-created on demand). */
+trec_status trec_context_trim(trec_context *ctx);
+
+/* Static description of a status code. */
+const char *trec_status_string(trec_status s);
+
+/* Message of the last failure on this context (empty if none). */
+const char *trec_last_error(const trec_context *ctx);
+
+/* Library version string, e.g. "tiltrec-hip 1.0 gfx950". */
+const char *trec_version(void);
+
+/* Host: per-tilt map coefficients as TiltRec-mpi builds them
+ * (main-mpi.cpp:85-183): coef_out[8*i + 0..3] = a0..a3 (detector x),
+ * [8*i + 4..7] = b0..b3 (detector y).  angles_deg is host memory. */
+trec_status trec_coefficients(const trec_geom *geom, const float *angles_deg,
+                              double *coef_out);
+
+/* Device workspace (bytes) trec_reconstruct needs for this geometry and
+ * method, beyond the caller's stack and volume. */
+trec_status trec_workspace_size(const trec_geom *geom, const trec_method *method,
+                                size_t *bytes);
+
+/* Reconstructs one tilt series.  stack is read only.  initial (may be NULL)
+ * seeds iterative methods and is the starting volume of BPT/WBP, as
+ * --initial does in TiltRec-mpi (main-mpi.cpp:778-791); it may alias
+ * volume.  angles_deg is host memory with geom->nproj values. */
+trec_status trec_reconstruct(trec_context *ctx, const trec_geom *geom,
+                             const float *angles_deg, const float *stack,
+                             const float *initial, const trec_method *method,
+                             float *volume, void *stream);
+
+/* R-weighting of nrows rows of nx floats in place (ApplyFilterInplace mode 2,
+ * filter_prj.cpp:652-802). */
+trec_status trec_filter_rweight(trec_context *ctx, float *rows, int nx,
+                                long long nrows, void *stream);
+
+/* Forward projection of a layout-Y volume: rval[p][y][x] = splat of the
+ * volume, rwt[p][y][x] = splat of ones (Reproject, main-mpi.cpp:286-344).
+ * rwt may be NULL. */
+trec_status trec_forward_project(trec_context *ctx, const trec_geom *geom,
+                                 const float *angles_deg, const float *volume,
+                                 float *rval, float *rwt, void *stream);
+
+/* Back-projection of a stack into a layout-Y volume, added to it:
+ * volume += sum over tilts of s/c (BackProject, main-mpi.cpp:411-450). */
+trec_status trec_backproject(trec_context *ctx, const trec_geom *geom,
+                             const float *angles_deg, const float *stack,
+                             float *volume, void *stream);
+
+/* One SIRT iteration on a layout-Y volume in place (main-mpi.cpp:627-658). */
+trec_status trec_sirt_iterate(trec_context *ctx, const trec_geom *geom,
+                              const float *angles_deg, const float *stack,
+                              float gamma, float *volume, void *stream);
+
+/* One job of a batch: a tilt series and where its volume goes. */
+typedef struct {
+    trec_geom geom;
+    trec_method method;
+    const float *angles_deg; /* host */
+    const float *stack;      /* device [nproj][ny][nx] */
+    const float *initial;    /* device or NULL */
+    float *volume;           /* device */
+} trec_job;
+
+/* Reconstructs njobs series of any mix of sizes and methods together: every
+ * phase of every job is issued in one multi-job kernel per phase, and the
+ * whole schedule is captured in one HIP graph and launched once.  Each job's
+ * result equals what trec_reconstruct gives for it alone. */
+trec_status trec_reconstruct_batch(trec_context *ctx, int njobs,
+                                   const trec_job *jobs, void *stream);
+
+/* Autotuning: tries the launch shapes of the forward projection for this
+ * geometry and keeps the fastest in the context (and in the process-wide
+ * table used by later contexts).  Synchronises the device. */
+trec_status trec_autotune(trec_context *ctx, const trec_geom *geom,
+                          const trec_method *method);
+
+/* Per-kernel timing: while enabled, every launch group of the
+ * reconstruction calls is bracketed by HIP events on the stream it runs on
+ * (HIP graphs are not used meanwhile).  kernel is one of "prep" (layout
+ * transposes, filter, weights), "forward", "combine", "back", "finish";
+ * trec_profile_read waits for the recorded events and returns the summed
+ * milliseconds and the number of launch groups since enabling. */
+trec_status trec_profile_enable(trec_context *ctx, int on);
+trec_status trec_profile_read(trec_context *ctx, const char *kernel,
+                              double *total_ms, long long *launches);
+
+/* ---------------------------------------------------------------- serving
+ * A server keeps a request loop on its own host thread.  Requests carry
+ * host stacks; the server copies each into device memory when it admits
+ * it, runs every admitted request one SIRT iteration (or one WBP/BPT pass)
+ * per step in shared multi-job launches, and admits new requests between
+ * steps while their device footprint fits the budget. */
+typedef struct trec_server trec_server;
+
+typedef struct {
+    trec_geom geom;
+    trec_method method;
+    const float *angles_deg; /* host, copied at submit */
+    const float *stack;      /* host [nproj][ny][nx], copied at submit */
+    float *volume_out;       /* host, written when the request completes */
+} trec_request;
+
+typedef struct {
+    trec_status status;
+    double submit_s;   /* host clock when submitted */
+    double admit_s;    /* when admitted to the device */
+    double done_s;     /* when its volume was back on the host */
+    size_t footprint;  /* device bytes it held */
+} trec_result;
+
+/* hbm_budget = 0 uses 90 % of the device's free memory. */
+trec_status trec_server_create(int device, size_t hbm_budget, int max_active,
+                               trec_server **out);
+trec_status trec_server_submit(trec_server *srv, const trec_request *req,
+                               uint64_t *id_out);
+/* *done = 1 once the request finished (successfully or not). */
+trec_status trec_server_poll(trec_server *srv, uint64_t id, int *done);
+/* Blocks until the request finished and returns its record; the id is
+ * released afterwards. */
+trec_status trec_server_wait(trec_server *srv, uint64_t id, trec_result *res);
+/* Device bytes a request would hold (the admission footprint). */
+trec_status trec_request_footprint(const trec_geom *geom,
+                                   const trec_method *method, size_t *bytes);
+void trec_server_destroy(trec_server *srv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TILTREC_H */

@@ -1,0 +1,230 @@
+"""HIP library vs the CPU oracle on identical seeded inputs.
+
+Tolerances: the plain FP32 form must be within relative L2 1e-5 of the FP32
+restatement of TiltRec-mpi; the achieved figures are
+printed.  The 16-bit storage forms are checked against the FP64 oracle at
+1e-3.  Every call goes through the C ABI (autom2_amd.tiltrec).
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import noisy_stack, smooth_volume, tilts, to_layout_z
+
+pytestmark = pytest.mark.gpu
+
+TOL_F32 = 1e-5
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def trec(torch_cuda):
+    from autom2_amd import tiltrec
+    tiltrec.lib()
+    return tiltrec
+
+
+@pytest.fixture(scope="module")
+def ctx(trec):
+    c = trec.Context(0)
+    yield c
+    c.close()
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def case(orc, P=9, Y=20, T=12, X=40, span=60.0, seed=0, offset=0.0, pitch=0.0, zshift=0.0):
+    ang = tilts(P, span)
+    coef = orc.coeffs_mpi(ang, offset, pitch, zshift)
+    org = orc.origin(X, T, T / 2)
+    vol = smooth_volume(Y, T, X, seed)
+    stack = noisy_stack(orc, coef, vol, org, seed + 1)
+    return ang, coef, org, stack
+
+
+def run(torch, trec, ctx, stack, ang, T, method, initial=None, offset=0.0, pitch=0.0, zshift=0.0,
+        rules=0, layout=0, storage=0):
+    P, Y, X = stack.shape
+    g = trec.make_geom(X, Y, P, T, offset, pitch, zshift, rules)
+    m = trec.make_method(method, storage=storage, layout=layout)
+    s = dev(torch, stack)
+    out = torch.empty((Y, T, X) if layout == 0 else (T, Y, X), dtype=torch.float32, device="cuda")
+    init = dev(torch, initial) if initial is not None else None
+    ctx.reconstruct(g, ang, s, m, out, initial=init)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def report(name, got, ref, ref64=None):
+    from oracle.orc import rel_l2
+    e = rel_l2(got, ref)
+    msg = f"{name}: rel L2 vs FP32 oracle {e:.3e}"
+    if ref64 is not None:
+        msg += f", vs FP64 oracle {rel_l2(got, ref64):.3e}"
+    print(msg)
+    return e
+
+
+@pytest.mark.parametrize("shape", [(9, 20, 12, 40), (7, 16, 5, 33), (13, 3, 16, 64)])
+def test_bpt(torch_cuda, trec, ctx, orc, shape):
+    P, Y, T, X = shape
+    ang, coef, org, stack = case(orc, P, Y, T, X)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "BPT")
+    ref = orc.bpt_f32(coef, stack, T, org)
+    ref64 = orc.bpt_f64(coef, stack.astype(np.float64), T, org)
+    assert report("BPT", got, ref, ref64) < 1e-6
+
+
+@pytest.mark.parametrize("iters", [1, 5])
+def test_sirt(torch_cuda, trec, ctx, orc, iters):
+    P, Y, T, X = 11, 18, 10, 48
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=3)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, f"SIRT,{iters},1.0")
+    ref = orc.sirt_f32(coef, stack, T, org, iters, 1.0)
+    ref64 = orc.sirt_f64(coef, stack.astype(np.float64), T, org, iters, 1.0)
+    assert report(f"SIRT x{iters}", got, ref, ref64) < TOL_F32
+
+
+def test_sirt_z_split_matches(torch_cuda, trec, ctx, orc, monkeypatch):
+    """The forward projection split into z chunks gives the same volume."""
+    P, Y, T, X = 9, 16, 24, 40
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=4)
+    ref = orc.sirt_f32(coef, stack, T, org, 3, 0.9)
+    for nzc in ("1", "3", "24"):
+        monkeypatch.setenv("TREC_NZC", nzc)
+        got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,3,0.9")
+        assert report(f"SIRT nzc={nzc}", got, ref) < TOL_F32
+
+
+def test_sirt_initial(torch_cuda, trec, ctx, orc):
+    P, Y, T, X = 8, 17, 9, 30
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=5)
+    init = smooth_volume(Y, T, X, 9)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,2,1.0", initial=init)
+    ref = orc.sirt_f32(coef, stack, T, org, 2, 1.0, vol=init)
+    assert report("SIRT initial", got, ref) < TOL_F32
+
+
+def test_sart(torch_cuda, trec, ctx, orc):
+    P, Y, T, X = 7, 16, 8, 36
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=6)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "SART,2,0.5")
+    ref = orc.sart_f32(coef, stack, T, org, 2, 0.5)
+    ref64 = orc.sart_f64(coef, stack.astype(np.float64), T, org, 2, 0.5)
+    assert report("SART", got, ref, ref64) < TOL_F32
+
+
+def test_wbp(torch_cuda, trec, ctx, orc):
+    P, Y, T, X = 9, 16, 10, 50
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=7)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "WBP")
+    filt = orc.rweight_f64(stack.astype(np.float64))
+    ref64 = orc.bpt_f64(coef, filt, T, org)
+    ref32 = orc.bpt_f32(coef, filt.astype(np.float32), T, org)
+    assert report("WBP", got, ref32, ref64) < TOL_F32
+
+
+def test_filter_api(torch_cuda, trec, ctx, orc):
+    rng = np.random.default_rng(8)
+    for X in (64, 100, 256):
+        rows = rng.standard_normal((10, X)).astype(np.float32)
+        t = dev(torch_cuda, rows)
+        ctx.filter_rweight(t)
+        torch_cuda.cuda.synchronize()
+        ref = orc.rweight_f64(rows.astype(np.float64))
+        assert orc.rel_l2(t.cpu().numpy(), ref) < 1e-5
+
+
+def test_layout_z(torch_cuda, trec, ctx, orc):
+    P, Y, T, X = 9, 20, 12, 40
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=9)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,2,1.0", layout=1)
+    ref = to_layout_z(orc.sirt_f32(coef, stack, T, org, 2, 1.0))
+    assert report("SIRT layout Z", got, ref) < TOL_F32
+
+
+@pytest.mark.parametrize("offset,zshift", [(3.0, 0.0), (0.0, 2.5), (-2.0, -1.5)])
+def test_offset_zshift_rows_path(torch_cuda, trec, ctx, orc, offset, zshift):
+    P, Y, T, X = 9, 16, 12, 40
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=10, offset=offset, zshift=zshift)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,3,1.0", offset=offset, zshift=zshift)
+    ref = orc.sirt_f32(coef, stack, T, org, 3, 1.0)
+    assert report(f"SIRT offset={offset} zshift={zshift}", got, ref) < TOL_F32
+
+
+@pytest.mark.parametrize("method", ["BPT", "SIRT,3,1.0", "SART,1,0.8", "WBP"])
+def test_pitch_general_path(torch_cuda, trec, ctx, orc, method):
+    P, Y, T, X = 7, 12, 8, 28
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=11, pitch=6.0, offset=1.0)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=6.0, offset=1.0)
+    name = method.split(",")[0]
+    if name == "BPT":
+        ref = orc.bpt_f32(coef, stack, T, org)
+    elif name == "SIRT":
+        ref = orc.sirt_f32(coef, stack, T, org, 3, 1.0)
+    elif name == "SART":
+        ref = orc.sart_f32(coef, stack, T, org, 1, 0.8)
+    else:
+        ref = orc.bpt_f32(coef, orc.rweight_f64(stack.astype(np.float64)).astype(np.float32), T, org)
+    assert report(f"pitch {method}", got, ref) < TOL_F32
+
+
+@pytest.mark.parametrize("pitch", [0.0, 5.0])
+def test_forward_project_api(torch_cuda, trec, ctx, orc, pitch):
+    P, Y, T, X = 7, 18, 10, 36
+    ang = tilts(P)
+    coef = orc.coeffs_mpi(ang, 0.0, pitch, 0.0)
+    org = orc.origin(X, T, T / 2)
+    vol = smooth_volume(Y, T, X, 12)
+    g = trec.make_geom(X, Y, P, T, 0.0, pitch, 0.0)
+    rv = torch_cuda.empty((P, Y, X), dtype=torch_cuda.float32, device="cuda")
+    rw = torch_cuda.empty_like(rv)
+    ctx.forward_project(g, ang, dev(torch_cuda, vol), rv, rw)
+    torch_cuda.cuda.synchronize()
+    ref = np.stack([orc.reproject_f32(coef[k], vol, org)[0] for k in range(P)])
+    refw = np.stack([orc.reproject_f32(coef[k], vol, org)[1] for k in range(P)])
+    assert report("forward value", rv.cpu().numpy(), ref) < 1e-6
+    assert report("forward weight", rw.cpu().numpy(), refw) < 1e-6
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_16bit_storage_vs_fp64(torch_cuda, trec, ctx, orc, storage):
+    P, Y, T, X = 11, 16, 10, 48
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=13)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,10,1.0", storage=storage)
+    ref64 = orc.sirt_f64(coef, stack.astype(np.float64), T, org, 10, 1.0)
+    e = orc.rel_l2(got, ref64)
+    print(f"SIRT x10 {storage}: rel L2 vs FP64 {e:.3e}")
+    assert e < 1e-3
+
+
+def test_cuda_rules_thresholds(torch_cuda, trec, ctx, orc):
+    """TREC_RULES_CUDA skips voxels whose weight sum is at most 1e-5
+    (basic.cu:184,409); interior voxels are unchanged."""
+    P, Y, T, X = 9, 16, 12, 40
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=14)
+    a = run(torch_cuda, trec, ctx, stack, ang, T, "BPT", rules=0)
+    b = run(torch_cuda, trec, ctx, stack, ang, T, "BPT", rules=1)
+    np.testing.assert_allclose(a[:, :, 12:28], b[:, :, 12:28], rtol=1e-6, atol=1e-6)
+
+
+def test_bad_arguments_fail_loudly(torch_cuda, trec, ctx):
+    g = trec.make_geom(0, 4, 3, 4)
+    m = trec.make_method("BPT")
+    s = torch_cuda.zeros((3, 4, 4), device="cuda")
+    out = torch_cuda.zeros((4, 4, 4), device="cuda")
+    with pytest.raises(trec.TrecError) as ei:
+        ctx.reconstruct(g, [0, 1, 2], s, m, out)
+    assert ei.value.status == trec.ERR_INVALID
+    g = trec.make_geom(4, 4, 3, 4)
+    with pytest.raises(trec.TrecError) as ei:
+        ctx.reconstruct(g, [0, 1, 2], s, trec.make_method("ADMM,5,3,1.0,0.1"), out)
+    assert ei.value.status == trec.ERR_UNSUPPORTED
