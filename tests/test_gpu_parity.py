@@ -195,15 +195,19 @@ def test_forward_project_api(torch_cuda, trec, ctx, orc, pitch):
     assert report("forward weight", rw.cpu().numpy(), refw) < 1e-6
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
-def test_16bit_storage_vs_fp64(torch_cuda, trec, ctx, orc, storage):
+# FP16 storage must stay within 1e-3 of the FP64 oracle.  BF16 keeps 8
+# mantissa bits, so the stored projections alone carry ~2^-9 relative error
+# and the volume inherits it (~2e-3): its bound is looser and its error is
+# reported, not held to the FP16 bar.
+@pytest.mark.parametrize("storage,bound", [("f16", 1e-3), ("bf16", 5e-3)])
+def test_16bit_storage_vs_fp64(torch_cuda, trec, ctx, orc, storage, bound):
     P, Y, T, X = 11, 16, 10, 48
     ang, coef, org, stack = case(orc, P, Y, T, X, seed=13)
     got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,10,1.0", storage=storage)
     ref64 = orc.sirt_f64(coef, stack.astype(np.float64), T, org, 10, 1.0)
     e = orc.rel_l2(got, ref64)
     print(f"SIRT x10 {storage}: rel L2 vs FP64 {e:.3e}")
-    assert e < 1e-3
+    assert e < bound
 
 
 def test_cuda_rules_thresholds(torch_cuda, trec, ctx, orc):
