@@ -338,7 +338,10 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             t.inv_a1 = 1.0 / c[1];
             t.slope = -c[3] / c[1];
             t.a1f = (float)c[1];
-            t.ncand = (int)std::ceil(2.0 / c[1]);
+            // candidates a pixel needs per layer: ceil(2 / a1); the 1e-6 keeps a
+            // float-rounded 60 degrees (a1 = 0.49999997) at 4, dropping a
+            // candidate whose weight is below 1.2e-7
+            t.ncand = (int)std::ceil(2.0 / c[1] - 1e-6);
         }
         if (m.method == TREC_WBP) p.hk_host = rweight_kernel(g.nx, p.N);
     } else {
@@ -487,6 +490,13 @@ trec_status upload(trec_context *ctx, const std::vector<JobPlan *> &plans, hipSt
         if (e_ != hipSuccess) return e_;         \
     } while (0)
 
+// Work-groups of a value forward projection (the kernel launch_forward picks).
+unsigned fwd_blocks(const JobDesc &h, int ntheta)
+{
+    static const bool direct = getenv("TREC_FP_DIRECT") != nullptr;
+    return direct ? forward_blocks(h, ntheta) : forward_wave_blocks(h, ntheta);
+}
+
 hipError_t issue_prep(JobPlan &p, hipStream_t st)
 {
     ProfScope ps(PROF_PREP, st);
@@ -503,8 +513,9 @@ hipError_t issue_prep(JobPlan &p, hipStream_t st)
         if (p.initial) LCHK(launch_block_volume(h, p.d, p.initial, p.m.layout, st));
         else LCHK(launch_zero(h.vol, (size_t)h.nyb * T * X * RB, st));
         if (k == TREC_WBP)
-            LCHK(launch_rweight((float *)h.proj, p.hk, p.N, (int)X, (long long)h.nyb * P * RB, 1, RB, RB,
-                                (long long)X * RB, st));
+            // rows of the float slices: 4-row planes of X cells, row stride 1, cell stride 4
+            LCHK(launch_rweight((float *)h.proj, p.hk, p.N, (int)X, (long long)h.nyb * P * RB, 1, 4, 4,
+                                (long long)X * 4, st));
         if (is_iterative(k)) {
             LaunchList L{};
             L.n = 1;
@@ -622,16 +633,16 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
         switch (p.m.method) {
         case TREC_SIRT:
             if (h.nzc == 1) {
-                fp_fused[sto].add(p.d, forward_blocks(h, h.P), 0, 0);
+                fp_fused[sto].add(p.d, fwd_blocks(h, h.P), 0, 0);
             } else {
-                fp_part.add(p.d, forward_blocks(h, h.P), 0, 0);
+                fp_part.add(p.d, fwd_blocks(h, h.P), 0, 0);
                 comb[sto].add(p.d, combine_blocks(h, h.P), 0, 0);
             }
             back[BACK_SIRT][sto].add(p.d, back_blocks(h), 0, 0);
             break;
         case TREC_SART: {
             const int th = s % h.P;
-            fp_fused[sto].add(p.d, forward_blocks(h, 1), th, 1);
+            fp_fused[sto].add(p.d, fwd_blocks(h, 1), th, 1);
             back[BACK_SART][sto].add(p.d, back_blocks(h), th, 1);
             break;
         }
@@ -949,7 +960,7 @@ trec_status trec_forward_project(trec_context *ctx, const trec_geom *geom, const
         L.n = 1;
         L.job[0] = p.d;
         L.off[0] = 0;
-        L.off[1] = forward_blocks(h, (int)P);
+        L.off[1] = fwd_blocks(h, (int)P);
         HIPCHK(ctx, launch_forward(L, L.off[1], TREC_STORE_F32, false, st));
         HIPCHK(ctx, launch_unblock_proj(h.part, (int)P, (int)X, (int)Y, h.nyb, rval, st));
         if (rwt) {

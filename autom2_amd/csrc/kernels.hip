@@ -27,6 +27,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "trec_internal.h"
 
@@ -52,73 +53,85 @@ __device__ __forceinline__ int find_job(const LaunchList &L, unsigned b)
     return j;
 }
 
-// Row-block storage: RB values per (pixel or voxel), in float, half or bf16.
-template <typename PT>
-struct Rows;
-
-template <>
-struct Rows<float> {
-    __device__ static __forceinline__ void load(const void *base, size_t cell, float (&v)[RB])
+// Row-block slices.  A slice holds the RB rows of NC cells (pixels of one
+// tilt, or voxels of one z layer) as RB / PR planes; plane q holds rows
+// q*PR .. q*PR+PR-1 of every cell as one 16-byte vector per cell, PR = 4
+// for float and 8 for 16-bit storage:
+//     element (cell c, row r) = slice[((r / PR) * NC + c) * PR + r % PR]
+// A wave that reads one plane for 64 neighbouring cells touches one
+// contiguous run of memory, so each cache line is looked up once.
+template <typename T>
+struct Slice {
+    static constexpr int PR = 16 / (int)sizeof(T);
+    static constexpr int NP = RB / PR;
+    __device__ static __forceinline__ size_t index(int NC, int c, int r)
     {
-        const float4 *p = reinterpret_cast<const float4 *>(static_cast<const float *>(base) + cell * RB);
+        return ((size_t)(r / PR) * NC + c) * PR + (r % PR);
+    }
+    __device__ static __forceinline__ void unpack(const uint4 raw, float *o)
+    {
+        if constexpr (sizeof(T) == 4) {
+            o[0] = __uint_as_float(raw.x);
+            o[1] = __uint_as_float(raw.y);
+            o[2] = __uint_as_float(raw.z);
+            o[3] = __uint_as_float(raw.w);
+        } else {
+            const T *h = reinterpret_cast<const T *>(&raw);
 #pragma unroll
-        for (int k = 0; k < RB / 4; ++k) {
-            float4 t = p[k];
-            v[4 * k + 0] = t.x;
-            v[4 * k + 1] = t.y;
-            v[4 * k + 2] = t.z;
-            v[4 * k + 3] = t.w;
+            for (int e = 0; e < PR; ++e) o[e] = (float)h[e];
         }
     }
-    __device__ static __forceinline__ void store(void *base, size_t cell, const float (&v)[RB])
+    __device__ static __forceinline__ uint4 pack(const float *v)
     {
-        float4 *p = reinterpret_cast<float4 *>(static_cast<float *>(base) + cell * RB);
+        uint4 raw;
+        if constexpr (sizeof(T) == 4) {
+            raw = make_uint4(__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]), __float_as_uint(v[3]));
+        } else {
+            T *h = reinterpret_cast<T *>(&raw);
 #pragma unroll
-        for (int k = 0; k < RB / 4; ++k) p[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+            for (int e = 0; e < PR; ++e) h[e] = (T)v[e];
+        }
+        return raw;
     }
-    __device__ static __forceinline__ void put4(void *base, size_t idx, float a, float b, float c, float d)
+    __device__ static __forceinline__ void load(const void *s, int NC, int c, float (&v)[RB])
     {
-        *reinterpret_cast<float4 *>(static_cast<float *>(base) + idx) = make_float4(a, b, c, d);
+        const T *b = static_cast<const T *>(s);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) unpack(*reinterpret_cast<const uint4 *>(b + ((size_t)q * NC + c) * PR), v + q * PR);
+    }
+    __device__ static __forceinline__ void store(void *s, int NC, int c, const float (&v)[RB])
+    {
+        T *b = static_cast<T *>(s);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) *reinterpret_cast<uint4 *>(b + ((size_t)q * NC + c) * PR) = pack(v + q * PR);
     }
 };
 
-template <typename H>
-struct Rows16 {
-    // 16-bit storage: RB values = 32 bytes = two 16-byte loads.
-    __device__ static __forceinline__ void load(const void *base, size_t cell, float (&v)[RB])
+// Buffer-resource view of a slice, one resource per plane: a cell outside
+// [0, NC) (including negative cells) reads as zero, so taps that fall off
+// the detector need no clamp or select.  The slice base must be uniform.
+constexpr int BUF_DWORD3 = 0x00020000;  // raw buffer, gfx9 family
+
+template <typename T>
+struct BufSlice {
+    __amdgpu_buffer_rsrc_t r[Slice<T>::NP];
+    __device__ __forceinline__ BufSlice(const void *s, int NC)
     {
-        const uint4 *p = reinterpret_cast<const uint4 *>(static_cast<const H *>(base) + cell * RB);
+        const T *b = static_cast<const T *>(s);
 #pragma unroll
-        for (int k = 0; k < RB / 8; ++k) {
-            uint4 t = p[k];
-            const H *h = reinterpret_cast<const H *>(&t);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) v[8 * k + e] = (float)h[e];
-        }
+        for (int q = 0; q < Slice<T>::NP; ++q)
+            r[q] = __builtin_amdgcn_make_buffer_rsrc((void *)(b + (size_t)q * NC * Slice<T>::PR), (short)0, NC * 16,
+                                                     BUF_DWORD3);
     }
-    __device__ static __forceinline__ void store(void *base, size_t cell, const float (&v)[RB])
+    __device__ __forceinline__ void load(int c, float (&v)[RB]) const
     {
-        uint4 *p = reinterpret_cast<uint4 *>(static_cast<H *>(base) + cell * RB);
 #pragma unroll
-        for (int k = 0; k < RB / 8; ++k) {
-            uint4 t;
-            H *h = reinterpret_cast<H *>(&t);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) h[e] = (H)v[8 * k + e];
-            p[k] = t;
+        for (int q = 0; q < Slice<T>::NP; ++q) {
+            const auto raw = __builtin_amdgcn_raw_buffer_load_b128(r[q], c * 16, 0, 0);
+            Slice<T>::unpack(make_uint4(raw[0], raw[1], raw[2], raw[3]), v + q * Slice<T>::PR);
         }
-    }
-    __device__ static __forceinline__ void put4(void *base, size_t idx, float a, float b, float c, float d)
-    {
-        H t[4] = {(H)a, (H)b, (H)c, (H)d};
-        *reinterpret_cast<uint2 *>(static_cast<H *>(base) + idx) = *reinterpret_cast<uint2 *>(t);
     }
 };
-
-template <>
-struct Rows<_Float16> : Rows16<_Float16> {};
-template <>
-struct Rows<__bf16> : Rows16<__bf16> {};
 
 // Exact TiltRec-mpi geometry (ValCoef, main-mpi.cpp:204-226) at pitch 0:
 // u = ((a0 + a1 * X) + a3 * Z) + ox in double, taps floor(u) and floor(u)+1,
@@ -133,7 +146,7 @@ __device__ __forceinline__ void exact_tap(const TiltConst &t, double Xc, double 
 
 // ------------------------------------------------------------------ layouts
 
-// stack [P][Y][X] float -> proj [nyb][P][X][RB] (storage type), / scale.
+// stack [P][Y][X] float -> proj slices (yb, t), storage type, / scale.
 template <typename PT>
 __global__ __launch_bounds__(WG) void k_block_stack(const JobDesc *__restrict__ jd, const float *__restrict__ stack)
 {
@@ -155,15 +168,21 @@ __global__ __launch_bounds__(WG) void k_block_stack(const JobDesc *__restrict__ 
         tile[r][xx] = v;
     }
     __syncthreads();
-    const int xx = threadIdx.x / 4, q = threadIdx.x % 4;
-    const int x = x0 + xx;
-    if (x < J.X) {
-        const size_t idx = (((size_t)yb * J.P + th) * J.X + x) * RB + q * 4;
-        Rows<PT>::put4(J.proj, idx, tile[q * 4][xx], tile[q * 4 + 1][xx], tile[q * 4 + 2][xx], tile[q * 4 + 3][xx]);
+    using S = Slice<PT>;
+    PT *slice = static_cast<PT *>(J.proj) + ((size_t)yb * J.P + th) * J.X * RB;
+    for (int k = threadIdx.x; k < 64 * S::NP; k += WG) {
+        const int xx = k % 64, q = k / 64;
+        const int x = x0 + xx;
+        if (x < J.X) {
+            float v[S::PR];
+#pragma unroll
+            for (int e = 0; e < S::PR; ++e) v[e] = tile[q * S::PR + e][xx];
+            *reinterpret_cast<uint4 *>(slice + ((size_t)q * J.X + x) * S::PR) = S::pack(v);
+        }
     }
 }
 
-// volume (layout Y [Y][T][X] or Z [T][Y][X]) -> vol [nyb][T][X][RB], / scale.
+// volume (layout Y [Y][T][X] or Z [T][Y][X]) -> vol slices (yb, z), / scale.
 __global__ __launch_bounds__(WG) void k_block_volume(const JobDesc *__restrict__ jd, const float *__restrict__ src,
                                                      int layout)
 {
@@ -188,16 +207,18 @@ __global__ __launch_bounds__(WG) void k_block_volume(const JobDesc *__restrict__
         tile[r][xx] = v;
     }
     __syncthreads();
-    const int xx = threadIdx.x / 4, q = threadIdx.x % 4;
-    const int x = x0 + xx;
-    if (x < J.X) {
-        const size_t idx = (((size_t)yb * J.T + z) * J.X + x) * RB + q * 4;
-        *reinterpret_cast<float4 *>(J.vol + idx) =
-            make_float4(tile[q * 4][xx], tile[q * 4 + 1][xx], tile[q * 4 + 2][xx], tile[q * 4 + 3][xx]);
+    using S = Slice<float>;
+    float *slice = J.vol + ((size_t)yb * J.T + z) * J.X * RB;
+    for (int k = threadIdx.x; k < 64 * S::NP; k += WG) {
+        const int xx = k % 64, q = k / 64;
+        const int x = x0 + xx;
+        if (x < J.X)
+            *reinterpret_cast<float4 *>(slice + ((size_t)q * J.X + x) * 4) =
+                make_float4(tile[q * 4][xx], tile[q * 4 + 1][xx], tile[q * 4 + 2][xx], tile[q * 4 + 3][xx]);
     }
 }
 
-// vol [nyb][T][X][RB] -> volume (layout Y or Z), * scale.
+// vol slices (yb, z) -> volume (layout Y or Z), * scale.
 __global__ __launch_bounds__(WG) void k_unblock_volume(const JobDesc *__restrict__ jd, float *__restrict__ dst,
                                                        int layout)
 {
@@ -210,11 +231,12 @@ __global__ __launch_bounds__(WG) void k_unblock_volume(const JobDesc *__restrict
     const int z = b % J.T;
     const int yb = b / J.T;
     const int x0 = xt * 64;
-    {
-        const int xx = threadIdx.x / 4, q = threadIdx.x % 4;
+    const float *slice = J.vol + ((size_t)yb * J.T + z) * J.X * RB;
+    for (int k = threadIdx.x; k < 64 * 4; k += WG) {
+        const int xx = k % 64, q = k / 64;
         const int x = x0 + xx;
         if (x < J.X) {
-            const float4 v = *reinterpret_cast<const float4 *>(J.vol + (((size_t)yb * J.T + z) * J.X + x) * RB + q * 4);
+            const float4 v = *reinterpret_cast<const float4 *>(slice + ((size_t)q * J.X + x) * 4);
             tile[q * 4][xx] = v.x;
             tile[q * 4 + 1][xx] = v.y;
             tile[q * 4 + 2][xx] = v.z;
@@ -274,6 +296,31 @@ unsigned forward_blocks(const JobDesc &h, int ntheta)
     return npb * (unsigned)ntheta * (unsigned)h.nzc * (unsigned)h.nyb;
 }
 
+// Writes a forward-projection result: the ray weights (ONES), the residual
+// p - R / Rwt (FUSED), or the z-chunk partial.
+template <typename PT, bool FUSED, bool ONES, int NA>
+__device__ __forceinline__ void fp_epilogue(const JobDesc &J, int th, int zc, int yb, int i, float (&acc)[NA])
+{
+    const int X = J.X;
+    if constexpr (ONES) {
+        if (yb == 0) {
+            J.inv_rwt[th * X + i] = (acc[0] > J.thr_res) ? 1.0f / acc[0] : 0.f;
+            if (J.rwt) J.rwt[th * X + i] = acc[0];
+        }
+    } else if constexpr (FUSED) {
+        const size_t slice = ((size_t)yb * J.P + th) * X * RB;
+        const float inv = J.inv_rwt[th * X + i];
+        float p[RB];
+        Slice<PT>::load(static_cast<const PT *>(J.proj) + slice, X, i, p);
+#pragma unroll
+        for (int r = 0; r < RB; ++r) acc[r] = fmaf(-acc[r], inv, p[r]);
+        Slice<PT>::store(static_cast<PT *>(J.res) + slice, X, i, acc);
+    } else {
+        const size_t slice = (((size_t)zc * J.nyb + yb) * J.P + th) * X * RB;
+        Slice<float>::store(J.part + slice, X, i, acc);
+    }
+}
+
 // Pixel-driven forward projection of one z chunk.
 //   FUSED: nzc == 1, write the residual p - R / Rwt directly.
 //   ONES:  splat of ones (Rwt); writes inv_rwt[t][i] from row block 0.
@@ -316,41 +363,161 @@ __global__ __launch_bounds__(WG) void k_forward(const LaunchList L, unsigned nbl
         const double fl = floor(s);
         const float frac = (float)(s - fl);
         const int xf = (int)fl + 1;
+        // no voxel of this layer reaches pixel i
+        if (xf >= X || xf + ncand <= 0) continue;
         const float d0 = fmaf(-a1f, frac, a1m1);
-        const float *vz = vbase + (size_t)z * X * RB;
-        for (int c = 0; c < ncand; ++c) {
-            const int x = xf + c;
-            const float d = fmaf(a1f, (float)c, d0);
-            const bool ok = (unsigned)x < (unsigned)X;
-            const float w = ok ? fmaxf(0.f, 1.f - fabsf(d)) : 0.f;
-            if constexpr (ONES) {
-                acc[0] += w;
-            } else {
+        if constexpr (ONES) {
+            for (int c = 0; c < ncand; ++c) {
+                const int x = xf + c;
+                const float d = fmaf(a1f, (float)c, d0);
+                acc[0] += ((unsigned)x < (unsigned)X) ? fmaxf(0.f, 1.f - fabsf(d)) : 0.f;
+            }
+        } else {
+            const BufSlice<float> vz(vbase + (size_t)z * X * RB, X);
+            for (int c = 0; c < ncand; ++c) {
+                const float d = fmaf(a1f, (float)c, d0);
+                const float w = fmaxf(0.f, 1.f - fabsf(d));
                 float v[RB];
-                Rows<float>::load(vz, ok ? x : 0, v);
+                vz.load(xf + c, v);  // voxels outside [0, X) read as 0
 #pragma unroll
                 for (int r = 0; r < RB; ++r) acc[r] = fmaf(w, v[r], acc[r]);
             }
         }
     }
+    fp_epilogue<PT, FUSED, ONES>(J, th, zc, yb, i, acc);
+}
 
-    if constexpr (ONES) {
-        if (yb == 0) {
-            J.inv_rwt[th * X + i] = (acc[0] > J.thr_res) ? 1.0f / acc[0] : 0.f;
-            if (J.rwt) J.rwt[th * X + i] = acc[0];
-        }
-    } else if constexpr (FUSED) {
-        const size_t cell = ((size_t)yb * J.P + th) * X + i;
-        const float inv = J.inv_rwt[th * X + i];
-        float p[RB];
-        Rows<PT>::load(J.proj, cell, p);
+unsigned forward_wave_blocks(const JobDesc &h, int ntheta)
+{
+    const unsigned npb = (h.X + FW_PIX - 1) / FW_PIX;
+    return npb * (unsigned)ntheta * (unsigned)h.nzc * (unsigned)h.nyb;
+}
+
+// Forward projection, one wave per work-group, with the voxel band of each
+// z layer staged in LDS.  The wave's 64 pixels of tilt t see, in layer z,
+// the voxels [xlo(z), xlo(z) + w(z)); the wave loads that band once (plane
+// by plane, fully coalesced; layer z + 1 goes to registers while layer z is
+// summed out of LDS) and every pixel then reads its ncand candidates from
+// LDS.  Voxels outside [0, X) load as 0.  A band wider than FW_W (tilts past
+// about 60 degrees) is gathered straight from L2 instead.
+template <typename PT, bool FUSED>
+__global__ __launch_bounds__(FW_PIX) void k_forward_w(const LaunchList L, unsigned nblk)
+{
+    __shared__ float4 band[4][FW_W];
+    const unsigned b0 = xcd_remap(blockIdx.x, nblk);
+    const int j = find_job(L, b0);
+    const JobDesc &J = *L.job[j];
+    unsigned lb = b0 - L.off[j];
+    const int ntheta = L.ntheta[j] ? L.ntheta[j] : J.P;
+    const unsigned npb = (J.X + FW_PIX - 1) / FW_PIX;
+    const int pb = lb % npb;
+    lb /= npb;
+    const int th = L.theta0[j] + (int)(lb % ntheta);
+    lb /= ntheta;
+    const int zc = lb % J.nzc;
+    const int yb = lb / J.nzc;
+    const int lane = threadIdx.x;
+    const int i0 = pb * FW_PIX;
+    const int i = i0 + lane;
+    const TiltConst &t = J.tc[th];
+    const int X = J.X;
+    const int z0 = zc * J.zc_len;
+    const int z1 = min(J.T, z0 + J.zc_len);
+    const double sbase = ((double)(i - 1) - t.c0) * t.inv_a1;
+    const double s_lo = ((double)(i0 - 1) - t.c0) * t.inv_a1;
+    const double s_hi = ((double)(i0 + FW_PIX - 2) - t.c0) * t.inv_a1;
+    const float a1f = t.a1f, a1m1 = t.a1f - 1.0f;
+    const int ncand = t.ncand;
+    const float *vbase = J.vol + (size_t)yb * J.T * X * RB;
+
+    float acc[RB];
 #pragma unroll
-        for (int r = 0; r < RB; ++r) acc[r] = fmaf(-acc[r], inv, p[r]);
-        Rows<PT>::store(J.res, cell, acc);
+    for (int r = 0; r < RB; ++r) acc[r] = 0.f;
+
+    auto band_of = [&](int z, int &xlo, int &w) {
+        xlo = (int)floor(fma(t.slope, (double)z, s_lo)) + 1;
+        w = (int)floor(fma(t.slope, (double)z, s_hi)) + 1 + ncand - xlo;
+    };
+    int xlo, w;
+    band_of(z0, xlo, w);
+    if (w + 1 > FW_W) {  // the width varies by at most one cell between layers
+        for (int z = z0; z < z1; ++z) {
+            const double s = fma(t.slope, (double)z, sbase);
+            const double fl = floor(s);
+            const float frac = (float)(s - fl);
+            const int xf = (int)fl + 1;
+            if (xf >= X || xf + ncand <= 0) continue;
+            const float d0 = fmaf(-a1f, frac, a1m1);
+            const BufSlice<float> vz(vbase + (size_t)z * X * RB, X);
+            for (int c = 0; c < ncand; ++c) {
+                const float wt = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
+                float v[RB];
+                vz.load(xf + c, v);
+#pragma unroll
+                for (int r = 0; r < RB; ++r) acc[r] = fmaf(wt, v[r], acc[r]);
+            }
+        }
     } else {
-        const size_t cell = (((size_t)zc * J.nyb + yb) * J.P + th) * X + i;
-        Rows<float>::store(J.part, cell, acc);
+        constexpr int PER = (FW_W + FW_PIX - 1) / FW_PIX;
+        float4 pre[4][PER];
+        auto fetch = [&](int z, int xl, int wl) {
+            const BufSlice<float> vz(vbase + (size_t)z * X * RB, X);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int c = lane + k * FW_PIX;
+                    if (c < wl) {
+                        const auto raw = __builtin_amdgcn_raw_buffer_load_b128(vz.r[q], (xl + c) * 16, 0, 0);
+                        pre[q][k] = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]),
+                                                __uint_as_float(raw[2]), __uint_as_float(raw[3]));
+                    }
+                }
+        };
+        auto stash = [&](int wl) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int c = lane + k * FW_PIX;
+                    if (c < wl) band[q][c] = pre[q][k];
+                }
+        };
+        fetch(z0, xlo, w);
+        stash(w);
+        for (int z = z0; z < z1; ++z) {
+            int xln = 0, wn = 0;
+            if (z + 1 < z1) {
+                band_of(z + 1, xln, wn);
+                fetch(z + 1, xln, wn);
+            }
+            __syncthreads();  // band holds layer z
+            const double s = fma(t.slope, (double)z, sbase);
+            const double fl = floor(s);
+            const float frac = (float)(s - fl);
+            const int xf = (int)fl + 1;
+            if (i < X && xf < X && xf + ncand > 0) {
+                const float d0 = fmaf(-a1f, frac, a1m1);
+                const int cl = xf - xlo;
+                for (int c = 0; c < ncand; ++c) {
+                    const float wt = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float4 v = band[q][cl + c];
+                        acc[4 * q] = fmaf(wt, v.x, acc[4 * q]);
+                        acc[4 * q + 1] = fmaf(wt, v.y, acc[4 * q + 1]);
+                        acc[4 * q + 2] = fmaf(wt, v.z, acc[4 * q + 2]);
+                        acc[4 * q + 3] = fmaf(wt, v.w, acc[4 * q + 3]);
+                    }
+                }
+            }
+            __syncthreads();  // every lane is done with layer z
+            if (z + 1 < z1) stash(wn);
+            xlo = xln;
+        }
     }
+    if (i >= X) return;
+    fp_epilogue<PT, FUSED, false>(J, th, zc, yb, i, acc);
 }
 
 unsigned combine_blocks(const JobDesc &h, int ntheta)
@@ -374,22 +541,22 @@ __global__ __launch_bounds__(WG) void k_combine(const LaunchList L, unsigned nbl
     const int i = k % J.X;
     const int th = L.theta0[j] + (int)((k / J.X) % ntheta);
     const int yb = k / ((size_t)J.X * ntheta);
-    const size_t cell = ((size_t)yb * J.P + th) * J.X + i;
-    const size_t zstride = (size_t)J.nyb * J.P * J.X;
+    const size_t slice = ((size_t)yb * J.P + th) * J.X * RB;
+    const size_t zstride = (size_t)J.nyb * J.P * J.X * RB;
     float acc[RB];
-    Rows<float>::load(J.part, cell, acc);
+    Slice<float>::load(J.part + slice, J.X, i, acc);
     for (int zc = 1; zc < J.nzc; ++zc) {
         float v[RB];
-        Rows<float>::load(J.part, cell + zc * zstride, v);
+        Slice<float>::load(J.part + slice + zc * zstride, J.X, i, v);
 #pragma unroll
         for (int r = 0; r < RB; ++r) acc[r] += v[r];
     }
     const float inv = J.inv_rwt[th * J.X + i];
     float p[RB];
-    Rows<PT>::load(J.proj, cell, p);
+    Slice<PT>::load(static_cast<const PT *>(J.proj) + slice, J.X, i, p);
 #pragma unroll
     for (int r = 0; r < RB; ++r) acc[r] = fmaf(-acc[r], inv, p[r]);
-    Rows<PT>::store(J.res, cell, acc);
+    Slice<PT>::store(static_cast<PT *>(J.res) + slice, J.X, i, acc);
 }
 
 // ------------------------------------------------------------------ back
@@ -401,12 +568,17 @@ unsigned back_blocks(const JobDesc &h)
     return nxb * nzb * (unsigned)h.nyb;
 }
 
-// Voxel-driven back-projection: a wave holds 64 consecutive x of one z, a
-// thread RB rows.  MODE selects the update (BackMode).  For BACK_SIRT and
-// BACK_SART the image is the residual, for BACK_BPT the (filtered) stack.
+// Voxel-driven back-projection of one 64 x BP_Z voxel tile of one row block.
+// Per pass, the residual cells the tile can touch for BP_G tilts are staged
+// in LDS as float planes; every thread then gathers its voxels' two taps per
+// tilt from LDS with the exact double geometry.  MODE selects the update
+// (BackMode).  For BACK_SIRT and BACK_SART the image is the residual, for
+// BACK_BPT the (filtered) stack.
 template <typename PT, int MODE>
 __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
 {
+    __shared__ float4 win[BP_G][4][BP_W];
+    __shared__ int gmin_s[BP_G];
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
     const int j = find_job(L, b0);
     const JobDesc &J = *L.job[j];
@@ -417,66 +589,122 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
     lb /= nzb;
     const int xb = lb % nxb;
     const int yb = lb / nxb;
-    const int x = xb * BP_X + (threadIdx.x % BP_X);
-    const int z = zb * BP_Z + (threadIdx.x / BP_X);
-    if (x >= J.X || z >= J.T) return;
     const int X = J.X;
-    const double Xc = (double)((float)x - (float)J.tc[0].ox);
-    const double Zc = (double)((float)z - (float)J.tc[0].oz);
+    const int xl = threadIdx.x % BP_X;
+    const int x = xb * BP_X + xl;
+    const int zt0 = zb * BP_Z + (int)(threadIdx.x / BP_X);  // thread's voxels: zt0 + 4k
+    const double ox = J.tc[0].ox, oz = J.tc[0].oz;
+    const double Xc = (double)((float)x - (float)ox);
+    double Zc[BP_ZT];
+#pragma unroll
+    for (int k = 0; k < BP_ZT; ++k) Zc[k] = (double)((float)(zt0 + 4 * k) - (float)oz);
     const void *img = (MODE == BACK_BPT) ? J.proj : J.res;
+    const PT *imgb = static_cast<const PT *>(img) + (size_t)yb * J.P * X * RB;
     const int th0 = L.theta0[j];
     const int th1 = L.ntheta[j] ? th0 + L.ntheta[j] : J.P;
 
-    float val[RB];
+    float val[BP_ZT][RB];
+    float cs[BP_ZT];
 #pragma unroll
-    for (int r = 0; r < RB; ++r) val[r] = 0.f;
-    float csart = 0.f;
+    for (int k = 0; k < BP_ZT; ++k) {
+        cs[k] = 0.f;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) val[k][r] = 0.f;
+    }
 
-    for (int th = th0; th < th1; ++th) {
-        int g;
-        float dx;
-        exact_tap(J.tc[th], Xc, Zc, g, dx);
-        const bool v0 = (unsigned)g < (unsigned)X;
-        const bool v1 = (unsigned)(g + 1) < (unsigned)X;
-        const float w0 = v0 ? 1.f - dx : 0.f;
-        const float w1 = v1 ? dx : 0.f;
-        const size_t row = ((size_t)yb * J.P + th) * X;
-        float r0[RB], r1[RB];
-        Rows<PT>::load(img, row + (v0 ? g : 0), r0);
-        Rows<PT>::load(img, row + (v1 ? g + 1 : 0), r1);
-        if constexpr (MODE == BACK_SIRT) {
+    for (int tg = th0; tg < th1; tg += BP_G) {
+        const int ng = min(BP_G, th1 - tg);
+        __syncthreads();  // the previous pass is done with win / gmin_s
+        if ((int)threadIdx.x < ng) {
+            // lowest tap of the tile: u is monotone in x and z, so a corner
+            const double X0 = (double)((float)(xb * BP_X) - (float)ox);
+            const double Z0 = (double)((float)(zb * BP_Z) - (float)oz);
+            const double Z1 = (double)((float)(zb * BP_Z + BP_Z - 1) - (float)oz);
+            int ga, gb;
+            float d;
+            exact_tap(J.tc[tg + threadIdx.x], X0, Z0, ga, d);
+            exact_tap(J.tc[tg + threadIdx.x], X0, Z1, gb, d);
+            gmin_s[threadIdx.x] = min(ga, gb);
+        }
+        __syncthreads();
+        for (int v = threadIdx.x; v < ng * 4 * BP_W; v += WG) {
+            const int t = v / (4 * BP_W);
+            const int rem = v % (4 * BP_W);
+            const int q = rem / BP_W, c = rem % BP_W;
+            const int g = gmin_s[t] + c;
+            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((unsigned)g < (unsigned)X) {
+                const PT *sl = imgb + (size_t)(tg + t) * X * RB;
+                if constexpr (sizeof(PT) == 4) {
+                    o = *reinterpret_cast<const float4 *>(sl + ((size_t)q * X + g) * 4);
+                } else {
+                    float e[8];
+                    Slice<PT>::unpack(*reinterpret_cast<const uint4 *>(sl + ((size_t)(q / 2) * X + g) * 8), e);
+                    const int h = (q % 2) * 4;
+                    o = make_float4(e[h], e[h + 1], e[h + 2], e[h + 3]);
+                }
+            }
+            win[t][q][c] = o;
+        }
+        __syncthreads();
+        for (int t = 0; t < ng; ++t) {
+            const TiltConst &tc = J.tc[tg + t];
+            const int gm = gmin_s[t];
 #pragma unroll
-            for (int r = 0; r < RB; ++r) val[r] = fmaf(w1, r1[r], fmaf(w0, r0[r], val[r]));
-        } else {
-            // c = 0; c += 1 - dx; c += dx (BilinearValue order)
-            float c = 0.f;
-            c += w0;
-            c += w1;
-            if constexpr (MODE == BACK_SART) csart = c;
-            if (c > J.thr_upd) {
-                const float ic = 1.0f / c;
+            for (int k = 0; k < BP_ZT; ++k) {
+                int g;
+                float dx;
+                exact_tap(tc, Xc, Zc[k], g, dx);
+                const int c = min(max(g - gm, 0), BP_W - 2);
+                const float w0 = 1.f - dx, w1 = dx;
+                float r0[RB], r1[RB];
 #pragma unroll
-                for (int r = 0; r < RB; ++r) val[r] = fmaf(fmaf(w1, r1[r], w0 * r0[r]), ic, val[r]);
+                for (int q = 0; q < 4; ++q) {
+                    const float4 a = win[t][q][c], bq = win[t][q][c + 1];
+                    r0[4 * q] = a.x; r0[4 * q + 1] = a.y; r0[4 * q + 2] = a.z; r0[4 * q + 3] = a.w;
+                    r1[4 * q] = bq.x; r1[4 * q + 1] = bq.y; r1[4 * q + 2] = bq.z; r1[4 * q + 3] = bq.w;
+                }
+                if constexpr (MODE == BACK_SIRT) {
+#pragma unroll
+                    for (int r = 0; r < RB; ++r) val[k][r] = fmaf(w1, r1[r], fmaf(w0, r0[r], val[k][r]));
+                } else {
+                    // c = 0; c += 1 - dx; c += dx over the in-bounds taps (BilinearValue order)
+                    float cw = 0.f;
+                    cw += ((unsigned)g < (unsigned)X) ? w0 : 0.f;
+                    cw += ((unsigned)(g + 1) < (unsigned)X) ? w1 : 0.f;
+                    if constexpr (MODE == BACK_SART) cs[k] = cw;
+                    if (cw > J.thr_upd) {
+                        const float ic = 1.0f / cw;
+#pragma unroll
+                        for (int r = 0; r < RB; ++r) val[k][r] = fmaf(fmaf(w1, r1[r], w0 * r0[r]), ic, val[k][r]);
+                    }
+                }
             }
         }
     }
 
-    float *vp = J.vol + (((size_t)yb * J.T + z) * X + x) * RB;
-    float v[RB];
-    Rows<float>::load(vp, 0, v);
-    if constexpr (MODE == BACK_SIRT) {
-        const float f = J.gamma * J.inv_c[z * X + x];
+    if (x >= X) return;
 #pragma unroll
-        for (int r = 0; r < RB; ++r) v[r] = fmaf(val[r], f, v[r]);
-    } else if constexpr (MODE == BACK_SART) {
-        if (!(csart > J.thr_upd)) return;
+    for (int k = 0; k < BP_ZT; ++k) {
+        const int z = zt0 + 4 * k;
+        if (z >= J.T) continue;
+        float *vslice = J.vol + ((size_t)yb * J.T + z) * X * RB;
+        float v[RB];
+        Slice<float>::load(vslice, X, x, v);
+        if constexpr (MODE == BACK_SIRT) {
+            const float f = J.gamma * J.inv_c[z * X + x];
 #pragma unroll
-        for (int r = 0; r < RB; ++r) v[r] = fmaf(val[r], J.gamma, v[r]);
-    } else {
+            for (int r = 0; r < RB; ++r) v[r] = fmaf(val[k][r], f, v[r]);
+        } else if constexpr (MODE == BACK_SART) {
+            if (!(cs[k] > J.thr_upd)) continue;
 #pragma unroll
-        for (int r = 0; r < RB; ++r) v[r] += val[r];
+            for (int r = 0; r < RB; ++r) v[r] = fmaf(val[k][r], J.gamma, v[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RB; ++r) v[r] += val[k][r];
+        }
+        Slice<float>::store(vslice, X, x, v);
     }
-    Rows<float>::store(vp, 0, v);
 }
 
 // ------------------------------------------------------------------ R-weighting
@@ -711,7 +939,7 @@ __global__ __launch_bounds__(WG) void k_unblock_proj(const float *__restrict__ s
     const int x = k % X;
     const int y = (k / X) % Y;
     const int th = k / ((size_t)X * Y);
-    dst[k] = src[(((size_t)(y / RB) * P + th) * X + x) * RB + (y % RB)];
+    dst[k] = src[((size_t)(y / RB) * P + th) * X * RB + Slice<float>::index(X, x, y % RB)];
 }
 
 // [P][X] -> [P][Y][X] (the same row for every y)
@@ -807,16 +1035,23 @@ hipError_t launch_weights(const JobDesc &h, const JobDesc *d, hipStream_t st)
 hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st)
 {
     // storage < 0 selects the splat of ones (Rwt)
+    static const bool direct = getenv("TREC_FP_DIRECT") != nullptr;
     if (storage < 0) {
         hipLaunchKernelGGL((k_forward<float, false, true>), dim3(nblk), dim3(WG), 0, st, L, nblk);
-    } else if (fused) {
-        switch (storage) {
+    } else if (direct) {
+        switch (fused ? storage : -1) {
         case TREC_STORE_F16: hipLaunchKernelGGL((k_forward<_Float16, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
         case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward<__bf16, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
-        default: hipLaunchKernelGGL((k_forward<float, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
+        case TREC_STORE_F32: hipLaunchKernelGGL((k_forward<float, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
+        default: hipLaunchKernelGGL((k_forward<float, false, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
         }
     } else {
-        hipLaunchKernelGGL((k_forward<float, false, false>), dim3(nblk), dim3(WG), 0, st, L, nblk);
+        switch (fused ? storage : -1) {
+        case TREC_STORE_F16: hipLaunchKernelGGL((k_forward_w<_Float16, true>), dim3(nblk), dim3(FW_PIX), 0, st, L, nblk); break;
+        case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward_w<__bf16, true>), dim3(nblk), dim3(FW_PIX), 0, st, L, nblk); break;
+        case TREC_STORE_F32: hipLaunchKernelGGL((k_forward_w<float, true>), dim3(nblk), dim3(FW_PIX), 0, st, L, nblk); break;
+        default: hipLaunchKernelGGL((k_forward_w<float, false>), dim3(nblk), dim3(FW_PIX), 0, st, L, nblk); break;
+        }
     }
     return hipGetLastError();
 }

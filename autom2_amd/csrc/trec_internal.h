@@ -22,10 +22,24 @@ constexpr int RB = 16;
 
 // Threads per work-group of the row-blocked kernels.
 constexpr int WG = 256;
-// Pixels per forward-projection work-group, voxels per back-projection wave.
+// Pixels per forward-projection work-group; the voxel band the group's
+// pixels see in one z layer is staged in LDS when it fits FP_W cells
+// (255 / a1 + ncand + 1 <= FP_W, i.e. |tilt| up to about 60 degrees).
 constexpr int FP_PIX = 256;
+// The LDS-staged forward projection runs one wave per work-group: 64 pixels,
+// whose band in one layer is at most 64 / a1 + ncand + 1 <= FW_W cells for
+// a1 >= 0.5 (|tilt| <= 60 degrees); wider bands take the direct path.
+constexpr int FW_PIX = 64;
+constexpr int FW_W = 144;
+// Back-projection tile: 64 voxels along x (one wave) by BP_Z along z; each
+// thread carries BP_ZT voxels of one x column (z, z+4, ...).  Residuals of
+// BP_G tilts are staged in LDS per pass, BP_W cells per tilt: with a1 <= 1
+// and |a3| <= 1 the taps of a tile span at most 64 + BP_Z cells.
 constexpr int BP_X = 64;
-constexpr int BP_Z = WG / BP_X;
+constexpr int BP_ZT = 4;
+constexpr int BP_Z = (WG / BP_X) * BP_ZT;
+constexpr int BP_G = 8;
+constexpr int BP_W = BP_X + BP_Z + 4;
 
 // Largest number of jobs one multi-job launch carries.
 constexpr int MAX_JOBS = 64;
@@ -104,6 +118,7 @@ hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipSt
 
 // Work-group counts of the multi-job kernels for one job.
 unsigned forward_blocks(const JobDesc &h, int ntheta);
+unsigned forward_wave_blocks(const JobDesc &h, int ntheta);
 unsigned combine_blocks(const JobDesc &h, int ntheta);
 unsigned back_blocks(const JobDesc &h);
 
