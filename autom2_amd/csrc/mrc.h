@@ -1,0 +1,62 @@
+// mrc.h -- MRC file I/O for the TiltRec CLI (host only).
+//
+// The 1024-byte header has the field layout of TiltRec's MRCheader
+// (src/TiltRec/src/mrcmx/mrcheader.h:24-101, packed), which is the MRC
+// format's own layout.  Reading follows MrcStackM::ReadBlock
+// (src/TiltRec/src/mrcmx/mrcstack.cpp:354-459): data start at 1024 + next,
+// mode 0 is read as signed char (savedate, :255-287), 1 as short, 2 as
+// float; modes 6 (unsigned short) and 12 (half) are read too.  Writing
+// follows InitializeHeader / WriteToFile / UpdateHeader (:14-142).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace trec_cli {
+
+#pragma pack(push, 1)
+struct MrcHeader {
+    int32_t nx, ny, nz, mode;
+    int32_t nxstart, nystart, nzstart;
+    int32_t mx, my, mz;
+    float xlen, ylen, zlen;
+    float alpha, beta, gamma;
+    int32_t mapc, mapr, maps;
+    float amin, amax, amean;
+    int16_t ispg, nsymbt;
+    int32_t next;
+    int16_t creatid;
+    char blank[30];
+    int16_t nint, nreal, sub, zfac;
+    float min2, max2, min3, max3, min4, max4;
+    int16_t idtype, lens, nd1, nd2, vd1, vd2;
+    float tiltangles[6];
+    float xorg, yorg, zorg;
+    char cmap[4];
+    char stamp[4];
+    float rms;
+    int32_t nlabl;
+    char labels[10][80];
+};
+#pragma pack(pop)
+static_assert(sizeof(MrcHeader) == 1024, "MRC header is 1024 bytes");
+
+enum { MRC_BYTE = 0, MRC_SHORT = 1, MRC_FLOAT = 2, MRC_USHORT = 6, MRC_HALF = 12 };
+
+// Reads the header; returns false with a message on failure.
+bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err);
+
+// Reads all sections as float, [nz][ny][nx].
+bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &data, std::string &err);
+
+// InitializeHeader (mrcstack.cpp:57-86) with the given size, mode 2.
+MrcHeader mrc_new_header(int nx, int ny, int nz);
+
+// Writes header and float data ([nz][ny][nx]); then sets amin/amax/amean
+// from section nz/2 as UpdateHeader does (mrcstack.cpp:104-142: amax
+// starts at FLT_MIN, the mean is accumulated in long double) and nlabl = 0.
+bool mrc_write_float(const std::string &path, MrcHeader h, const float *data, std::string &err);
+
+}  // namespace trec_cli

@@ -1,0 +1,174 @@
+// tiltrec_main.cpp -- TiltRec-compatible command-line front end of
+// libtiltrec_hip.so.
+//
+// Installed (or symlinked) as TiltRec-cuda / TiltRecZ-cuda it takes the
+// argv AuTom2's workflow passes (CommandChainManager.qml:161-206 composes
+// "./TiltRec[Z]-cuda --input .. --output .. --tiltfile .. --geometry
+// off,pitch,zshift,thick --method SIRT,iters,relax"), reads the MRC tilt
+// stack and the tilt file, reconstructs on one GPU through the C ABI and
+// writes the MRC volume in the binary's orientation: TiltRec-* (X, T, Y),
+// TiltRecZ-* (X, Y, T) (main-cuda.cpp:266-268, main-cudaz.cpp:172).
+//
+// The reference runs under `mpirun -n N` and does all GPU work on rank 0
+// (main-cuda.cpp:303-348); this program does the same: a process whose
+// launcher rank is not 0 exits 0 at once.  Unlike the reference, every
+// failure exits non-zero (the reference returns 0 after most failures,
+// main-cuda.cpp:406-411), so the workflow sees it.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "mrc.h"
+#include "opts.h"
+#include "tiltrec.h"
+
+using namespace trec_cli;
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int launcher_rank()
+{
+    const char *vars[] = {"OMPI_COMM_WORLD_RANK", "PMI_RANK", "PMIX_RANK", "MV2_COMM_WORLD_RANK"};
+    for (const char *v : vars)
+        if (const char *s = getenv(v)) return atoi(s);
+    return 0;
+}
+
+// ReadAngles (main-cuda.cpp:51-72): floats until the first parse failure.
+bool read_angles(const std::string &path, std::vector<float> &angles)
+{
+    std::ifstream in(path);
+    if (!in.good()) return false;
+    float v;
+    while (in >> v) angles.push_back(v);
+    return true;
+}
+
+std::string base_name(const char *p)
+{
+    std::string s(p);
+    size_t k = s.find_last_of('/');
+    return k == std::string::npos ? s : s.substr(k + 1);
+}
+
+int die(int code, const char *fmt, const char *a = "", const char *b = "")
+{
+    fprintf(stderr, "TiltRec error: ");
+    fprintf(stderr, fmt, a, b);
+    fprintf(stderr, "\n");
+    return code;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    const int rank = launcher_rank();
+    if (rank > 0) {
+        printf("rank %d: the reconstruction runs on rank 0\n", rank);
+        return 0;
+    }
+    Options o;
+    const int r = get_opts(argc, argv, o);
+    if (r == 0) return 0;
+    if (r < 0) return 2;
+
+    const std::string prog = base_name(argv[0]);
+    const bool zbin = prog.rfind("TiltRecZ", 0) == 0;
+    const int layout = (zbin || o.axis == "z" || o.axis == "Z") ? TREC_LAYOUT_Z : TREC_LAYOUT_Y;
+    int rules = TREC_RULES_CUDA;
+    if (o.rules == "mpi" || (o.rules.empty() && prog.find("-mpi") != std::string::npos)) rules = TREC_RULES_MPI;
+    if (o.dry_run) {
+        std::string j = opts_json(o);
+        j.pop_back();
+        printf("%s,\"layout\":\"%s\",\"resolved_rules\":\"%s\"}\n", j.c_str(), layout == TREC_LAYOUT_Z ? "z" : "y",
+               rules == TREC_RULES_MPI ? "mpi" : "cuda");
+        return 0;
+    }
+    if (o.f2b) return die(3, "--f2b (byte output) is not provided; write float and convert afterwards");
+    int kind;
+    if (o.method == "BPT") kind = TREC_BPT;
+    else if (o.method == "WBP") kind = TREC_WBP;
+    else if (o.method == "SIRT") kind = TREC_SIRT;
+    else if (o.method == "SART") kind = TREC_SART;
+    else return die(3, "method %s is not provided", o.method.c_str());
+
+    const double t0 = now_s();
+    MrcHeader hin;
+    std::vector<float> stack;
+    std::string err;
+    if (!mrc_read_float(o.input, hin, stack, err)) return die(4, "%s", err.c_str());
+    std::vector<float> angles;
+    if (!read_angles(o.angle, angles)) return die(4, "cannot read tilt file %s", o.angle.c_str());
+    if ((int)angles.size() < hin.nz)
+        return die(4, "tilt file %s has fewer angles than the stack has projections", o.angle.c_str());
+    angles.resize(hin.nz);
+    const int X = hin.nx, Y = hin.ny, P = hin.nz, T = o.thickness;
+    printf("TiltRec-hip: %d projections of %d x %d -> %d x %d x %d (%s), method %s\n", P, X, Y, X, Y, T,
+           layout == TREC_LAYOUT_Z ? "TiltRecZ orientation" : "TiltRec orientation", o.method.c_str());
+
+    std::vector<float> init;
+    if (!o.initial.empty()) {
+        MrcHeader hi;
+        if (!mrc_read_float(o.initial, hi, init, err)) return die(4, "%s", err.c_str());
+        const bool ok = layout == TREC_LAYOUT_Y ? (hi.nx == X && hi.ny == T && hi.nz == Y)
+                                                : (hi.nx == X && hi.ny == Y && hi.nz == T);
+        if (!ok) return die(4, "initial volume %s does not match the output size", o.initial.c_str());
+    }
+    const double t_read = now_s();
+
+    trec_geom g{X, Y, P, T, o.offset, o.pitch, o.zshift, rules};
+    const int storage = o.storage == "f16" ? TREC_STORE_F16 : (o.storage == "bf16" ? TREC_STORE_BF16 : TREC_STORE_F32);
+    trec_method m{kind, o.iteration, o.gamma, storage, layout};
+    if (hipSetDevice(o.device) != hipSuccess) return die(5, "HIP device %s is not available", std::to_string(o.device).c_str());
+    const size_t nstack = (size_t)P * Y * X, nvol = (size_t)X * Y * T;
+    DevBuf dstack, dvol;
+    if (hipMalloc(&dstack.p, 4 * nstack) != hipSuccess || hipMalloc(&dvol.p, 4 * nvol) != hipSuccess)
+        return die(5, "device allocation failed");
+    if (hipMemcpy(dstack.p, stack.data(), 4 * nstack, hipMemcpyHostToDevice) != hipSuccess)
+        return die(5, "copy to device failed");
+    if (!init.empty() && hipMemcpy(dvol.p, init.data(), 4 * nvol, hipMemcpyHostToDevice) != hipSuccess)
+        return die(5, "copy to device failed");
+    trec_context *ctx = nullptr;
+    trec_status s = trec_context_create(o.device, &ctx);
+    if (s != TREC_OK) return die(5, "context: %s", trec_status_string(s));
+    const double t_up = now_s();
+    s = trec_reconstruct(ctx, &g, angles.data(), (const float *)dstack.p, init.empty() ? nullptr : (const float *)dvol.p,
+                         &m, (float *)dvol.p, nullptr);
+    if (s == TREC_OK && hipDeviceSynchronize() != hipSuccess) s = TREC_ERR_DEVICE;
+    if (s != TREC_OK) {
+        std::string msg = std::string(trec_status_string(s)) + ": " + trec_last_error(ctx);
+        trec_context_destroy(ctx);
+        return die(6, "reconstruction failed: %s", msg.c_str());
+    }
+    const double t_rec = now_s();
+    trec_context_destroy(ctx);
+    std::vector<float> vol(nvol);
+    if (hipMemcpy(vol.data(), dvol.p, 4 * nvol, hipMemcpyDeviceToHost) != hipSuccess)
+        return die(5, "copy from device failed");
+    MrcHeader hout = layout == TREC_LAYOUT_Y ? mrc_new_header(X, T, Y) : mrc_new_header(X, Y, T);
+    if (!mrc_write_float(o.output, hout, vol.data(), err)) return die(4, "%s", err.c_str());
+    const double t_end = now_s();
+    printf("read %.3f s, upload %.3f s, reconstruct %.3f s, write %.3f s, total %.3f s\n", t_read - t0,
+           t_up - t_read, t_rec - t_up, t_end - t_rec, t_end - t0);
+    return 0;
+}

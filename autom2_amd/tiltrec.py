@@ -291,3 +291,49 @@ class Context:
 
     def trim(self):
         _check(lib().trec_context_trim(self._h), self._h)
+
+
+class Server:
+    """Continuous-batching request loop (trec_server_*).  Stacks and output
+    volumes are host numpy arrays; both are held until wait()."""
+
+    def __init__(self, device=0, budget=0, max_active=8):
+        self._h = ctypes.c_void_p()
+        self._keep = {}
+        _check(lib().trec_server_create(device, budget, max_active, ctypes.byref(self._h)))
+
+    def submit(self, geom, method, angles, stack, out):
+        import numpy as np
+        assert stack.dtype == np.float32 and stack.flags["C_CONTIGUOUS"]
+        assert out.dtype == np.float32 and out.flags["C_CONTIGUOUS"]
+        pa, keep = _angles(angles)
+        req = Request(geom, method, pa, stack.ctypes.data, out.ctypes.data)
+        rid = ctypes.c_uint64()
+        _check(lib().trec_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)))
+        self._keep[rid.value] = (stack, out, keep)
+        return rid.value
+
+    def poll(self, rid):
+        d = ctypes.c_int()
+        _check(lib().trec_server_poll(self._h, rid, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid):
+        r = Result()
+        st = lib().trec_server_wait(self._h, rid, ctypes.byref(r))
+        self._keep.pop(rid, None)
+        _check(st)
+        return {"status": r.status, "submit_s": r.submit_s, "admit_s": r.admit_s, "done_s": r.done_s,
+                "footprint": r.footprint}
+
+    def close(self):
+        if self._h:
+            lib().trec_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

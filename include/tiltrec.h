@@ -208,18 +208,23 @@ trec_status trec_profile_read(trec_context *ctx, const char *kernel,
                               double *total_ms, long long *launches);
 
 /* ---------------------------------------------------------------- serving
- * A server keeps a request loop on its own host thread.  Requests carry
- * host stacks; the server copies each into device memory when it admits
- * it, runs every admitted request one SIRT iteration (or one WBP/BPT pass)
- * per step in shared multi-job launches, and admits new requests between
- * steps while their device footprint fits the budget. */
+ * A server keeps a request loop on its own host thread and one device
+ * arena of hbm_budget bytes.  Requests carry host stacks; the loop admits
+ * queued requests first come, first served while at most max_active are
+ * active and the request's footprint (trec_request_footprint) fits the
+ * arena, copies the admitted stack in, and then advances every active
+ * request by one stage per step (one SIRT iteration, one SART tilt, one
+ * BPT/WBP pass) in shared multi-job launches.  Requests arriving meanwhile
+ * join at the next step.  A request's volume is the same, bit for bit,
+ * whether it ran alone or with others. */
 typedef struct trec_server trec_server;
 
 typedef struct {
     trec_geom geom;
     trec_method method;
     const float *angles_deg; /* host, copied at submit */
-    const float *stack;      /* host [nproj][ny][nx], copied at submit */
+    const float *stack;      /* host [nproj][ny][nx]; must stay valid until
+                                the request is done */
     float *volume_out;       /* host, written when the request completes */
 } trec_request;
 
@@ -231,15 +236,18 @@ typedef struct {
     size_t footprint;  /* device bytes it held */
 } trec_result;
 
-/* hbm_budget = 0 uses 90 % of the device's free memory. */
+/* hbm_budget = 0 uses 90 % of the device's free memory; max_active is at
+ * most 64. */
 trec_status trec_server_create(int device, size_t hbm_budget, int max_active,
                                trec_server **out);
+/* Queues a request; TREC_ERR_BUSY if its footprint exceeds the whole
+ * budget (it could never be admitted). */
 trec_status trec_server_submit(trec_server *srv, const trec_request *req,
                                uint64_t *id_out);
 /* *done = 1 once the request finished (successfully or not). */
 trec_status trec_server_poll(trec_server *srv, uint64_t id, int *done);
-/* Blocks until the request finished and returns its record; the id is
- * released afterwards. */
+/* Blocks until the request finished and returns its record and status;
+ * the id is released afterwards. */
 trec_status trec_server_wait(trec_server *srv, uint64_t id, trec_result *res);
 /* Device bytes a request would hold (the admission footprint). */
 trec_status trec_request_footprint(const trec_geom *geom,
