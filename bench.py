@@ -74,6 +74,95 @@ def cpu_baseline(ang, stack, T, iters, rows):
     return rows * T * X * P * iters / dt, dt
 
 
+def serving_mix(n, seed):
+    """Requests drawn from the batching mix: size {512,1024,2048} px square,
+    tilts {41,61,81} (3, 2, 1.5 degree steps), thickness 128..512, method
+    WBP, SIRT x10 or SIRT x20.  Stacks are random (their content does not
+    change the work); one stack per (size, tilts) is shared."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    steps = {41: 3.0, 61: 2.0, 81: 1.5}
+    reqs, pool = [], {}
+    for _ in range(n):
+        X = int(rng.choice([512, 1024, 2048]))
+        P = int(rng.choice([41, 61, 81]))
+        T = int(rng.integers(128, 513))
+        method = str(rng.choice(["WBP", "SIRT,10,1.0", "SIRT,20,1.0"]))
+        if (X, P) not in pool:
+            pool[(X, P)] = (rng.random((P, X, X), dtype=np.float32) * 50.0)
+        ang = (np.arange(P) * steps[P] - (P - 1) * steps[P] / 2).astype(np.float32)
+        reqs.append(dict(X=X, P=P, T=T, method=method, ang=ang, stack=pool[(X, P)]))
+    return reqs
+
+
+def serve_server(trec, reqs, arrivals, storage):
+    """Continuous-batching server: submit at the arrival times, wait all."""
+    import numpy as np
+    srv = trec.Server(0, budget=0, max_active=16)
+    t0 = time.perf_counter()
+    ids, outs = [], []
+    for r, ta in zip(reqs, arrivals):
+        dt = ta - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        out = np.empty((r["X"], r["T"], r["X"]), np.float32)
+        outs.append(out)
+        ids.append(srv.submit(trec.make_geom(r["X"], r["X"], r["P"], r["T"]),
+                              trec.make_method(r["method"], storage=storage), r["ang"], r["stack"], out))
+    recs = [srv.wait(i) for i in ids]
+    srv.close()
+    lat = [(rec["done_s"] - rec["submit_s"]) * 1e3 for rec in recs]
+    span = max(rec["done_s"] for rec in recs) - min(rec["submit_s"] for rec in recs)
+    return lat, span
+
+
+def serve_sequential(trec, torch, reqs, arrivals):
+    """Plain FP32, one request at a time in arrival order (copy in,
+    reconstruct, copy out)."""
+    ctx = trec.Context(0)
+    t0 = time.perf_counter()
+    lat = []
+    for r, ta in zip(reqs, arrivals):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        s = torch.from_numpy(r["stack"]).cuda()
+        vol = torch.empty((r["X"], r["T"], r["X"]), dtype=torch.float32, device="cuda")
+        ctx.reconstruct(trec.make_geom(r["X"], r["X"], r["P"], r["T"]), r["ang"], s,
+                        trec.make_method(r["method"]), vol)
+        vol.cpu()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0 - ta) * 1e3)
+        del s, vol
+    span = time.perf_counter() - t0 - arrivals[0]
+    ctx.close()
+    torch.cuda.empty_cache()
+    return lat, span
+
+
+def serving_form(trec, torch, n, rate, seed=1004):
+    import numpy as np
+    reqs = serving_mix(n, seed)
+    gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, n)
+    arrivals = list(np.cumsum(gaps) - gaps[0])
+    out = {"requests": n, "arrival_rate_per_s": rate, "storage": "f16"}
+    for mode in ("poisson", "saturated"):
+        arr = arrivals if mode == "poisson" else [0.0] * n
+        log(f"[bench] serving ({mode}): continuous batching, FP16")
+        lat_s, span_s = serve_server(trec, reqs, arr, trec.STORE_F16)
+        log(f"[bench] serving ({mode}): plain FP32 one at a time")
+        lat_q, span_q = serve_sequential(trec, torch, reqs, arr)
+        out[mode] = {
+            "batched_f16": {"series_per_s": round(n / span_s, 3), "p50_ms": round(percentile(lat_s, 0.5), 1),
+                            "p99_ms": round(percentile(lat_s, 0.99), 1)},
+            "plain_f32": {"series_per_s": round(n / span_q, 3), "p50_ms": round(percentile(lat_q, 0.5), 1),
+                          "p99_ms": round(percentile(lat_q, 0.99), 1)},
+            "throughput_ratio": round(span_q / span_s, 3),
+        }
+        log(f"[bench] serving ({mode}): {out[mode]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,7 +172,8 @@ def main():
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=2)
-    ap.add_argument("--no-forms", action="store_true", help="skip the FP16/BF16 form lines")
+    ap.add_argument("--no-forms", action="store_true", help="skip the FP16/BF16 and serving forms")
+    ap.add_argument("--serve-requests", type=int, default=16)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -147,6 +237,11 @@ def main():
             forms[name] = {"ms_per_step": round(fm, 3), "value": updates / (fm / 1e3),
                            "speedup_vs_f32": round(ms_mean / fm, 3)}
             log(f"[bench] {name}: {fm:.2f} ms/step")
+
+    if not args.no_forms:
+        del vol
+        torch.cuda.empty_cache()
+        forms["serving"] = serving_form(trec, torch, args.serve_requests, 2.0)
 
     cpu = None
     if not args.no_cpu:
