@@ -98,7 +98,7 @@ def serving_mix(n, seed):
 def serve_server(trec, reqs, arrivals, storage):
     """Continuous-batching server: submit at the arrival times, wait all."""
     import numpy as np
-    srv = trec.Server(0, budget=0, max_active=16)
+    srv = trec.Server(0, budget=96 << 30, max_active=16)
     t0 = time.perf_counter()
     ids, outs = [], []
     for r, ta in zip(reqs, arrivals):
@@ -163,6 +163,43 @@ def serving_form(trec, torch, n, rate, seed=1004):
     return out
 
 
+def counters_run(iters):
+    """The run the counter profiles measure: `iters` SIRT iterations of the
+    bench workload through the library, then a 1 GiB device copy whose
+    FETCH_SIZE / WRITE_SIZE calibrate the HBM byte counters."""
+    import torch
+    from autom2_amd import synth, tiltrec as trec
+    c = synth.CONFIGS["sirt1024"]
+    ang, stack, _, _ = synth.config_series("sirt1024")
+    s = torch.from_numpy(stack).cuda()
+    g = trec.make_geom(c["X"], c["Y"], c["P"], c["T"])
+    ctx = trec.Context(0)
+    vol = torch.empty((c["Y"], c["T"], c["X"]), device="cuda")
+    ctx.reconstruct(g, ang, s, trec.make_method(trec.SIRT, iters, 1.0), vol)
+    torch.cuda.synchronize()
+    a = torch.empty(256 * 1024 * 1024, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("[bench] counters run done")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (profiles/sirt1024_f32_counters.md), or None."""
+    path = os.path.join(ROOT, "profiles", "sirt1024_f32_counters.md")
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 5 and cells[1].startswith(f"`k_{kernel}"):
+                return (float(cells[3]) + float(cells[5])) * 1e9
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,7 +211,11 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=2)
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16/BF16 and serving forms")
     ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--counters", action="store_true", help="only the run the counter profiles measure")
     args = ap.parse_args()
+    if args.counters:
+        counters_run(2)
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -224,8 +265,10 @@ def main():
         alg["forward"] = 4 * V + 8 * PA
     dom = max(("forward", "back"), key=lambda k: kern[k]["ms_total"])
     achieved = alg[dom] / (kern[dom]["ms_avg"] / 1e3) / 1e9
+    kname = {"forward": "forward_w<float, false>", "back": "back<float, 0>"}[dom]
+    traffic = profiled_traffic(kname) if args.config == "sirt1024" else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "bytes_per_launch": alg[dom], "ms_per_launch": round(kern[dom]["ms_avg"], 4),
                 "kernels_ms": {k: round(v["ms_avg"], 4) for k, v in kern.items()}}
 
