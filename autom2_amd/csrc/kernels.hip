@@ -45,6 +45,19 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n)
     return xcd * q + (xcd < r ? xcd : r) + k;
 }
 
+// Job and tilt tables are read-only while a kernel runs.  Reading them
+// through the constant address space turns uniform fields into scalar loads:
+// they do not wait on the vector-memory counter (so they do not serialise a
+// wave's prefetches) and live in SGPRs.
+#define TREC_KCONST __attribute__((address_space(4)))
+using KJob = const TREC_KCONST JobDesc;
+using KTilt = const TREC_KCONST TiltConst;
+template <typename T>
+__device__ __forceinline__ const TREC_KCONST T *kconst(const T *p)
+{
+    return (const TREC_KCONST T *)p;
+}
+
 // Locates the job that owns work-group `b` of a multi-job launch.
 __device__ __forceinline__ int find_job(const LaunchList &L, unsigned b)
 {
@@ -136,7 +149,8 @@ struct BufSlice {
 // Exact TiltRec-mpi geometry (ValCoef, main-mpi.cpp:204-226) at pitch 0:
 // u = ((a0 + a1 * X) + a3 * Z) + ox in double, taps floor(u) and floor(u)+1,
 // fraction rounded to float as Weight::x_min_del is.
-__device__ __forceinline__ void exact_tap(const TiltConst &t, double Xc, double Zc, int &g, float &dx)
+template <typename TC>
+__device__ __forceinline__ void exact_tap(const TC &t, double Xc, double Zc, int &g, float &dx)
 {
     double u = __dadd_rn(__dadd_rn(__dadd_rn(t.a0, __dmul_rn(t.a1, Xc)), __dmul_rn(t.a3, Zc)), t.ox);
     double f = floor(u);
@@ -298,8 +312,8 @@ unsigned forward_blocks(const JobDesc &h, int ntheta)
 
 // Writes a forward-projection result: the ray weights (ONES), the residual
 // p - R / Rwt (FUSED), or the z-chunk partial.
-template <typename PT, bool FUSED, bool ONES, int NA>
-__device__ __forceinline__ void fp_epilogue(const JobDesc &J, int th, int zc, int yb, int i, float (&acc)[NA])
+template <typename PT, bool FUSED, bool ONES, int NA, typename JD>
+__device__ __forceinline__ void fp_epilogue(const JD &J, int th, int zc, int yb, int i, float (&acc)[NA])
 {
     const int X = J.X;
     if constexpr (ONES) {
@@ -333,7 +347,7 @@ __global__ __launch_bounds__(WG) void k_forward(const LaunchList L, unsigned nbl
 {
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
     const int j = find_job(L, b0);
-    const JobDesc &J = *L.job[j];
+    KJob &J = *kconst(L.job[j]);
     unsigned lb = b0 - L.off[j];
     const int ntheta = L.ntheta[j] ? L.ntheta[j] : J.P;
     const unsigned npb = (J.X + FP_PIX - 1) / FP_PIX;
@@ -345,7 +359,7 @@ __global__ __launch_bounds__(WG) void k_forward(const LaunchList L, unsigned nbl
     const int yb = lb / J.nzc;
     const int i = pb * FP_PIX + threadIdx.x;
     if (i >= J.X) return;
-    const TiltConst &t = J.tc[th];
+    KTilt &t = kconst(J.tc)[th];
     const int X = J.X;
     const int z0 = zc * J.zc_len;
     const int z1 = min(J.T, z0 + J.zc_len);
@@ -397,56 +411,76 @@ unsigned forward_wave_blocks(const JobDesc &h, int ntheta)
 // z layer staged in LDS.  The wave's 64 pixels of tilt t see, in layer z,
 // the voxels [xlo(z), xlo(z) + w(z)); the wave loads that band once (plane
 // by plane, fully coalesced; layer z + 1 goes to registers while layer z is
-// summed out of LDS) and every pixel then reads its ncand candidates from
-// LDS.  Voxels outside [0, X) load as 0.  A band wider than FW_W (tilts past
-// about 60 degrees) is gathered straight from L2 instead.
-template <typename PT, bool FUSED>
-__global__ __launch_bounds__(FW_PIX) void k_forward_w(const LaunchList L, unsigned nblk)
+// summed out of LDS) and every pixel then reads its NC candidates from LDS
+// (NC = ncand as a compile-time count; 0 = run-time loop).  Voxels outside
+// [0, X) load as 0.  A band wider than FW_W (tilts past about 60 degrees) is
+// gathered straight from L2 instead.
+//
+// Every lane's candidates lie inside the band: xf(i) is monotone in i and the
+// band is bounded by the first and last pixel's windows computed with the
+// same expressions, so 0 <= xf - xlo and xf - xlo + ncand <= w.
+#ifndef FW_WAVES
+#define FW_WAVES 4  // waves per SIMD the register budget is sized for
+#endif
+
+template <int NC>
+__device__ __forceinline__ void fw_gather(const float4 (*band)[FW_W], int cl, int ncand, float d0, float a1f,
+                                          float (&acc)[RB])
 {
-    __shared__ float4 band[4][FW_W];
-    const unsigned b0 = xcd_remap(blockIdx.x, nblk);
-    const int j = find_job(L, b0);
-    const JobDesc &J = *L.job[j];
-    unsigned lb = b0 - L.off[j];
-    const int ntheta = L.ntheta[j] ? L.ntheta[j] : J.P;
-    const unsigned npb = (J.X + FW_PIX - 1) / FW_PIX;
-    const int pb = lb % npb;
-    lb /= npb;
-    const int th = L.theta0[j] + (int)(lb % ntheta);
-    lb /= ntheta;
-    const int zc = lb % J.nzc;
-    const int yb = lb / J.nzc;
+    const int n = NC ? NC : ncand;
+#pragma unroll
+    for (int c = 0; c < n; ++c) {
+        // candidates are read two at a time: the compiler would otherwise
+        // hoist every read of the layer and run out of registers
+        if (c > 0 && c % 2 == 0) __builtin_amdgcn_sched_barrier(0);
+        const float wt = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 v = band[q][cl + c];
+            acc[4 * q] = fmaf(wt, v.x, acc[4 * q]);
+            acc[4 * q + 1] = fmaf(wt, v.y, acc[4 * q + 1]);
+            acc[4 * q + 2] = fmaf(wt, v.z, acc[4 * q + 2]);
+            acc[4 * q + 3] = fmaf(wt, v.w, acc[4 * q + 3]);
+        }
+    }
+}
+
+template <typename PT, bool FUSED, int NC>
+__device__ __forceinline__ void fw_body(KJob &J, int th, int zc, int yb, int i0, float4 (*band)[FW_W])
+{
     const int lane = threadIdx.x;
-    const int i0 = pb * FW_PIX;
     const int i = i0 + lane;
-    const TiltConst &t = J.tc[th];
+    KTilt &t = kconst(J.tc)[th];
     const int X = J.X;
     const int z0 = zc * J.zc_len;
     const int z1 = min(J.T, z0 + J.zc_len);
-    const double sbase = ((double)(i - 1) - t.c0) * t.inv_a1;
-    const double s_lo = ((double)(i0 - 1) - t.c0) * t.inv_a1;
-    const double s_hi = ((double)(i0 + FW_PIX - 2) - t.c0) * t.inv_a1;
+    const double slope = t.slope, inv_a1 = t.inv_a1, c0 = t.c0;
     const float a1f = t.a1f, a1m1 = t.a1f - 1.0f;
-    const int ncand = t.ncand;
+    const int ncand = NC ? NC : t.ncand;
+    const double sbase = ((double)(i - 1) - c0) * inv_a1;
+    const double s_lo = ((double)(i0 - 1) - c0) * inv_a1;
+    const double s_hi = ((double)(i0 + FW_PIX - 2) - c0) * inv_a1;
     const float *vbase = J.vol + (size_t)yb * J.T * X * RB;
 
     float acc[RB];
 #pragma unroll
     for (int r = 0; r < RB; ++r) acc[r] = 0.f;
 
+    // first cell and width of the band of layer z (wave-uniform)
     auto band_of = [&](int z, int &xlo, int &w) {
-        xlo = (int)floor(fma(t.slope, (double)z, s_lo)) + 1;
-        w = (int)floor(fma(t.slope, (double)z, s_hi)) + 1 + ncand - xlo;
+        const int lo = (int)floor(fma(slope, (double)z, s_lo)) + 1;
+        const int hi = (int)floor(fma(slope, (double)z, s_hi)) + 1 + ncand;
+        xlo = __builtin_amdgcn_readfirstlane(lo);
+        w = __builtin_amdgcn_readfirstlane(hi - lo);
     };
     int xlo, w;
     band_of(z0, xlo, w);
-    if (w + 1 > FW_W) {  // the width varies by at most one cell between layers
+    if (w + 2 > FW_W) {  // floors of rounded ends: the width varies by at most two cells between layers
         for (int z = z0; z < z1; ++z) {
-            const double s = fma(t.slope, (double)z, sbase);
+            const double s = fma(slope, (double)z, sbase);
             const double fl = floor(s);
             const float frac = (float)(s - fl);
             const int xf = (int)fl + 1;
-            if (xf >= X || xf + ncand <= 0) continue;
             const float d0 = fmaf(-a1f, frac, a1m1);
             const BufSlice<float> vz(vbase + (size_t)z * X * RB, X);
             for (int c = 0; c < ncand; ++c) {
@@ -459,65 +493,90 @@ __global__ __launch_bounds__(FW_PIX) void k_forward_w(const LaunchList L, unsign
         }
     } else {
         constexpr int PER = (FW_W + FW_PIX - 1) / FW_PIX;
+        constexpr int TAIL = FW_W - (PER - 1) * FW_PIX;  // lanes of the last chunk inside the band
         float4 pre[4][PER];
-        auto fetch = [&](int z, int xl, int wl) {
+        // chunks k < kn of the band are loaded; cells past w are loaded too
+        // (in bounds of the buffer or zero) but never read
+        auto fetch = [&](int z, int xl, int kn) {
             const BufSlice<float> vz(vbase + (size_t)z * X * RB, X);
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int k = 0; k < PER; ++k) {
+                if (k < kn) {
 #pragma unroll
-                for (int k = 0; k < PER; ++k) {
-                    const int c = lane + k * FW_PIX;
-                    if (c < wl) {
-                        const auto raw = __builtin_amdgcn_raw_buffer_load_b128(vz.r[q], (xl + c) * 16, 0, 0);
+                    for (int q = 0; q < 4; ++q) {
+                        const auto raw =
+                            __builtin_amdgcn_raw_buffer_load_b128(vz.r[q], (xl + lane + k * FW_PIX) * 16, 0, 0);
                         pre[q][k] = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]),
                                                 __uint_as_float(raw[2]), __uint_as_float(raw[3]));
                     }
                 }
+            }
         };
-        auto stash = [&](int wl) {
+        auto stash = [&](int kn) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int k = 0; k < PER; ++k) {
+                if (k < kn && (k < PER - 1 || lane < TAIL)) {
 #pragma unroll
-                for (int k = 0; k < PER; ++k) {
-                    const int c = lane + k * FW_PIX;
-                    if (c < wl) band[q][c] = pre[q][k];
+                    for (int q = 0; q < 4; ++q) band[q][lane + k * FW_PIX] = pre[q][k];
                 }
+            }
         };
-        fetch(z0, xlo, w);
-        stash(w);
+        auto chunks = [](int wl) { return (wl + FW_PIX - 1) / FW_PIX; };
+        fetch(z0, xlo, chunks(w));
+        stash(chunks(w));
         for (int z = z0; z < z1; ++z) {
             int xln = 0, wn = 0;
             if (z + 1 < z1) {
                 band_of(z + 1, xln, wn);
-                fetch(z + 1, xln, wn);
+                fetch(z + 1, xln, chunks(wn));
             }
             __syncthreads();  // band holds layer z
-            const double s = fma(t.slope, (double)z, sbase);
+            const double s = fma(slope, (double)z, sbase);
             const double fl = floor(s);
             const float frac = (float)(s - fl);
-            const int xf = (int)fl + 1;
-            if (i < X && xf < X && xf + ncand > 0) {
-                const float d0 = fmaf(-a1f, frac, a1m1);
-                const int cl = xf - xlo;
-                for (int c = 0; c < ncand; ++c) {
-                    const float wt = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
+            const int cl = (int)fl + 1 - xlo;
+            const float d0 = fmaf(-a1f, frac, a1m1);
+            fw_gather<NC>(band, cl, ncand, d0, a1f, acc);
+            // keep the sums of layer z ahead of the wait for layer z + 1's
+            // loads (they are what hides that wait): the empty asm needs the
+            // accumulators here, so the compiler cannot sink the FMAs past
+            // the stash
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const float4 v = band[q][cl + c];
-                        acc[4 * q] = fmaf(wt, v.x, acc[4 * q]);
-                        acc[4 * q + 1] = fmaf(wt, v.y, acc[4 * q + 1]);
-                        acc[4 * q + 2] = fmaf(wt, v.z, acc[4 * q + 2]);
-                        acc[4 * q + 3] = fmaf(wt, v.w, acc[4 * q + 3]);
-                    }
-                }
-            }
+            for (int r = 0; r < RB; ++r) asm volatile("" : "+v"(acc[r]));
             __syncthreads();  // every lane is done with layer z
-            if (z + 1 < z1) stash(wn);
+            if (z + 1 < z1) stash(chunks(wn));
             xlo = xln;
         }
     }
     if (i >= X) return;
     fp_epilogue<PT, FUSED, false>(J, th, zc, yb, i, acc);
+}
+
+template <typename PT, bool FUSED>
+__global__ __launch_bounds__(FW_PIX) __attribute__((amdgpu_waves_per_eu(FW_WAVES))) void k_forward_w(
+    const LaunchList L, unsigned nblk)
+{
+    __shared__ float4 band[4][FW_W];
+    const unsigned b0 = xcd_remap(blockIdx.x, nblk);
+    const int j = find_job(L, b0);
+    KJob &J = *kconst(L.job[j]);
+    unsigned lb = b0 - L.off[j];
+    const int ntheta = L.ntheta[j] ? L.ntheta[j] : J.P;
+    const unsigned npb = (J.X + FW_PIX - 1) / FW_PIX;
+    const int pb = lb % npb;
+    lb /= npb;
+    const int th = L.theta0[j] + (int)(lb % ntheta);
+    lb /= ntheta;
+    const int zc = lb % J.nzc;
+    const int yb = lb / J.nzc;
+    const int i0 = pb * FW_PIX;
+    switch (kconst(J.tc)[th].ncand) {
+    case 2: fw_body<PT, FUSED, 2>(J, th, zc, yb, i0, band); break;
+    case 3: fw_body<PT, FUSED, 3>(J, th, zc, yb, i0, band); break;
+    case 4: fw_body<PT, FUSED, 4>(J, th, zc, yb, i0, band); break;
+    case 5: fw_body<PT, FUSED, 5>(J, th, zc, yb, i0, band); break;
+    default: fw_body<PT, FUSED, 0>(J, th, zc, yb, i0, band); break;
+    }
 }
 
 unsigned combine_blocks(const JobDesc &h, int ntheta)
@@ -532,7 +591,7 @@ __global__ __launch_bounds__(WG) void k_combine(const LaunchList L, unsigned nbl
 {
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
     const int j = find_job(L, b0);
-    const JobDesc &J = *L.job[j];
+    KJob &J = *kconst(L.job[j]);
     const unsigned lb = b0 - L.off[j];
     const int ntheta = L.ntheta[j] ? L.ntheta[j] : J.P;
     const size_t k = (size_t)lb * WG + threadIdx.x;
@@ -581,7 +640,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
     __shared__ int gmin_s[BP_G];
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
     const int j = find_job(L, b0);
-    const JobDesc &J = *L.job[j];
+    KJob &J = *kconst(L.job[j]);
     unsigned lb = b0 - L.off[j];
     const unsigned nzb = (J.T + BP_Z - 1) / BP_Z;
     const unsigned nxb = (J.X + BP_X - 1) / BP_X;
@@ -593,7 +652,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
     const int xl = threadIdx.x % BP_X;
     const int x = xb * BP_X + xl;
     const int zt0 = zb * BP_Z + (int)(threadIdx.x / BP_X);  // thread's voxels: zt0 + 4k
-    const double ox = J.tc[0].ox, oz = J.tc[0].oz;
+    const double ox = kconst(J.tc)[0].ox, oz = kconst(J.tc)[0].oz;
     const double Xc = (double)((float)x - (float)ox);
     double Zc[BP_ZT];
 #pragma unroll
@@ -622,7 +681,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
             const double Z1 = (double)((float)(zb * BP_Z + BP_Z - 1) - (float)oz);
             int ga, gb;
             float d;
-            exact_tap(J.tc[tg + threadIdx.x], X0, Z0, ga, d);
+            exact_tap(J.tc[tg + threadIdx.x], X0, Z0, ga, d);  // per-lane tilt: vector loads
             exact_tap(J.tc[tg + threadIdx.x], X0, Z1, gb, d);
             gmin_s[threadIdx.x] = min(ga, gb);
         }
@@ -648,7 +707,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
         }
         __syncthreads();
         for (int t = 0; t < ng; ++t) {
-            const TiltConst &tc = J.tc[tg + t];
+            KTilt &tc = kconst(J.tc)[tg + t];
             const int gm = gmin_s[t];
 #pragma unroll
             for (int k = 0; k < BP_ZT; ++k) {
