@@ -391,7 +391,10 @@ size_t carve(JobPlan &p, char *base)
         p.d = (JobDesc *)take(sizeof(JobDesc));
         p.d_ones = (JobDesc *)take(sizeof(JobDesc));
         h.tc = (TiltConst *)take(sizeof(TiltConst) * P);
-        h.vol = (float *)take(4 * nyb * T * X * RB);
+        // the forward projection's band loads may run up to FGS cells
+        // (16 bytes each) past a slice's last plane; the tail keeps even the
+        // last slice's overrun inside the workspace
+        h.vol = (float *)take(4 * nyb * T * X * RB + 16 * FGS);
         h.proj = take(es * nyb * P * X * RB);
         h.res = is_iterative(k) ? take(es * nyb * P * X * RB) : nullptr;
         h.part = (k == TREC_SIRT && h.nzc > 1) ? (float *)take(4 * (size_t)h.nzc * nyb * P * X * RB) : nullptr;
@@ -493,8 +496,7 @@ trec_status upload(trec_context *ctx, const std::vector<JobPlan *> &plans, hipSt
 // Work-groups of a value forward projection (the kernel launch_forward picks).
 unsigned fwd_blocks(const JobDesc &h, int ntheta)
 {
-    static const bool direct = getenv("TREC_FP_DIRECT") != nullptr;
-    return direct ? forward_blocks(h, ntheta) : forward_wave_blocks(h, ntheta);
+    return fwd_value_blocks(h, ntheta);
 }
 
 hipError_t issue_prep(JobPlan &p, hipStream_t st)

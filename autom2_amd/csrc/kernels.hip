@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "trec_internal.h"
 
@@ -423,8 +424,8 @@ unsigned forward_wave_blocks(const JobDesc &h, int ntheta)
 #define FW_WAVES 4  // waves per SIMD the register budget is sized for
 #endif
 
-template <int NC>
-__device__ __forceinline__ void fw_gather(const float4 (*band)[FW_W], int cl, int ncand, float d0, float a1f,
+template <int NC, int S>
+__device__ __forceinline__ void fw_gather(const float4 (*band)[S], int cl, int ncand, float d0, float a1f,
                                           float (&acc)[RB])
 {
     const int n = NC ? NC : ncand;
@@ -446,7 +447,7 @@ __device__ __forceinline__ void fw_gather(const float4 (*band)[FW_W], int cl, in
 }
 
 template <typename PT, bool FUSED, int NC>
-__device__ __forceinline__ void fw_body(KJob &J, int th, int zc, int yb, int i0, float4 (*band)[FW_W])
+__device__ __forceinline__ void fw_body(KJob &J, int th, int zc, int yb, int i0, float4 (*band)[FW_S])
 {
     const int lane = threadIdx.x;
     const int i = i0 + lane;
@@ -473,9 +474,12 @@ __device__ __forceinline__ void fw_body(KJob &J, int th, int zc, int yb, int i0,
         xlo = __builtin_amdgcn_readfirstlane(lo);
         w = __builtin_amdgcn_readfirstlane(hi - lo);
     };
+    // band loads per layer: ncand = 2, 3, 4 means a1 >= 1, 2/3, 1/2, so
+    // w <= floor(63 / a1) + 1 + ncand <= 66, 98, 131 cells
+    constexpr int KN = NC == 2 ? 5 : NC == 3 ? 7 : (FW_W + FW_G - 1) / FW_G;
     int xlo, w;
     band_of(z0, xlo, w);
-    if (w + 2 > FW_W) {  // floors of rounded ends: the width varies by at most two cells between layers
+    if (w + 2 > KN * FW_G) {  // floors of rounded ends: the width varies by at most two cells between layers
         for (int z = z0; z < z1; ++z) {
             const double s = fma(slope, (double)z, sbase);
             const double fl = floor(s);
@@ -492,36 +496,38 @@ __device__ __forceinline__ void fw_body(KJob &J, int th, int zc, int yb, int i0,
             }
         }
     } else {
-        constexpr int PER = (FW_W + FW_PIX - 1) / FW_PIX;
-        constexpr int TAIL = FW_W - (PER - 1) * FW_PIX;  // lanes of the last chunk inside the band
-        float4 pre[4][PER];
-        // chunks k < kn of the band are loaded; cells past w are loaded too
-        // (in bounds of the buffer or zero) but never read
+        // The band is loaded 16 cells at a time: lane l of load m carries
+        // plane l / 16 of cell 16 m + l % 16, so a load moves 1 KB as four
+        // 256-byte runs (neighbouring lanes stay in one cache line) and the
+        // stores of 16 lanes fill one 256-byte bank row.  Cells past w are
+        // loaded too (in bounds or zero) but never read.
+        constexpr int NL = KN;
+        float4 pre[NL];
+        const int lq = lane >> 4, lc = lane & 15;
+        const unsigned slice_bytes = (unsigned)X * RB * 4;
+        // All NL loads are always issued (any past the band with an
+        // out-of-range offset, which returns zeros without touching memory):
+        // a fixed count of loads in flight lets each store wait for exactly
+        // its own load.
         auto fetch = [&](int z, int xl, int kn) {
-            const BufSlice<float> vz(vbase + (size_t)z * X * RB, X);
+            const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)(vbase + (size_t)z * X * RB), (short)0,
+                                                              (int)slice_bytes, BUF_DWORD3);
 #pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                if (k < kn) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const auto raw =
-                            __builtin_amdgcn_raw_buffer_load_b128(vz.r[q], (xl + lane + k * FW_PIX) * 16, 0, 0);
-                        pre[q][k] = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]),
-                                                __uint_as_float(raw[2]), __uint_as_float(raw[3]));
-                    }
-                }
+            for (int m = 0; m < NL; ++m) {
+                const int x = xl + m * FW_G + lc;
+                const bool live = m < kn && (unsigned)x < (unsigned)X;
+                const unsigned off = live ? ((unsigned)lq * X + x) * 16u : slice_bytes;
+                const auto raw = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
+                pre[m] = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]), __uint_as_float(raw[2]),
+                                     __uint_as_float(raw[3]));
             }
         };
         auto stash = [&](int kn) {
 #pragma unroll
-            for (int k = 0; k < PER; ++k) {
-                if (k < kn && (k < PER - 1 || lane < TAIL)) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) band[q][lane + k * FW_PIX] = pre[q][k];
-                }
-            }
+            for (int m = 0; m < NL; ++m)
+                if (m < kn) band[lq][m * FW_G + lc] = pre[m];
         };
-        auto chunks = [](int wl) { return (wl + FW_PIX - 1) / FW_PIX; };
+        auto chunks = [](int wl) { return (wl + FW_G - 1) / FW_G; };
         fetch(z0, xlo, chunks(w));
         stash(chunks(w));
         for (int z = z0; z < z1; ++z) {
@@ -536,7 +542,7 @@ __device__ __forceinline__ void fw_body(KJob &J, int th, int zc, int yb, int i0,
             const float frac = (float)(s - fl);
             const int cl = (int)fl + 1 - xlo;
             const float d0 = fmaf(-a1f, frac, a1m1);
-            fw_gather<NC>(band, cl, ncand, d0, a1f, acc);
+            fw_gather<NC, FW_S>(band, cl, ncand, d0, a1f, acc);
             // keep the sums of layer z ahead of the wait for layer z + 1's
             // loads (they are what hides that wait): the empty asm needs the
             // accumulators here, so the compiler cannot sink the FMAs past
@@ -556,7 +562,7 @@ template <typename PT, bool FUSED>
 __global__ __launch_bounds__(FW_PIX) __attribute__((amdgpu_waves_per_eu(FW_WAVES))) void k_forward_w(
     const LaunchList L, unsigned nblk)
 {
-    __shared__ float4 band[4][FW_W];
+    __shared__ float4 band[4][FW_S];
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
     const int j = find_job(L, b0);
     KJob &J = *kconst(L.job[j]);
@@ -577,6 +583,224 @@ __global__ __launch_bounds__(FW_PIX) __attribute__((amdgpu_waves_per_eu(FW_WAVES
     case 5: fw_body<PT, FUSED, 5>(J, th, zc, yb, i0, band); break;
     default: fw_body<PT, FUSED, 0>(J, th, zc, yb, i0, band); break;
     }
+}
+
+// ------------------------------------------------------------------ forward, tilt groups
+
+// Uniform double in scalar registers.
+__device__ __forceinline__ double uniform_d(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+unsigned forward_group_blocks(const JobDesc &h, int ntheta)
+{
+    const unsigned npb = (h.X + FW_PIX - 1) / FW_PIX;
+    const unsigned ntg = ((unsigned)ntheta + FG - 1) / FG;
+    return npb * ntg * (unsigned)h.nzc * (unsigned)h.nyb;
+}
+
+// Forward projection of FG neighbouring tilts per work-group, one wave per
+// tilt, for the same 64 pixels, row block and z chunk.  Neighbouring tilts
+// see nearly the same voxels of a layer, so the group stages the union of
+// their bands in LDS once (loaded by all four waves, 16 cells a time) and
+// each wave gathers its own candidates from it; this divides the loads and
+// LDS stores per tilt by up to FG.  Unions wider than the LDS band (tilts
+// past about 60 degrees far from the tilt axis) are gathered straight from
+// L2 by every wave instead.  Per pixel, the sums and their order are those
+// of k_forward and k_forward_w.
+//
+// The union of layer z is [ulo, ulo + uw) with ulo the least first cell and
+// ulo + uw the greatest end over the group's tilts, each computed with the
+// expressions the pixels use, so every lane's candidates lie inside it.
+template <typename PT, bool FUSED>
+__global__ __launch_bounds__(FW_PIX *FG) void k_forward_g(const LaunchList L, unsigned nblk)
+{
+    __shared__ float4 band[4][FGS];
+    const unsigned b0 = xcd_remap(blockIdx.x, nblk);
+    const int j = find_job(L, b0);
+    KJob &J = *kconst(L.job[j]);
+    unsigned lb = b0 - L.off[j];
+    const int ntheta = L.ntheta[j] ? L.ntheta[j] : J.P;
+    const int ntg = (ntheta + FG - 1) / FG;
+    const unsigned npb = (J.X + FW_PIX - 1) / FW_PIX;
+    const int pb = lb % npb;
+    lb /= npb;
+    const int tg = lb % ntg;
+    lb /= ntg;
+    const int zc = lb % J.nzc;
+    const int yb = lb / J.nzc;
+    const int th0 = L.theta0[j] + tg * FG;
+    const int nt = min(FG, L.theta0[j] + ntheta - th0);  // tilts of this group
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / FW_PIX));
+    const int lane = threadIdx.x % FW_PIX;
+    const bool mine = wv < nt;  // wave wv owns tilt th0 + wv
+    const int th = th0 + min(wv, nt - 1);
+    const int i0 = pb * FW_PIX;
+    const int i = i0 + lane;
+    const int X = J.X;
+    const int z0 = zc * J.zc_len;
+    const int z1 = min(J.T, z0 + J.zc_len);
+    const float *vbase = J.vol + (size_t)yb * J.T * X * RB;
+
+    // band ends of every tilt of the group (tilts past nt repeat the last)
+    double sl[FG], slo[FG], shi[FG];
+    int nca[FG];
+#pragma unroll
+    for (int k = 0; k < FG; ++k) {
+        KTilt &tk = kconst(J.tc)[th0 + min(k, nt - 1)];
+        sl[k] = tk.slope;
+        slo[k] = uniform_d(((double)(i0 - 1) - tk.c0) * tk.inv_a1);
+        shi[k] = uniform_d(((double)(i0 + FW_PIX - 2) - tk.c0) * tk.inv_a1);
+        nca[k] = tk.ncand;
+    }
+    // Union ends of 64 layers at once: lane l holds layer zb + l.
+    auto union_block = [&](int zb, int &lo_v, int &hi_v) {
+        const double zl = (double)(zb + lane);
+        int lo = 0x7fffffff, hi = -0x7fffffff;
+#pragma unroll
+        for (int k = 0; k < FG; ++k) {
+            lo = min(lo, (int)floor(fma(sl[k], zl, slo[k])) + 1);
+            hi = max(hi, (int)floor(fma(sl[k], zl, shi[k])) + 1 + nca[k]);
+        }
+        lo_v = lo;
+        hi_v = hi;
+    };
+    // widest union of the chunk (identical in every wave of the group)
+    int umax = 0;
+    for (int zb = z0; zb < z1; zb += FW_PIX) {
+        int lo_v, hi_v;
+        union_block(zb, lo_v, hi_v);
+        int wmax = zb + lane < z1 ? hi_v - lo_v : 0;
+#pragma unroll
+        for (int o = FW_PIX / 2; o > 0; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o, FW_PIX));
+        umax = max(umax, __builtin_amdgcn_readfirstlane(wmax));
+    }
+
+    KTilt &t = kconst(J.tc)[th];
+    const double slope = t.slope;
+    const double sbase = ((double)(i - 1) - t.c0) * t.inv_a1;
+    const float a1f = t.a1f, a1m1 = t.a1f - 1.0f;
+    const int ncand = t.ncand;
+    float acc[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) acc[r] = 0.f;
+
+    if (umax + FW_G > FGS) {
+        if (mine) {
+            for (int z = z0; z < z1; ++z) {
+                const double s = fma(slope, (double)z, sbase);
+                const double fl = floor(s);
+                const float frac = (float)(s - fl);
+                const int xf = (int)fl + 1;
+                const float d0 = fmaf(-a1f, frac, a1m1);
+                const BufSlice<float> vz(vbase + (size_t)z * X * RB, X);
+                for (int c = 0; c < ncand; ++c) {
+                    const float wt = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
+                    float v[RB];
+                    vz.load(xf + c, v);
+#pragma unroll
+                    for (int r = 0; r < RB; ++r) acc[r] = fmaf(wt, v[r], acc[r]);
+                }
+            }
+        }
+    } else {
+        // Chunk c of the union (16 cells) is loaded by wave c % FG; lane l
+        // carries plane l / 16 of cell 16 c + l % 16.
+        constexpr int NLW = (FGS / FW_G + FG - 1) / FG;
+        float4 pre[NLW];
+#pragma unroll
+        for (int m = 0; m < NLW; ++m) pre[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int lq = lane >> 4, lc = lane & 15;
+        const unsigned slice_bytes = (unsigned)X * RB * 4;
+        auto chunks = [](int wl) { return (wl + FW_G - 1) / FW_G; };
+        // Lane offsets differ by 1 KB per chunk.  A cell outside [0, X) may
+        // load a neighbouring plane's data (or zero past the slice), so at
+        // the detector's edges the stash stores zero for it instead.
+        int x0 = 0;
+        auto fetch = [&](int z, int xl, int nch) {
+            const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)(vbase + (size_t)z * X * RB), (short)0,
+                                                              (int)slice_bytes, BUF_DWORD3);
+            x0 = xl + wv * FW_G + lc;
+            const int voff = (lq * X + x0) * 16;
+#pragma unroll
+            for (int m = 0; m < NLW; ++m) {
+                if (wv + FG * m < nch) {
+                    const auto raw = __builtin_amdgcn_raw_buffer_load_b128(rs, voff + m * (FG * FW_G * 16), 0, 0);
+                    pre[m] = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]), __uint_as_float(raw[2]),
+                                         __uint_as_float(raw[3]));
+                }
+            }
+        };
+        auto stash = [&](int xl, int nch) {
+            if (xl < 0 || xl + nch * FW_G > X) {
+#pragma unroll
+                for (int m = 0; m < NLW; ++m)
+                    if ((unsigned)(x0 + m * (FG * FW_G)) >= (unsigned)X) pre[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int m = 0; m < NLW; ++m) {
+                const int c = wv + FG * m;
+                if (c < nch) band[lq][c * FW_G + lc] = pre[m];
+            }
+            // Every path now holds all of pre[] (a load skipped above was
+            // skipped when fetching too): saying so keeps the next fetch from
+            // waiting on each of its own loads.
+#pragma unroll
+            for (int m = 0; m < NLW; ++m)
+                asm volatile("" ::"v"(pre[m].x), "v"(pre[m].y), "v"(pre[m].z), "v"(pre[m].w));
+        };
+        // The layer loop, compiled once per candidate count: every variant
+        // passes the same barriers, so the waves of a group may run
+        // different ones.
+        auto layers = [&](auto nc_const) {
+            constexpr int NC = decltype(nc_const)::value;
+            int zb = z0, lo_v, hi_v;
+            union_block(zb, lo_v, hi_v);
+            int ulo = __builtin_amdgcn_readfirstlane(lo_v);
+            const int uw = __builtin_amdgcn_readfirstlane(hi_v) - ulo;
+            fetch(z0, ulo, chunks(uw));
+            stash(ulo, chunks(uw));
+            __syncthreads();
+            for (int z = z0; z < z1; ++z) {
+                int ulon = 0, uwn = 0;
+                if (z + 1 < z1) {
+                    if (z + 1 - zb == FW_PIX) {
+                        zb += FW_PIX;
+                        union_block(zb, lo_v, hi_v);
+                    }
+                    ulon = __builtin_amdgcn_readlane(lo_v, z + 1 - zb);
+                    uwn = __builtin_amdgcn_readlane(hi_v, z + 1 - zb) - ulon;
+                    fetch(z + 1, ulon, chunks(uwn));
+                }
+                if (mine) {
+                    const double s = fma(slope, (double)z, sbase);
+                    const double fl = floor(s);
+                    const float frac = (float)(s - fl);
+                    const int cl = (int)fl + 1 - ulo;
+                    const float d0 = fmaf(-a1f, frac, a1m1);
+                    fw_gather<NC, FGS>(band, cl, ncand, d0, a1f, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < RB; ++r) asm volatile("" : "+v"(acc[r]));
+                __syncthreads();  // every wave is done with layer z
+                if (z + 1 < z1) stash(ulon, chunks(uwn));
+                __syncthreads();  // the band holds layer z + 1
+                ulo = ulon;
+            }
+        };
+        switch (ncand) {
+        case 2: layers(std::integral_constant<int, 2>{}); break;
+        case 3: layers(std::integral_constant<int, 3>{}); break;
+        case 4: layers(std::integral_constant<int, 4>{}); break;
+        default: layers(std::integral_constant<int, 0>{}); break;
+        }
+    }
+    if (!mine || i >= X) return;
+    fp_epilogue<PT, FUSED, false>(J, th, zc, yb, i, acc);
 }
 
 unsigned combine_blocks(const JobDesc &h, int ntheta)
@@ -1091,13 +1315,43 @@ hipError_t launch_weights(const JobDesc &h, const JobDesc *d, hipStream_t st)
     return hipGetLastError();
 }
 
+// Forward-projection kernel for value projections: 0 direct gather,
+// 1 one wave per tilt with an LDS band, 2 tilt groups sharing a band.
+int forward_kind()
+{
+    static const int kind = [] {
+        const char *e = getenv("TREC_FP_KERNEL");
+        if (getenv("TREC_FP_DIRECT")) return 0;
+        const int v = e ? atoi(e) : 2;
+        return v >= 0 && v <= 2 ? v : 2;
+    }();
+    return kind;
+}
+
+unsigned fwd_value_blocks(const JobDesc &h, int ntheta)
+{
+    switch (forward_kind()) {
+    case 0: return forward_blocks(h, ntheta);
+    case 1: return forward_wave_blocks(h, ntheta);
+    default: return forward_group_blocks(h, ntheta);
+    }
+}
+
 hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st)
 {
     // storage < 0 selects the splat of ones (Rwt)
-    static const bool direct = getenv("TREC_FP_DIRECT") != nullptr;
+    const int kind = forward_kind();
     if (storage < 0) {
         hipLaunchKernelGGL((k_forward<float, false, true>), dim3(nblk), dim3(WG), 0, st, L, nblk);
-    } else if (direct) {
+    } else if (kind == 2) {
+        const dim3 b(FW_PIX * FG);
+        switch (fused ? storage : -1) {
+        case TREC_STORE_F16: hipLaunchKernelGGL((k_forward_g<_Float16, true>), dim3(nblk), b, 0, st, L, nblk); break;
+        case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward_g<__bf16, true>), dim3(nblk), b, 0, st, L, nblk); break;
+        case TREC_STORE_F32: hipLaunchKernelGGL((k_forward_g<float, true>), dim3(nblk), b, 0, st, L, nblk); break;
+        default: hipLaunchKernelGGL((k_forward_g<float, false>), dim3(nblk), b, 0, st, L, nblk); break;
+        }
+    } else if (kind == 0) {
         switch (fused ? storage : -1) {
         case TREC_STORE_F16: hipLaunchKernelGGL((k_forward<_Float16, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
         case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward<__bf16, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;

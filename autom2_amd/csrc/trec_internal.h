@@ -31,6 +31,14 @@ constexpr int FP_PIX = 256;
 // a1 >= 0.5 (|tilt| <= 60 degrees); wider bands take the direct path.
 constexpr int FW_PIX = 64;
 constexpr int FW_W = 144;
+constexpr int FW_G = 16;   // cells per band load
+constexpr int FW_S = 148;  // LDS plane stride in cells (2368 bytes: planes start in different banks)
+static_assert(FW_S >= ((FW_W + FW_G - 1) / FW_G) * FW_G, "band loads must fit the plane");
+// Tilt groups: FG tilts (one wave each) share an LDS band of FGS cells per
+// plane, a whole number of 16-cell load chunks.
+constexpr int FG = 4;
+constexpr int FGS = 608;
+static_assert(FGS % FW_G == 0, "the group band is loaded in whole chunks");
 // Back-projection tile: 64 voxels along x (one wave) by BP_Z along z; each
 // thread carries BP_ZT voxels of one x column (z, z+4, ...).  Residuals of
 // BP_G tilts are staged in LDS per pass, BP_W cells per tilt: with a1 <= 1
@@ -119,6 +127,9 @@ hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipSt
 // Work-group counts of the multi-job kernels for one job.
 unsigned forward_blocks(const JobDesc &h, int ntheta);
 unsigned forward_wave_blocks(const JobDesc &h, int ntheta);
+unsigned forward_group_blocks(const JobDesc &h, int ntheta);
+// work-groups of a value forward projection for the kernel launch_forward picks
+unsigned fwd_value_blocks(const JobDesc &h, int ntheta);
 unsigned combine_blocks(const JobDesc &h, int ntheta);
 unsigned back_blocks(const JobDesc &h);
 
