@@ -2,22 +2,33 @@
 //
 // AuTom2 starts one TiltRec process per tilt series; a server instead keeps
 // one process and one device arena and reconstructs series as they arrive.
-// Its loop thread repeats a step:
-//   1. admit queued requests (first come, first served) while the active
-//      set is below max_active and the request's footprint fits the arena;
-//      an admitted request's stack is copied in and its plan prepared;
-//   2. advance every active request by one stage (a SIRT iteration, a SART
-//      tilt, a BPT/WBP pass): the stages of all active requests are merged
-//      into one launch per kernel kind;
-//   3. finish the requests whose last stage ran: copy the volume out, free
-//      their arena block, wake the waiters.
+// Two host threads share the work:
+//
+//   the loop thread repeats a step:
+//     1. admit queued requests (first come, first served) while fewer than
+//        max_active are in flight and the request's footprint fits the
+//        arena; an admitted request's stack is handed to the copier;
+//     2. prepare the requests whose stack arrived (plans, descriptors,
+//        layout kernels) and add them to the active set;
+//     3. advance every active request by one stage (a SIRT iteration, a
+//        SART tilt, a BPT/WBP pass): the stages of all active requests are
+//        merged into one launch per kernel kind;
+//     4. for requests whose last stage ran, queue the output kernels and
+//        hand the volume to the copier.
+//   two copier threads, one for stacks in and one for volumes out, move
+//   data through two pinned staging buffers each on streams of their own,
+//   so large copies overlap the reconstructions of the other requests and
+//   each other: the host-side memcpy of one chunk runs while the DMA of the
+//   next is in flight.  When a volume is out the request's arena block is
+//   freed and the waiters are woken.
+//
 // A request that arrives while others run joins at the next step, so the
 // device stays busy with a changing mix.  The kernels compute each job from
 // its own data in a fixed order, so a request's volume is bit for bit the
 // same whether it ran alone or in any mix.
 #include <hip/hip_runtime.h>
 
-#include <atomic>
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -36,6 +47,28 @@ namespace {
 
 constexpr size_t ALIGN = 256;
 size_t align_up(size_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
+
+// staging chunk of the copier (two of them, pinned)
+constexpr size_t CHUNK = 32u << 20;
+
+// memcpy of a staging chunk split over a few host threads (one thread
+// copies about 10 GB/s, well below the DMA rate)
+void par_memcpy(char *dst, const char *src, size_t n)
+{
+    constexpr int NT = 4;
+    if (n < ((size_t)4 << 20)) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const size_t part = ((n + NT - 1) / NT + 4095) & ~(size_t)4095;  // part * NT >= n
+    std::thread th[NT - 1];
+    for (int t = 1; t < NT; ++t) {
+        const size_t o = std::min(n, part * t), e = std::min(n, part * (t + 1));
+        th[t - 1] = std::thread([=] { memcpy(dst + o, src + o, e - o); });
+    }
+    memcpy(dst, src, std::min(n, part));
+    for (auto &t : th) t.join();
+}
 
 double now_s()
 {
@@ -96,6 +129,13 @@ class Arena {
         }
     }
     size_t size() const { return size_; }
+    bool fits(size_t bytes) const
+    {
+        bytes = align_up(bytes);
+        for (const auto &f : free_)
+            if (f.second >= bytes) return true;
+        return false;
+    }
 
   private:
     char *base_ = nullptr;
@@ -115,7 +155,45 @@ struct Request {
     trec_status status = TREC_OK;
     bool done = false;
     char *block = nullptr;  // arena block: stack | volume | workspace
+    hipEvent_t out_ready = nullptr;  // the volume is final on the device
     JobPlan plan;
+
+    size_t stack_bytes() const { return 4 * (size_t)g.nproj * g.ny * g.nx; }
+    size_t volume_bytes() const { return 4 * (size_t)g.nx * g.ny * g.thickness; }
+    float *dstack() const { return (float *)block; }
+    float *dvolume() const { return (float *)(block + align_up(stack_bytes())); }
+    char *workspace() const { return block + align_up(stack_bytes()) + align_up(volume_bytes()); }
+};
+
+// One direction of host <-> device traffic.
+struct Copier {
+    hipStream_t stream = nullptr;
+    char *pinned[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::deque<Request *> jobs;
+    std::thread th;
+
+    bool init()
+    {
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return false;
+        for (int b = 0; b < 2; ++b)
+            if (hipHostMalloc((void **)&pinned[b], CHUNK, hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) != hipSuccess)
+                return false;
+        return true;
+    }
+    void release()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (int b = 0; b < 2; ++b) {
+            if (pinned[b]) (void)hipHostFree(pinned[b]);
+            if (ev[b]) (void)hipEventDestroy(ev[b]);
+            pinned[b] = nullptr;
+            ev[b] = nullptr;
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
 };
 
 }  // namespace
@@ -124,31 +202,121 @@ struct trec_server {
     int device = 0;
     int max_active = 8;
     trec_context *ctx = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // reconstructions
+    Copier cin, cout;              // stacks in, volumes out
     Arena arena;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    bool stop = false;
+    std::condition_variable cv_work, cv_done, cv_copy;
+    bool stop = false;          // no more requests: the loop ends when all are done
+    bool copier_stop = false;   // the loop has ended: the copier ends when idle
     uint64_t next_id = 1;
-    std::deque<Request *> queue;
+    int in_flight = 0;                  // admitted and not finished
+    std::deque<Request *> queue;        // waiting for admission
+    std::deque<Request *> ready;        // stack on the device, not yet prepared
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
     std::thread loop;
 
     void run();
-    trec_status admit(Request &r);
+    void run_copier(Copier &c, bool in);
+    trec_status prepare(Request &r);
+    trec_status copy_in(Copier &c, Request &r);
+    trec_status copy_out(Copier &c, Request &r);
+    void finish(Request &r, trec_status st);  // with mu held
 };
 
-trec_status trec_server::admit(Request &r)
+// Stack: host -> pinned chunk (memcpy) -> device (DMA).  The memcpy of chunk
+// k + 1 runs while chunk k's DMA is in flight.
+trec_status trec_server::copy_in(Copier &c, Request &r)
 {
-    const size_t X = r.g.nx, Y = r.g.ny, T = r.g.thickness, P = r.g.nproj;
-    float *dstack = (float *)r.block;
-    float *dvol = (float *)(r.block + align_up(4 * P * Y * X));
-    char *ws = r.block + align_up(4 * P * Y * X) + align_up(4 * X * Y * T);
-    trec_status s = plan_job(ctx, r.plan, r.g, r.m, r.angles.data(), dstack, nullptr, dvol);
+    const char *src = reinterpret_cast<const char *>(r.stack_host);
+    char *dst = reinterpret_cast<char *>(r.dstack());
+    const size_t n = r.stack_bytes();
+    int b = 0;
+    for (size_t off = 0; off < n; off += CHUNK, b ^= 1) {
+        const size_t len = std::min(CHUNK, n - off);
+        if (hipEventSynchronize(c.ev[b]) != hipSuccess) return TREC_ERR_DEVICE;  // buffer b is free
+        par_memcpy(c.pinned[b], src + off, len);
+        if (hipMemcpyAsync(dst + off, c.pinned[b], len, hipMemcpyHostToDevice, c.stream) != hipSuccess ||
+            hipEventRecord(c.ev[b], c.stream) != hipSuccess)
+            return TREC_ERR_DEVICE;
+    }
+    return hipStreamSynchronize(c.stream) == hipSuccess ? TREC_OK : TREC_ERR_DEVICE;
+}
+
+// Volume: device -> pinned chunk (DMA) -> host (memcpy), chunk k + 1's DMA
+// overlapping chunk k's memcpy.
+trec_status trec_server::copy_out(Copier &c, Request &r)
+{
+    const char *src = reinterpret_cast<const char *>(r.dvolume());
+    char *dst = reinterpret_cast<char *>(r.volume_out);
+    const size_t n = r.volume_bytes();
+    if (hipStreamWaitEvent(c.stream, r.out_ready, 0) != hipSuccess) return TREC_ERR_DEVICE;
+    const size_t nchunk = (n + CHUNK - 1) / CHUNK;
+    auto issue = [&](size_t k) {
+        const size_t off = k * CHUNK, len = std::min(CHUNK, n - off);
+        return hipMemcpyAsync(c.pinned[k & 1], src + off, len, hipMemcpyDeviceToHost, c.stream) == hipSuccess &&
+               hipEventRecord(c.ev[k & 1], c.stream) == hipSuccess;
+    };
+    if (nchunk && !issue(0)) return TREC_ERR_DEVICE;
+    for (size_t k = 0; k < nchunk; ++k) {
+        if (k + 1 < nchunk && !issue(k + 1)) return TREC_ERR_DEVICE;
+        if (hipEventSynchronize(c.ev[k & 1]) != hipSuccess) return TREC_ERR_DEVICE;
+        const size_t off = k * CHUNK, len = std::min(CHUNK, n - off);
+        par_memcpy(dst + off, c.pinned[k & 1], len);
+    }
+    return TREC_OK;
+}
+
+void trec_server::finish(Request &r, trec_status st)
+{
+    if (r.status == TREC_OK) r.status = st;
+    if (r.block) arena.release(r.block);
+    r.block = nullptr;
+    if (r.out_ready) (void)hipEventDestroy(r.out_ready);
+    r.out_ready = nullptr;
+    r.done = true;
+    r.done_s = now_s();
+    --in_flight;
+    cv_done.notify_all();
+    cv_work.notify_all();
+}
+
+void trec_server::run_copier(Copier &c, bool in)
+{
+    (void)hipSetDevice(device);
+    for (;;) {
+        Request *job;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_copy.wait(lk, [&] { return copier_stop || !c.jobs.empty(); });
+            if (c.jobs.empty()) return;
+            job = c.jobs.front();
+            c.jobs.pop_front();
+        }
+        Request &r = *job;
+        if (in) {
+            const trec_status st = copy_in(c, r);
+            std::lock_guard<std::mutex> lk(mu);
+            if (st == TREC_OK) {
+                ready.push_back(&r);
+                cv_work.notify_all();
+            } else {
+                finish(r, st);
+            }
+        } else {
+            const trec_status st = copy_out(c, r);
+            std::lock_guard<std::mutex> lk(mu);
+            finish(r, st);
+        }
+    }
+}
+
+// Plans an arrived request and queues its preparation kernels.
+trec_status trec_server::prepare(Request &r)
+{
+    trec_status s = plan_job(ctx, r.plan, r.g, r.m, r.angles.data(), r.dstack(), nullptr, r.dvolume());
     if (s != TREC_OK) return s;
-    carve(r.plan, ws);
-    if (hipMemcpyAsync(dstack, r.stack_host, 4 * P * Y * X, hipMemcpyHostToDevice, stream) != hipSuccess)
-        return TREC_ERR_DEVICE;
+    carve(r.plan, r.workspace());
     s = upload(ctx, {&r.plan}, stream);
     if (s != TREC_OK) return s;
     if (issue_prep(r.plan, stream) != hipSuccess) return TREC_ERR_DEVICE;
@@ -161,32 +329,45 @@ void trec_server::run()
     std::vector<Request *> active;
     std::vector<int> stage;
     for (;;) {
-        std::vector<Request *> admitted;
+        std::vector<Request *> arrived;
         {
             std::unique_lock<std::mutex> lk(mu);
-            if (active.empty()) cv_work.wait(lk, [&] { return stop || !queue.empty(); });
-            if (stop && queue.empty() && active.empty()) return;
-            while (!queue.empty() && (int)active.size() + (int)admitted.size() < max_active) {
+            auto admissible = [&] {
+                return !queue.empty() && in_flight < max_active && arena.fits(queue.front()->footprint);
+            };
+            if (active.empty())
+                cv_work.wait(lk, [&] { return (stop && queue.empty() && in_flight == 0) || !ready.empty() || admissible(); });
+            if (stop && queue.empty() && in_flight == 0 && active.empty()) return;
+            // first come, first served: stop at the first request that does not fit yet
+            while (!queue.empty() && in_flight < max_active) {
                 Request *r = queue.front();
                 r->block = arena.alloc(r->footprint);
-                if (!r->block) break;  // first come, first served: wait for memory
+                if (!r->block) break;
                 queue.pop_front();
                 r->admit_s = now_s();
-                admitted.push_back(r);
+                ++in_flight;
+                if (hipEventCreateWithFlags(&r->out_ready, hipEventDisableTiming) != hipSuccess) {
+                    r->out_ready = nullptr;
+                    finish(*r, TREC_ERR_DEVICE);
+                    continue;
+                }
+                cin.jobs.push_back(r);
+                cv_copy.notify_all();
+            }
+            while (!ready.empty()) {
+                arrived.push_back(ready.front());
+                ready.pop_front();
             }
         }
-        for (Request *r : admitted) {
-            r->status = admit(*r);
-            if (r->status == TREC_OK) {
+        for (Request *r : arrived) {
+            const trec_status st = prepare(*r);
+            if (st == TREC_OK) {
                 active.push_back(r);
                 stage.push_back(0);
             } else {
                 (void)hipStreamSynchronize(stream);
                 std::lock_guard<std::mutex> lk(mu);
-                arena.release(r->block);
-                r->done = true;
-                r->done_s = now_s();
-                cv_done.notify_all();
+                finish(*r, st);
             }
         }
         if (active.empty()) continue;
@@ -198,33 +379,26 @@ void trec_server::run()
             if (stage[k] < active[k]->plan.stages) continue;
             Request &r = *active[k];
             e = issue_finish(r.plan, stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(r.volume_out, r.plan.volume, 4 * (size_t)r.g.nx * r.g.ny * r.g.thickness,
-                                   hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipEventRecord(r.out_ready, stream);
             finished.push_back(k);
         }
+        // one step in flight at a time: the host prepares the next step's
+        // launches while the device runs this one
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
         if (e != hipSuccess) {
             // a device failure ends every active request with an error
             (void)hipGetLastError();
-            finished.clear();
-            for (size_t k = 0; k < active.size(); ++k) {
-                active[k]->status = TREC_ERR_DEVICE;
-                finished.push_back(k);
-            }
+            std::lock_guard<std::mutex> lk(mu);
+            for (Request *r : active) finish(*r, TREC_ERR_DEVICE);
+            active.clear();
+            stage.clear();
+            continue;
         }
         if (finished.empty()) continue;
         {
             std::lock_guard<std::mutex> lk(mu);
-            const double t = now_s();
-            for (size_t k : finished) {
-                Request &r = *active[k];
-                arena.release(r.block);
-                r.done = true;
-                r.done_s = t;
-            }
-            cv_done.notify_all();
-            cv_work.notify_all();
+            for (size_t k : finished) cout.jobs.push_back(active[k]);
+            cv_copy.notify_all();
         }
         std::vector<Request *> keep_r;
         std::vector<int> keep_s;
@@ -254,25 +428,24 @@ trec_status trec_server_create(int device, size_t hbm_budget, int max_active, tr
     if (st != TREC_OK) return st;
     s->device = device;
     s->max_active = max_active;
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    auto fail_out = [&](trec_status code) {
+        s->cin.release();
+        s->cout.release();
+        if (s->stream) (void)hipStreamDestroy(s->stream);
         trec_context_destroy(s->ctx);
-        return TREC_ERR_DEVICE;
-    }
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return fail_out(TREC_ERR_DEVICE);
+    if (!s->cin.init() || !s->cout.init()) return fail_out(TREC_ERR_NOMEM);
     if (hbm_budget == 0) {
         size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
-            (void)hipStreamDestroy(s->stream);
-            trec_context_destroy(s->ctx);
-            return TREC_ERR_DEVICE;
-        }
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail_out(TREC_ERR_DEVICE);
         hbm_budget = fr / 10 * 9;
     }
-    if (!s->arena.init(align_up(hbm_budget))) {
-        (void)hipStreamDestroy(s->stream);
-        trec_context_destroy(s->ctx);
-        return TREC_ERR_NOMEM;
-    }
+    if (!s->arena.init(align_up(hbm_budget))) return fail_out(TREC_ERR_NOMEM);
     trec_server *raw = s.release();
+    raw->cin.th = std::thread([raw] { raw->run_copier(raw->cin, true); });
+    raw->cout.th = std::thread([raw] { raw->run_copier(raw->cout, false); });
     raw->loop = std::thread([raw] { raw->run(); });
     *out = raw;
     return TREC_OK;
@@ -342,7 +515,16 @@ void trec_server_destroy(trec_server *srv)
         srv->cv_work.notify_all();
     }
     if (srv->loop.joinable()) srv->loop.join();
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->copier_stop = true;
+        srv->cv_copy.notify_all();
+    }
+    if (srv->cin.th.joinable()) srv->cin.th.join();
+    if (srv->cout.th.joinable()) srv->cout.th.join();
     (void)hipStreamSynchronize(srv->stream);
+    srv->cin.release();
+    srv->cout.release();
     (void)hipStreamDestroy(srv->stream);
     trec_context_destroy(srv->ctx);
     delete srv;
