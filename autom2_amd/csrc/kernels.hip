@@ -851,15 +851,21 @@ unsigned back_blocks(const JobDesc &h)
     return nxb * nzb * (unsigned)h.nyb;
 }
 
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
 // Voxel-driven back-projection of one 64 x BP_Z voxel tile of one row block.
 // Per pass, the residual cells the tile can touch for BP_G tilts are staged
-// in LDS as float planes; every thread then gathers its voxels' two taps per
-// tilt from LDS with the exact double geometry.  MODE selects the update
-// (BackMode).  For BACK_SIRT and BACK_SART the image is the residual, for
-// BACK_BPT the (filtered) stack.
+// in LDS as float planes (half pairs for half storage); every thread then
+// gathers its voxels' two taps per tilt from LDS with the exact double
+// geometry.  MODE selects the update (BackMode).  For BACK_SIRT and
+// BACK_SART the image is the residual, for BACK_BPT the (filtered) stack.
+
 template <typename PT, int MODE>
 __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
 {
+    // half storage keeps the residual in half in LDS: entry (t, q, c) holds
+    // rows 4q .. 4q+3 of cells c and c+1 as (c, c+1) pairs
+    constexpr bool HALF = std::is_same<PT, _Float16>::value;
     __shared__ float4 win[BP_G][4][BP_W];
     __shared__ int gmin_s[BP_G];
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
@@ -910,14 +916,35 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
             gmin_s[threadIdx.x] = min(ga, gb);
         }
         __syncthreads();
-        for (int v = threadIdx.x; v < ng * 4 * BP_W; v += WG) {
+        if constexpr (HALF) {
+            // plane p (rows 8p .. 8p+7) of cells g and g+1 -> entries q = 2p
+            // and 2p + 1 as (g, g+1) pairs
+            for (int v = threadIdx.x; v < ng * 2 * BP_W; v += WG) {
+                const int t = v / (2 * BP_W);
+                const int rem = v % (2 * BP_W);
+                const int pl = rem / BP_W, c = rem % BP_W;
+                const int g = gmin_s[t] + c;
+                const PT *rows = imgb + (size_t)(tg + t) * X * RB + (size_t)pl * X * 8;
+                uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
+                if ((unsigned)g < (unsigned)X) a = *reinterpret_cast<const uint4 *>(rows + (size_t)g * 8);
+                if ((unsigned)(g + 1) < (unsigned)X) b = *reinterpret_cast<const uint4 *>(rows + (size_t)(g + 1) * 8);
+                *reinterpret_cast<uint4 *>(&win[t][2 * pl][c]) =
+                    make_uint4((a.x & 0xffffu) | (b.x << 16), (a.x >> 16) | (b.x & 0xffff0000u),
+                               (a.y & 0xffffu) | (b.y << 16), (a.y >> 16) | (b.y & 0xffff0000u));
+                *reinterpret_cast<uint4 *>(&win[t][2 * pl + 1][c]) =
+                    make_uint4((a.z & 0xffffu) | (b.z << 16), (a.z >> 16) | (b.z & 0xffff0000u),
+                               (a.w & 0xffffu) | (b.w << 16), (a.w >> 16) | (b.w & 0xffff0000u));
+            }
+        }
+        // float and bfloat16: rows 4q .. 4q+3 of cell g as floats
+        for (int v = threadIdx.x; !HALF && v < ng * 4 * BP_W; v += WG) {
             const int t = v / (4 * BP_W);
             const int rem = v % (4 * BP_W);
             const int q = rem / BP_W, c = rem % BP_W;
             const int g = gmin_s[t] + c;
+            const PT *sl = imgb + (size_t)(tg + t) * X * RB;
             float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
             if ((unsigned)g < (unsigned)X) {
-                const PT *sl = imgb + (size_t)(tg + t) * X * RB;
                 if constexpr (sizeof(PT) == 4) {
                     o = *reinterpret_cast<const float4 *>(sl + ((size_t)q * X + g) * 4);
                 } else {
@@ -940,6 +967,43 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
                 exact_tap(tc, Xc, Zc[k], g, dx);
                 const int c = min(max(g - gm, 0), BP_W - 2);
                 const float w0 = 1.f - dx, w1 = dx;
+                if constexpr (HALF) {
+                    // both taps of a row in one v_dot2_f32_f16: weights and
+                    // residuals in half, products and sums in float
+                    // dx arrives as a float rounded from double: the empty asm
+                    // keeps the compiler from fusing the two roundings into a
+                    // (software) double-to-half conversion
+                    float w1f = w1;
+                    asm volatile("" : "+v"(w1f));
+                    const half2v w = {(_Float16)w0, (_Float16)w1f};
+                    float ic = 1.f;
+                    bool upd = true;
+                    if constexpr (MODE != BACK_SIRT) {
+                        float cw = 0.f;
+                        cw += ((unsigned)g < (unsigned)X) ? w0 : 0.f;
+                        cw += ((unsigned)(g + 1) < (unsigned)X) ? w1 : 0.f;
+                        if constexpr (MODE == BACK_SART) cs[k] = cw;
+                        upd = cw > J.thr_upd;
+                        ic = upd ? 1.0f / cw : 0.f;
+                    }
+                    if (upd) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const uint4 e = *reinterpret_cast<const uint4 *>(&win[t][q][c]);
+                            const unsigned ev[4] = {e.x, e.y, e.z, e.w};
+#pragma unroll
+                            for (int h = 0; h < 4; ++h) {
+                                const half2v pr = __builtin_bit_cast(half2v, ev[h]);
+                                if constexpr (MODE == BACK_SIRT)
+                                    val[k][4 * q + h] = __builtin_amdgcn_fdot2(pr, w, val[k][4 * q + h], false);
+                                else
+                                    val[k][4 * q + h] =
+                                        fmaf(__builtin_amdgcn_fdot2(pr, w, 0.f, false), ic, val[k][4 * q + h]);
+                            }
+                        }
+                    }
+                    continue;
+                }
                 float r0[RB], r1[RB];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
