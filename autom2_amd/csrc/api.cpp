@@ -37,10 +37,10 @@ size_t align_up(size_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
 std::mutex g_tune_mu;
 std::map<std::string, int> g_tuned_nzc;  // process-wide autotune results
 
-std::string tune_key(int X, int T, int P, int storage)
+std::string tune_key(const trec_geom &g, int storage)
 {
     char b[96];
-    snprintf(b, sizeof b, "%d:%d:%d:%d", X, T, P, storage);
+    snprintf(b, sizeof b, "%d:%d:%d:%d:%d", g.nx, g.ny, g.thickness, g.nproj, storage);
     return b;
 }
 
@@ -252,7 +252,7 @@ int pick_nzc(trec_context *ctx, const trec_geom &g, int storage)
         int v = atoi(e);
         if (v >= 1) return std::min(v, g.thickness);
     }
-    const std::string key = tune_key(g.nx, g.thickness, g.nproj, storage);
+    const std::string key = tune_key(g, storage);
     if (ctx) {
         auto it = ctx->tuned.find(key);
         if (it != ctx->tuned.end()) return std::min(it->second, g.thickness);
@@ -262,10 +262,16 @@ int pick_nzc(trec_context *ctx, const trec_geom &g, int storage)
         auto it = g_tuned_nzc.find(key);
         if (it != g_tuned_nzc.end()) return std::min(it->second, g.thickness);
     }
-    // Keep one z chunk of one row block (T/nzc * X * RB floats) near 2 MiB,
-    // so a chunk and its tilts stay in one XCD's L2.
-    const double slab = (double)g.thickness * g.nx * RB * 4.0;
-    int nzc = (int)std::ceil(slab / (2.0 * 1024 * 1024));
+    // One z chunk (the residual written by the forward projection itself, no
+    // combine pass) unless the grid would then have fewer than about eight
+    // work-groups per CU; more chunks split the layers between work-groups.
+    JobDesc h{};
+    h.X = g.nx;
+    h.nyb = (g.ny + RB - 1) / RB;
+    h.nzc = 1;
+    const unsigned wg1 = fwd_value_blocks(h, g.nproj);
+    int nzc = 1;
+    while (nzc < g.thickness && (unsigned long long)wg1 * nzc < 2048ull) nzc *= 2;
     return std::max(1, std::min(nzc, g.thickness));
 }
 
@@ -660,8 +666,8 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
                 [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, true, st); }));
         LCHK(fp_part.flush([&](const LaunchList &L, unsigned n) { return launch_forward(L, n, 0, false, st); }));
     }
-    {
-        ProfScope ps(PROF_COMBINE, st);
+    if (!comb[0].items.empty() || !comb[1].items.empty() || !comb[2].items.empty()) {
+        ProfScope ps(PROF_COMBINE, st);  // timed only when there is a combine pass
         for (int sto = 0; sto < 3; ++sto)
             LCHK(comb[sto].flush([&](const LaunchList &L, unsigned n) { return launch_combine(L, n, sto, st); }));
     }
@@ -1088,7 +1094,7 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
     (void)hipEventCreate(&e1);
     int best = 1;
     float best_ms = 1e30f;
-    const int cands[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
+    const int cands[] = {1, 2, 4, 8, 16, 32};
     trec_status res = TREC_OK;
     for (int nzc : cands) {
         if (nzc > (int)T) break;
@@ -1108,15 +1114,21 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
             res = fail(ctx, TREC_ERR_DEVICE, "autotune launch failed");
             break;
         }
-        if (hipEventRecord(e0, st) != hipSuccess || issue_step(pl, stage, st) != hipSuccess ||
-            issue_step(pl, stage, st) != hipSuccess || hipEventRecord(e1, st) != hipSuccess ||
-            hipEventSynchronize(e1) != hipSuccess) {
-            res = fail(ctx, TREC_ERR_DEVICE, "autotune run failed");
-            break;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 2 && res == TREC_OK; ++rep) {
+            float t = 0.f;
+            if (hipEventRecord(e0, st) != hipSuccess || issue_step(pl, stage, st) != hipSuccess ||
+                hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                res = fail(ctx, TREC_ERR_DEVICE, "autotune run failed");
+                break;
+            }
+            (void)hipEventElapsedTime(&t, e0, e1);
+            ms = std::min(ms, t);
         }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best_ms) {
+        if (res != TREC_OK) break;
+        // more chunks must win clearly: one chunk needs no combine pass and
+        // fewer chunks leave more room to batch jobs
+        if (ms < best_ms * 0.98f) {
             best_ms = ms;
             best = p.h.nzc;
         }
@@ -1127,7 +1139,7 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
     (void)hipFree(stack);
     (void)hipFree(vol);
     if (res != TREC_OK) return res;
-    const std::string key = tune_key(geom->nx, geom->thickness, geom->nproj, method->storage);
+    const std::string key = tune_key(*geom, method->storage);
     ctx->tuned[key] = best;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -175,7 +175,9 @@ def counters_run(iters):
     g = trec.make_geom(c["X"], c["Y"], c["P"], c["T"])
     ctx = trec.Context(0)
     vol = torch.empty((c["Y"], c["T"], c["X"]), device="cuda")
-    ctx.reconstruct(g, ang, s, trec.make_method(trec.SIRT, iters, 1.0), vol)
+    m = trec.make_method(trec.SIRT, iters, 1.0)
+    ctx.autotune(g, m)  # the launch shapes the bench uses
+    ctx.reconstruct(g, ang, s, m, vol)
     torch.cuda.synchronize()
     a = torch.empty(256 * 1024 * 1024, device="cuda")
     b = torch.empty_like(a)
@@ -187,14 +189,20 @@ def counters_run(iters):
 
 
 def profiled_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed counter summary
-    (profiles/sirt1024_f32_counters.md), or None."""
+    """HBM bytes per dispatch of the kernel whose name starts with `kernel`,
+    from the committed counter summary (profiles/sirt1024_f32_counters.md):
+    its calibrated read GB plus write GB, or None."""
     path = os.path.join(ROOT, "profiles", "sirt1024_f32_counters.md")
     try:
+        cols = None
         for line in open(path):
             cells = [c.strip() for c in line.split("|")]
-            if len(cells) > 5 and cells[1].startswith(f"`k_{kernel}"):
-                return (float(cells[3]) + float(cells[5])) * 1e9
+            if len(cells) < 4:
+                continue
+            if "read GB" in cells and "write GB" in cells:
+                cols = (cells.index("read GB"), cells.index("write GB"))
+            elif cols and cells[1].startswith(f"`k_{kernel}"):
+                return (float(cells[cols[0]]) + float(cells[cols[1]])) * 1e9
     except (OSError, ValueError):
         pass
     return None
@@ -265,7 +273,8 @@ def main():
         alg["forward"] = 4 * V + 8 * PA
     dom = max(("forward", "back"), key=lambda k: kern[k]["ms_total"])
     achieved = alg[dom] / (kern[dom]["ms_avg"] / 1e3) / 1e9
-    kname = {"forward": "forward_w<float, false>", "back": "back<float, 0>"}[dom]
+    fused = kern["combine"]["launches"] == 0
+    kname = {"forward": f"forward_g<float, {'true' if fused else 'false'}>", "back": "back<float, 0>"}[dom]
     traffic = profiled_traffic(kname) if args.config == "sirt1024" else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
