@@ -401,6 +401,11 @@ size_t carve(JobPlan &p, char *base)
         // (16 bytes each) past a slice's last plane; the tail keeps even the
         // last slice's overrun inside the workspace
         h.vol = (float *)take(4 * nyb * T * X * RB + 16 * FGS);
+        // half storage: the forward projection reads a half copy of the
+        // volume (same tail for the band loads' overrun)
+        h.hvol = (p.m.storage == TREC_STORE_F16 && is_iterative(k))
+                     ? take(64 * nyb * T * (X + 1) + 16 * (FGS + FW_G))
+                     : nullptr;
         h.proj = take(es * nyb * P * X * RB);
         h.res = is_iterative(k) ? take(es * nyb * P * X * RB) : nullptr;
         h.part = (k == TREC_SIRT && h.nzc > 1) ? (float *)take(4 * (size_t)h.nzc * nyb * P * X * RB) : nullptr;
@@ -520,6 +525,7 @@ hipError_t issue_prep(JobPlan &p, hipStream_t st)
         LCHK(launch_block_stack(h, p.d, p.stack, st));
         if (p.initial) LCHK(launch_block_volume(h, p.d, p.initial, p.m.layout, st));
         else LCHK(launch_zero(h.vol, (size_t)h.nyb * T * X * RB, st));
+        if (h.hvol) LCHK(launch_shadow(h, p.d, st));
         if (k == TREC_WBP)
             // rows of the float slices: 4-row planes of X cells, row stride 1, cell stride 4
             LCHK(launch_rweight((float *)h.proj, p.hk, p.N, (int)X, (long long)h.nyb * P * RB, 1, 4, 4,
@@ -626,7 +632,7 @@ struct Batcher {
 
 hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &stage, hipStream_t st)
 {
-    Batcher fp_fused[3], fp_part, comb[3], back[3][3];
+    Batcher fp_fused[3], fp_part[3], comb[3], back[3][3];
     for (size_t j = 0; j < plans.size(); ++j) {
         JobPlan &p = *plans[j];
         const int s = stage[j];
@@ -643,7 +649,7 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
             if (h.nzc == 1) {
                 fp_fused[sto].add(p.d, fwd_blocks(h, h.P), 0, 0);
             } else {
-                fp_part.add(p.d, fwd_blocks(h, h.P), 0, 0);
+                fp_part[sto].add(p.d, fwd_blocks(h, h.P), 0, 0);
                 comb[sto].add(p.d, combine_blocks(h, h.P), 0, 0);
             }
             back[BACK_SIRT][sto].add(p.d, back_blocks(h), 0, 0);
@@ -664,7 +670,9 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
         for (int sto = 0; sto < 3; ++sto)
             LCHK(fp_fused[sto].flush(
                 [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, true, st); }));
-        LCHK(fp_part.flush([&](const LaunchList &L, unsigned n) { return launch_forward(L, n, 0, false, st); }));
+        for (int sto = 0; sto < 3; ++sto)
+            LCHK(fp_part[sto].flush(
+                [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, false, st); }));
     }
     if (!comb[0].items.empty() || !comb[1].items.empty() || !comb[2].items.empty()) {
         ProfScope ps(PROF_COMBINE, st);  // timed only when there is a combine pass

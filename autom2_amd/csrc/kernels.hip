@@ -303,6 +303,45 @@ __global__ __launch_bounds__(WG) void k_voxel_weights(const JobDesc *__restrict_
     J.inv_c[idx] = (C > J.thr_upd) ? 1.0f / C : 0.f;
 }
 
+// ------------------------------------------------------------------ half volume copy
+
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned pack_half2(float a, float b)
+{
+    const half2v h = {(_Float16)a, (_Float16)b};
+    return __builtin_bit_cast(unsigned, h);
+}
+
+// hvol entry c + 1 (c in [-1, X)) of every quad from vol cells c and c+1
+// (zero outside [0, X)).
+__global__ __launch_bounds__(WG) void k_shadow(const JobDesc *__restrict__ jd)
+{
+    const JobDesc &J = *jd;
+    const int X = J.X;
+    const int ne = X + 1;
+    const unsigned nb = (unsigned)(ne + WG - 1) / WG;
+    unsigned b = blockIdx.x;
+    const int e = (int)(b % nb) * WG + threadIdx.x;
+    b /= nb;
+    const int z = b % J.T;
+    const int yb = b / J.T;
+    if (e >= ne) return;
+    const int c = e - 1;
+    const float *slice = J.vol + ((size_t)yb * J.T + z) * X * RB;
+    float v0[RB], v1[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) v0[r] = v1[r] = 0.f;
+    if (c >= 0) Slice<float>::load(slice, X, c, v0);
+    if (c + 1 < X) Slice<float>::load(slice, X, c + 1, v1);
+    uint4 *hs = static_cast<uint4 *>(J.hvol) + ((size_t)yb * J.T + z) * 4 * ne;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        hs[(size_t)q * ne + e] = make_uint4(pack_half2(v0[4 * q], v1[4 * q]), pack_half2(v0[4 * q + 1], v1[4 * q + 1]),
+                                            pack_half2(v0[4 * q + 2], v1[4 * q + 2]),
+                                            pack_half2(v0[4 * q + 3], v1[4 * q + 3]));
+}
+
 // ------------------------------------------------------------------ forward
 
 unsigned forward_blocks(const JobDesc &h, int ntheta)
@@ -442,6 +481,30 @@ __device__ __forceinline__ void fw_gather(const float4 (*band)[S], int cl, int n
             acc[4 * q + 1] = fmaf(wt, v.y, acc[4 * q + 1]);
             acc[4 * q + 2] = fmaf(wt, v.z, acc[4 * q + 2]);
             acc[4 * q + 3] = fmaf(wt, v.w, acc[4 * q + 3]);
+        }
+    }
+}
+
+// The same sums from the half copy of the volume: candidates c and c+1
+// share one (c, c+1) pair entry, applied by one v_dot2_f32_f16 per row with
+// half weights and float accumulation.  A missing odd candidate gets weight 0.
+template <int NC, int S>
+__device__ __forceinline__ void fw_gather_h(const float4 (*band)[S], int cl, int ncand, float d0, float a1f,
+                                            float (&acc)[RB])
+{
+    const int n = NC ? NC : ncand;
+#pragma unroll
+    for (int c = 0; c < n; c += 2) {
+        const float wa = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
+        const float wb = c + 1 < n ? fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)(c + 1), d0))) : 0.f;
+        const half2v w = {(_Float16)wa, (_Float16)wb};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 e = *reinterpret_cast<const uint4 *>(&band[q][cl + c]);
+            acc[4 * q] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, e.x), w, acc[4 * q], false);
+            acc[4 * q + 1] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, e.y), w, acc[4 * q + 1], false);
+            acc[4 * q + 2] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, e.z), w, acc[4 * q + 2], false);
+            acc[4 * q + 3] = __builtin_amdgcn_fdot2(__builtin_bit_cast(half2v, e.w), w, acc[4 * q + 3], false);
         }
     }
 }
@@ -611,12 +674,13 @@ unsigned forward_group_blocks(const JobDesc &h, int ntheta)
 // LDS stores per tilt by up to FG.  Unions wider than the LDS band (tilts
 // past about 60 degrees far from the tilt axis) are gathered straight from
 // L2 by every wave instead.  Per pixel, the sums and their order are those
-// of k_forward and k_forward_w.
+// of k_forward and k_forward_w.  HV (half storage) stages the half copy of
+// the volume, hvol, whose pair entries the waves apply with fw_gather_h.
 //
 // The union of layer z is [ulo, ulo + uw) with ulo the least first cell and
 // ulo + uw the greatest end over the group's tilts, each computed with the
 // expressions the pixels use, so every lane's candidates lie inside it.
-template <typename PT, bool FUSED>
+template <typename PT, bool FUSED, bool HV>
 __global__ __launch_bounds__(FW_PIX *FG) void k_forward_g(const LaunchList L, unsigned nblk)
 {
     __shared__ float4 band[4][FGS];
@@ -715,17 +779,23 @@ __global__ __launch_bounds__(FW_PIX *FG) void k_forward_g(const LaunchList L, un
 #pragma unroll
         for (int m = 0; m < NLW; ++m) pre[m] = make_float4(0.f, 0.f, 0.f, 0.f);
         const int lq = lane >> 4, lc = lane & 15;
-        const unsigned slice_bytes = (unsigned)X * RB * 4;
+        // a slice is 4 planes (float rows 4q .. 4q+3) or 4 quads of pair
+        // entries (half), PS entries of 16 bytes each; cell x is entry x + E0
+        const int PS = HV ? X + 1 : X;
+        constexpr int E0 = HV ? 1 : 0;
+        const unsigned slice_bytes = 4u * (unsigned)PS * 16u;
+        const char *bsrc = HV ? static_cast<const char *>(J.hvol) + (size_t)yb * J.T * slice_bytes
+                              : reinterpret_cast<const char *>(vbase);
         auto chunks = [](int wl) { return (wl + FW_G - 1) / FW_G; };
-        // Lane offsets differ by 1 KB per chunk.  A cell outside [0, X) may
+        // Lane offsets differ by 1 KB per chunk.  A cell outside [-E0, X) may
         // load a neighbouring plane's data (or zero past the slice), so at
         // the detector's edges the stash stores zero for it instead.
         int x0 = 0;
         auto fetch = [&](int z, int xl, int nch) {
-            const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)(vbase + (size_t)z * X * RB), (short)0,
+            const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)(bsrc + (size_t)z * slice_bytes), (short)0,
                                                               (int)slice_bytes, BUF_DWORD3);
             x0 = xl + wv * FW_G + lc;
-            const int voff = (lq * X + x0) * 16;
+            const int voff = (lq * PS + x0 + E0) * 16;
 #pragma unroll
             for (int m = 0; m < NLW; ++m) {
                 if (wv + FG * m < nch) {
@@ -736,10 +806,11 @@ __global__ __launch_bounds__(FW_PIX *FG) void k_forward_g(const LaunchList L, un
             }
         };
         auto stash = [&](int xl, int nch) {
-            if (xl < 0 || xl + nch * FW_G > X) {
+            if (xl < -E0 || xl + nch * FW_G > X) {
 #pragma unroll
                 for (int m = 0; m < NLW; ++m)
-                    if ((unsigned)(x0 + m * (FG * FW_G)) >= (unsigned)X) pre[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if ((unsigned)(x0 + m * (FG * FW_G) + E0) >= (unsigned)(X + E0))
+                        pre[m] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
             for (int m = 0; m < NLW; ++m) {
@@ -782,7 +853,10 @@ __global__ __launch_bounds__(FW_PIX *FG) void k_forward_g(const LaunchList L, un
                     const float frac = (float)(s - fl);
                     const int cl = (int)fl + 1 - ulo;
                     const float d0 = fmaf(-a1f, frac, a1m1);
-                    fw_gather<NC, FGS>(band, cl, ncand, d0, a1f, acc);
+                    if constexpr (HV)
+                        fw_gather_h<NC, FGS>(band, cl, ncand, d0, a1f, acc);
+                    else
+                        fw_gather<NC, FGS>(band, cl, ncand, d0, a1f, acc);
                 }
 #pragma unroll
                 for (int r = 0; r < RB; ++r) asm volatile("" : "+v"(acc[r]));
@@ -850,8 +924,6 @@ unsigned back_blocks(const JobDesc &h)
     const unsigned nzb = (h.T + BP_Z - 1) / BP_Z;
     return nxb * nzb * (unsigned)h.nyb;
 }
-
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
 // Voxel-driven back-projection of one 64 x BP_Z voxel tile of one row block.
 // Per pass, the residual cells the tile can touch for BP_G tilts are staged
@@ -1030,27 +1102,71 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
         }
     }
 
-    if (x >= X) return;
+    // a wave holds 64 consecutive x of one z row (xl is the lane)
+    const bool live = x < X;
 #pragma unroll
     for (int k = 0; k < BP_ZT; ++k) {
         const int z = zt0 + 4 * k;
-        if (z >= J.T) continue;
+        if (z >= J.T) continue;  // uniform in the wave
         float *vslice = J.vol + ((size_t)yb * J.T + z) * X * RB;
         float v[RB];
-        Slice<float>::load(vslice, X, x, v);
-        if constexpr (MODE == BACK_SIRT) {
-            const float f = J.gamma * J.inv_c[z * X + x];
 #pragma unroll
-            for (int r = 0; r < RB; ++r) v[r] = fmaf(val[k][r], f, v[r]);
-        } else if constexpr (MODE == BACK_SART) {
-            if (!(cs[k] > J.thr_upd)) continue;
+        for (int r = 0; r < RB; ++r) v[r] = 0.f;
+        if (live) {
+            Slice<float>::load(vslice, X, x, v);
+            bool upd = true;
+            if constexpr (MODE == BACK_SIRT) {
+                const float f = J.gamma * J.inv_c[z * X + x];
 #pragma unroll
-            for (int r = 0; r < RB; ++r) v[r] = fmaf(val[k][r], J.gamma, v[r]);
-        } else {
+                for (int r = 0; r < RB; ++r) v[r] = fmaf(val[k][r], f, v[r]);
+            } else if constexpr (MODE == BACK_SART) {
+                upd = cs[k] > J.thr_upd;
+                if (upd) {
 #pragma unroll
-            for (int r = 0; r < RB; ++r) v[r] += val[k][r];
+                    for (int r = 0; r < RB; ++r) v[r] = fmaf(val[k][r], J.gamma, v[r]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < RB; ++r) v[r] += val[k][r];
+            }
+            if (upd) Slice<float>::store(vslice, X, x, v);
         }
-        Slice<float>::store(vslice, X, x, v);
+        if constexpr (HALF && MODE != BACK_BPT) {
+            // Keep the half copy in step: lane xl writes pair entry x (cells
+            // x and x+1, the latter from lane xl+1); the entry straddling two
+            // tiles is written in halves, its first cell by this tile's last
+            // lane and its second by the next tile's first lane.
+            float vn[RB];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) vn[r] = __shfl_down(v[r], 1, BP_X);
+            const int ne = X + 1;
+            char *hs = static_cast<char *>(J.hvol) + ((size_t)yb * J.T + z) * 4 * (size_t)ne * 16;
+            if (live && xl < BP_X - 1) {
+                const bool next = x + 1 < X;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    *reinterpret_cast<uint4 *>(hs + ((size_t)q * ne + x + 1) * 16) =
+                        make_uint4(pack_half2(v[4 * q], next ? vn[4 * q] : 0.f),
+                                   pack_half2(v[4 * q + 1], next ? vn[4 * q + 1] : 0.f),
+                                   pack_half2(v[4 * q + 2], next ? vn[4 * q + 2] : 0.f),
+                                   pack_half2(v[4 * q + 3], next ? vn[4 * q + 3] : 0.f));
+            } else if (live) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int h = 0; h < 4; ++h)
+                        *reinterpret_cast<_Float16 *>(hs + ((size_t)q * ne + x + 1) * 16 + 4 * h) =
+                            (_Float16)v[4 * q + h];
+            }
+            if (live && xl == 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int h = 0; h < 4; ++h)
+                        *reinterpret_cast<_Float16 *>(hs + ((size_t)q * ne + x) * 16 + 4 * h + 2) =
+                            (_Float16)v[4 * q + h];
+            }
+        }
     }
 }
 
@@ -1379,6 +1495,13 @@ hipError_t launch_weights(const JobDesc &h, const JobDesc *d, hipStream_t st)
     return hipGetLastError();
 }
 
+hipError_t launch_shadow(const JobDesc &h, const JobDesc *d, hipStream_t st)
+{
+    const size_t blocks = cdiv((size_t)h.X + 1, WG) * h.T * h.nyb;
+    hipLaunchKernelGGL(k_shadow, dim3((unsigned)blocks), dim3(WG), 0, st, d);
+    return hipGetLastError();
+}
+
 // Forward-projection kernel for value projections: 0 direct gather,
 // 1 one wave per tilt with an LDS band, 2 tilt groups sharing a band.
 int forward_kind()
@@ -1408,12 +1531,18 @@ hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool 
     if (storage < 0) {
         hipLaunchKernelGGL((k_forward<float, false, true>), dim3(nblk), dim3(WG), 0, st, L, nblk);
     } else if (kind == 2) {
+        // half storage reads the half copy of the volume
         const dim3 b(FW_PIX * FG);
-        switch (fused ? storage : -1) {
-        case TREC_STORE_F16: hipLaunchKernelGGL((k_forward_g<_Float16, true>), dim3(nblk), b, 0, st, L, nblk); break;
-        case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward_g<__bf16, true>), dim3(nblk), b, 0, st, L, nblk); break;
-        case TREC_STORE_F32: hipLaunchKernelGGL((k_forward_g<float, true>), dim3(nblk), b, 0, st, L, nblk); break;
-        default: hipLaunchKernelGGL((k_forward_g<float, false>), dim3(nblk), b, 0, st, L, nblk); break;
+        const bool hv = storage == TREC_STORE_F16;
+        if (!fused) {
+            if (hv) hipLaunchKernelGGL((k_forward_g<float, false, true>), dim3(nblk), b, 0, st, L, nblk);
+            else hipLaunchKernelGGL((k_forward_g<float, false, false>), dim3(nblk), b, 0, st, L, nblk);
+        } else {
+            switch (storage) {
+            case TREC_STORE_F16: hipLaunchKernelGGL((k_forward_g<_Float16, true, true>), dim3(nblk), b, 0, st, L, nblk); break;
+            case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward_g<__bf16, true, false>), dim3(nblk), b, 0, st, L, nblk); break;
+            default: hipLaunchKernelGGL((k_forward_g<float, true, false>), dim3(nblk), b, 0, st, L, nblk); break;
+            }
         }
     } else if (kind == 0) {
         switch (fused ? storage : -1) {

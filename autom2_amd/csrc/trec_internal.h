@@ -97,6 +97,10 @@ struct JobDesc {
     float *inv_rwt;          // [P][X]
     float *inv_c;            // [T][X]
     float *rwt;              // [P][X] raw ray weights, or NULL
+    // half storage (iterative methods): a half copy of vol for the forward
+    // projection, as (c, c+1) pairs: entry c + 1 of quad q of slice (yb, z)
+    // holds rows 4q .. 4q+3 of cells c and c+1, c in [-1, X); else NULL
+    void *hvol;              // [nyb][T][4][X + 1] x 16 bytes
 };
 
 // A multi-job launch: jobs[k] owns work-groups [off[k], off[k+1]).
@@ -115,6 +119,8 @@ hipError_t launch_block_volume(const JobDesc &h, const JobDesc *d, const float *
 hipError_t launch_zero(float *p, size_t n, hipStream_t st);
 hipError_t launch_unblock_volume(const JobDesc &h, const JobDesc *d, float *dst, int layout, hipStream_t st);
 hipError_t launch_weights(const JobDesc &h, const JobDesc *d, hipStream_t st);
+// builds hvol from vol
+hipError_t launch_shadow(const JobDesc &h, const JobDesc *d, hipStream_t st);
 hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st);
 hipError_t launch_combine(const LaunchList &L, unsigned nblk, int storage, hipStream_t st);
 hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode, hipStream_t st);
