@@ -463,6 +463,7 @@ unsigned forward_wave_blocks(const JobDesc &h, int ntheta)
 #define FW_WAVES 4  // waves per SIMD the register budget is sized for
 #endif
 
+
 template <int NC, int S>
 __device__ __forceinline__ void fw_gather(const float4 (*band)[S], int cl, int ncand, float d0, float a1f,
                                           float (&acc)[RB])
@@ -474,6 +475,10 @@ __device__ __forceinline__ void fw_gather(const float4 (*band)[S], int cl, int n
         // hoist every read of the layer and run out of registers
         if (c > 0 && c % 2 == 0) __builtin_amdgcn_sched_barrier(0);
         const float wt = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
+        // the last candidate has weight 0 for most pixels unless 2 / a1 is
+        // near a whole number: those lanes skip its reads (adding 0 * v
+        // changes nothing), which relieves the LDS banks
+        if (NC && c == n - 1 && !(wt > 0.f)) continue;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const float4 v = band[q][cl + c];
@@ -497,6 +502,8 @@ __device__ __forceinline__ void fw_gather_h(const float4 (*band)[S], int cl, int
     for (int c = 0; c < n; c += 2) {
         const float wa = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
         const float wb = c + 1 < n ? fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)(c + 1), d0))) : 0.f;
+        // as in fw_gather: lanes whose last pair weighs nothing skip it
+        if (NC && c + 2 >= n && !(wa > 0.f) && !(wb > 0.f)) continue;
         const half2v w = {(_Float16)wa, (_Float16)wb};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -681,7 +688,8 @@ unsigned forward_group_blocks(const JobDesc &h, int ntheta)
 // ulo + uw the greatest end over the group's tilts, each computed with the
 // expressions the pixels use, so every lane's candidates lie inside it.
 template <typename PT, bool FUSED, bool HV>
-__global__ __launch_bounds__(FW_PIX *FG) void k_forward_g(const LaunchList L, unsigned nblk)
+__global__ __launch_bounds__(FW_PIX *FG) __attribute__((amdgpu_waves_per_eu(6))) void k_forward_g(
+    const LaunchList L, unsigned nblk)
 {
     __shared__ float4 band[4][FGS];
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
@@ -926,7 +934,7 @@ unsigned back_blocks(const JobDesc &h)
 }
 
 // Voxel-driven back-projection of one 64 x BP_Z voxel tile of one row block.
-// Per pass, the residual cells the tile can touch for BP_G tilts are staged
+// Per pass, the residual cells the tile can touch for G tilts are staged
 // in LDS as float planes (half pairs for half storage); every thread then
 // gathers its voxels' two taps per tilt from LDS with the exact double
 // geometry.  MODE selects the update (BackMode).  For BACK_SIRT and
@@ -938,8 +946,9 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
     // half storage keeps the residual in half in LDS: entry (t, q, c) holds
     // rows 4q .. 4q+3 of cells c and c+1 as (c, c+1) pairs
     constexpr bool HALF = std::is_same<PT, _Float16>::value;
-    __shared__ float4 win[BP_G][4][BP_W];
-    __shared__ int gmin_s[BP_G];
+    constexpr int G = sizeof(PT) == 4 ? BP_G32 : BP_G;  // tilts per pass
+    __shared__ float4 win[G][4][BP_W];
+    __shared__ int gmin_s[G];
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
     const int j = find_job(L, b0);
     KJob &J = *kconst(L.job[j]);
@@ -973,8 +982,8 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
         for (int r = 0; r < RB; ++r) val[k][r] = 0.f;
     }
 
-    for (int tg = th0; tg < th1; tg += BP_G) {
-        const int ng = min(BP_G, th1 - tg);
+    for (int tg = th0; tg < th1; tg += G) {
+        const int ng = min(G, th1 - tg);
         __syncthreads();  // the previous pass is done with win / gmin_s
         if ((int)threadIdx.x < ng) {
             // lowest tap of the tile: u is monotone in x and z, so a corner

@@ -37,7 +37,7 @@ static_assert(FW_S >= ((FW_W + FW_G - 1) / FW_G) * FW_G, "band loads must fit th
 // Tilt groups: FG tilts (one wave each) share an LDS band of FGS cells per
 // plane, a whole number of 16-cell load chunks.
 constexpr int FG = 4;
-constexpr int FGS = 608;
+constexpr int FGS = 400;  // 25 KB of LDS per group: six groups (24 waves) fit a CU
 static_assert(FGS % FW_G == 0, "the group band is loaded in whole chunks");
 // Back-projection tile: 64 voxels along x (one wave) by BP_Z along z; each
 // thread carries BP_ZT voxels of one x column (z, z+4, ...).  Residuals of
@@ -46,7 +46,8 @@ static_assert(FGS % FW_G == 0, "the group band is loaded in whole chunks");
 constexpr int BP_X = 64;
 constexpr int BP_ZT = 4;
 constexpr int BP_Z = (WG / BP_X) * BP_ZT;
-constexpr int BP_G = 8;
+constexpr int BP_G = 8;  // 16-bit storage; float passes stage BP_G32
+constexpr int BP_G32 = 6;  // fewer tilts, less LDS: four work-groups per CU
 constexpr int BP_W = BP_X + BP_Z + 4;
 
 // Largest number of jobs one multi-job launch carries.
