@@ -274,7 +274,7 @@ def main():
     dom = max(("forward", "back"), key=lambda k: kern[k]["ms_total"])
     achieved = alg[dom] / (kern[dom]["ms_avg"] / 1e3) / 1e9
     fused = kern["combine"]["launches"] == 0
-    kname = {"forward": f"forward_g<float, {'true' if fused else 'false'}>", "back": "back<float, 0>"}[dom]
+    kname = {"forward": f"forward_g<float, {'true' if fused else 'false'}, false>", "back": "back<float, 0>"}[dom]
     traffic = profiled_traffic(kname) if args.config == "sirt1024" else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
