@@ -232,3 +232,26 @@ def test_bad_arguments_fail_loudly(torch_cuda, trec, ctx):
     with pytest.raises(trec.TrecError) as ei:
         ctx.reconstruct(g, [0, 1, 2], s, trec.make_method("ADMM,5,3,1.0,0.1"), out)
     assert ei.value.status == trec.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("method", ["SIRT,3,1.0", "SART,1,0.5"])
+def test_half_volume_copy_edges(torch_cuda, trec, ctx, orc, method):
+    """Half storage projects a half copy of the volume kept as (c, c+1) pair
+    entries: built at preparation, rewritten by every back-projection with
+    the entries at 64-voxel tile seams written in halves by two tiles, plus
+    an entry before cell 0.  Spikes in the starting volume at the detector's
+    edges and at the tile seam (x = 63, 64) make any entry written wrong or
+    left stale show up far above the half-precision error."""
+    P, Y, T, X = 9, 16, 10, 70
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=21)
+    init = smooth_volume(Y, T, X, 22)
+    for x in (0, 1, 62, 63, 64, 65, 68, 69):
+        init[:, :, x] += 3.0
+    got = run(torch_cuda, trec, ctx, stack, ang, T, method, initial=init, storage="f16")
+    f32 = run(torch_cuda, trec, ctx, stack, ang, T, method, initial=init)
+    name, it, gamma = method.split(",")
+    fn = orc.sirt_f64 if name == "SIRT" else orc.sart_f64
+    ref64 = fn(coef, stack.astype(np.float64), T, org, int(it), float(gamma), vol=init.astype(np.float64))
+    e64, e32 = orc.rel_l2(got, ref64), orc.rel_l2(got, f32)
+    print(f"{method} half copy: rel L2 vs FP64 {e64:.3e}, vs float storage {e32:.3e}")
+    assert e64 < 1e-3 and e32 < 1e-3
