@@ -1017,24 +1017,34 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
                                (a.w & 0xffffu) | (b.w << 16), (a.w >> 16) | (b.w & 0xffff0000u));
             }
         }
-        // float and bfloat16: rows 4q .. 4q+3 of cell g as floats
-        for (int v = threadIdx.x; !HALF && v < ng * 4 * BP_W; v += WG) {
+        if constexpr (!HALF && sizeof(PT) == 2) {
+            // bfloat16: plane p (rows 8p .. 8p+7) of cell g -> entries 2p and
+            // 2p + 1 as floats
+            for (int v = threadIdx.x; v < ng * 2 * BP_W; v += WG) {
+                const int t = v / (2 * BP_W);
+                const int rem = v % (2 * BP_W);
+                const int pl = rem / BP_W, c = rem % BP_W;
+                const int g = gmin_s[t] + c;
+                float e[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) e[k] = 0.f;
+                if ((unsigned)g < (unsigned)X)
+                    Slice<PT>::unpack(*reinterpret_cast<const uint4 *>(imgb + (size_t)(tg + t) * X * RB +
+                                                                       ((size_t)pl * X + g) * 8),
+                                      e);
+                win[t][2 * pl][c] = make_float4(e[0], e[1], e[2], e[3]);
+                win[t][2 * pl + 1][c] = make_float4(e[4], e[5], e[6], e[7]);
+            }
+        }
+        // float: rows 4q .. 4q+3 of cell g
+        for (int v = threadIdx.x; sizeof(PT) == 4 && v < ng * 4 * BP_W; v += WG) {
             const int t = v / (4 * BP_W);
             const int rem = v % (4 * BP_W);
             const int q = rem / BP_W, c = rem % BP_W;
             const int g = gmin_s[t] + c;
-            const PT *sl = imgb + (size_t)(tg + t) * X * RB;
             float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-            if ((unsigned)g < (unsigned)X) {
-                if constexpr (sizeof(PT) == 4) {
-                    o = *reinterpret_cast<const float4 *>(sl + ((size_t)q * X + g) * 4);
-                } else {
-                    float e[8];
-                    Slice<PT>::unpack(*reinterpret_cast<const uint4 *>(sl + ((size_t)(q / 2) * X + g) * 8), e);
-                    const int h = (q % 2) * 4;
-                    o = make_float4(e[h], e[h + 1], e[h + 2], e[h + 3]);
-                }
-            }
+            if ((unsigned)g < (unsigned)X)
+                o = *reinterpret_cast<const float4 *>(imgb + (size_t)(tg + t) * X * RB + ((size_t)q * X + g) * 4);
             win[t][q][c] = o;
         }
         __syncthreads();
