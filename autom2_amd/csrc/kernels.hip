@@ -441,29 +441,11 @@ __global__ __launch_bounds__(WG) void k_forward(const LaunchList L, unsigned nbl
     fp_epilogue<PT, FUSED, ONES>(J, th, zc, yb, i, acc);
 }
 
-unsigned forward_wave_blocks(const JobDesc &h, int ntheta)
-{
-    const unsigned npb = (h.X + FW_PIX - 1) / FW_PIX;
-    return npb * (unsigned)ntheta * (unsigned)h.nzc * (unsigned)h.nyb;
-}
-
-// Forward projection, one wave per work-group, with the voxel band of each
-// z layer staged in LDS.  The wave's 64 pixels of tilt t see, in layer z,
-// the voxels [xlo(z), xlo(z) + w(z)); the wave loads that band once (plane
-// by plane, fully coalesced; layer z + 1 goes to registers while layer z is
-// summed out of LDS) and every pixel then reads its NC candidates from LDS
-// (NC = ncand as a compile-time count; 0 = run-time loop).  Voxels outside
-// [0, X) load as 0.  A band wider than FW_W (tilts past about 60 degrees) is
-// gathered straight from L2 instead.
-//
-// Every lane's candidates lie inside the band: xf(i) is monotone in i and the
-// band is bounded by the first and last pixel's windows computed with the
-// same expressions, so 0 <= xf - xlo and xf - xlo + ncand <= w.
-#ifndef FW_WAVES
-#define FW_WAVES 4  // waves per SIMD the register budget is sized for
-#endif
-
-
+// Candidate gathers of the forward projection.  For pixel i of a tilt, the
+// voxels of layer z with weight > 0 are cells xf .. xf + ncand - 1 of the
+// layer's LDS band (cl = xf - first cell of the band); candidate c sits at
+// distance d0 + a1 c from the pixel and weighs max(0, 1 - |d|).  NC is
+// ncand as a compile-time count (0 = run-time loop).
 template <int NC, int S>
 __device__ __forceinline__ void fw_gather(const float4 (*band)[S], int cl, int ncand, float d0, float a1f,
                                           float (&acc)[RB])
@@ -516,145 +498,6 @@ __device__ __forceinline__ void fw_gather_h(const float4 (*band)[S], int cl, int
     }
 }
 
-template <typename PT, bool FUSED, int NC>
-__device__ __forceinline__ void fw_body(KJob &J, int th, int zc, int yb, int i0, float4 (*band)[FW_S])
-{
-    const int lane = threadIdx.x;
-    const int i = i0 + lane;
-    KTilt &t = kconst(J.tc)[th];
-    const int X = J.X;
-    const int z0 = zc * J.zc_len;
-    const int z1 = min(J.T, z0 + J.zc_len);
-    const double slope = t.slope, inv_a1 = t.inv_a1, c0 = t.c0;
-    const float a1f = t.a1f, a1m1 = t.a1f - 1.0f;
-    const int ncand = NC ? NC : t.ncand;
-    const double sbase = ((double)(i - 1) - c0) * inv_a1;
-    const double s_lo = ((double)(i0 - 1) - c0) * inv_a1;
-    const double s_hi = ((double)(i0 + FW_PIX - 2) - c0) * inv_a1;
-    const float *vbase = J.vol + (size_t)yb * J.T * X * RB;
-
-    float acc[RB];
-#pragma unroll
-    for (int r = 0; r < RB; ++r) acc[r] = 0.f;
-
-    // first cell and width of the band of layer z (wave-uniform)
-    auto band_of = [&](int z, int &xlo, int &w) {
-        const int lo = (int)floor(fma(slope, (double)z, s_lo)) + 1;
-        const int hi = (int)floor(fma(slope, (double)z, s_hi)) + 1 + ncand;
-        xlo = __builtin_amdgcn_readfirstlane(lo);
-        w = __builtin_amdgcn_readfirstlane(hi - lo);
-    };
-    // band loads per layer: ncand = 2, 3, 4 means a1 >= 1, 2/3, 1/2, so
-    // w <= floor(63 / a1) + 1 + ncand <= 66, 98, 131 cells
-    constexpr int KN = NC == 2 ? 5 : NC == 3 ? 7 : (FW_W + FW_G - 1) / FW_G;
-    int xlo, w;
-    band_of(z0, xlo, w);
-    if (w + 2 > KN * FW_G) {  // floors of rounded ends: the width varies by at most two cells between layers
-        for (int z = z0; z < z1; ++z) {
-            const double s = fma(slope, (double)z, sbase);
-            const double fl = floor(s);
-            const float frac = (float)(s - fl);
-            const int xf = (int)fl + 1;
-            const float d0 = fmaf(-a1f, frac, a1m1);
-            const BufSlice<float> vz(vbase + (size_t)z * X * RB, X);
-            for (int c = 0; c < ncand; ++c) {
-                const float wt = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
-                float v[RB];
-                vz.load(xf + c, v);
-#pragma unroll
-                for (int r = 0; r < RB; ++r) acc[r] = fmaf(wt, v[r], acc[r]);
-            }
-        }
-    } else {
-        // The band is loaded 16 cells at a time: lane l of load m carries
-        // plane l / 16 of cell 16 m + l % 16, so a load moves 1 KB as four
-        // 256-byte runs (neighbouring lanes stay in one cache line) and the
-        // stores of 16 lanes fill one 256-byte bank row.  Cells past w are
-        // loaded too (in bounds or zero) but never read.
-        constexpr int NL = KN;
-        float4 pre[NL];
-        const int lq = lane >> 4, lc = lane & 15;
-        const unsigned slice_bytes = (unsigned)X * RB * 4;
-        // All NL loads are always issued (any past the band with an
-        // out-of-range offset, which returns zeros without touching memory):
-        // a fixed count of loads in flight lets each store wait for exactly
-        // its own load.
-        auto fetch = [&](int z, int xl, int kn) {
-            const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)(vbase + (size_t)z * X * RB), (short)0,
-                                                              (int)slice_bytes, BUF_DWORD3);
-#pragma unroll
-            for (int m = 0; m < NL; ++m) {
-                const int x = xl + m * FW_G + lc;
-                const bool live = m < kn && (unsigned)x < (unsigned)X;
-                const unsigned off = live ? ((unsigned)lq * X + x) * 16u : slice_bytes;
-                const auto raw = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off, 0, 0);
-                pre[m] = make_float4(__uint_as_float(raw[0]), __uint_as_float(raw[1]), __uint_as_float(raw[2]),
-                                     __uint_as_float(raw[3]));
-            }
-        };
-        auto stash = [&](int kn) {
-#pragma unroll
-            for (int m = 0; m < NL; ++m)
-                if (m < kn) band[lq][m * FW_G + lc] = pre[m];
-        };
-        auto chunks = [](int wl) { return (wl + FW_G - 1) / FW_G; };
-        fetch(z0, xlo, chunks(w));
-        stash(chunks(w));
-        for (int z = z0; z < z1; ++z) {
-            int xln = 0, wn = 0;
-            if (z + 1 < z1) {
-                band_of(z + 1, xln, wn);
-                fetch(z + 1, xln, chunks(wn));
-            }
-            __syncthreads();  // band holds layer z
-            const double s = fma(slope, (double)z, sbase);
-            const double fl = floor(s);
-            const float frac = (float)(s - fl);
-            const int cl = (int)fl + 1 - xlo;
-            const float d0 = fmaf(-a1f, frac, a1m1);
-            fw_gather<NC, FW_S>(band, cl, ncand, d0, a1f, acc);
-            // keep the sums of layer z ahead of the wait for layer z + 1's
-            // loads (they are what hides that wait): the empty asm needs the
-            // accumulators here, so the compiler cannot sink the FMAs past
-            // the stash
-#pragma unroll
-            for (int r = 0; r < RB; ++r) asm volatile("" : "+v"(acc[r]));
-            __syncthreads();  // every lane is done with layer z
-            if (z + 1 < z1) stash(chunks(wn));
-            xlo = xln;
-        }
-    }
-    if (i >= X) return;
-    fp_epilogue<PT, FUSED, false>(J, th, zc, yb, i, acc);
-}
-
-template <typename PT, bool FUSED>
-__global__ __launch_bounds__(FW_PIX) __attribute__((amdgpu_waves_per_eu(FW_WAVES))) void k_forward_w(
-    const LaunchList L, unsigned nblk)
-{
-    __shared__ float4 band[4][FW_S];
-    const unsigned b0 = xcd_remap(blockIdx.x, nblk);
-    const int j = find_job(L, b0);
-    KJob &J = *kconst(L.job[j]);
-    unsigned lb = b0 - L.off[j];
-    const int ntheta = L.ntheta[j] ? L.ntheta[j] : J.P;
-    const unsigned npb = (J.X + FW_PIX - 1) / FW_PIX;
-    const int pb = lb % npb;
-    lb /= npb;
-    const int th = L.theta0[j] + (int)(lb % ntheta);
-    lb /= ntheta;
-    const int zc = lb % J.nzc;
-    const int yb = lb / J.nzc;
-    const int i0 = pb * FW_PIX;
-    switch (kconst(J.tc)[th].ncand) {
-    case 2: fw_body<PT, FUSED, 2>(J, th, zc, yb, i0, band); break;
-    case 3: fw_body<PT, FUSED, 3>(J, th, zc, yb, i0, band); break;
-    case 4: fw_body<PT, FUSED, 4>(J, th, zc, yb, i0, band); break;
-    case 5: fw_body<PT, FUSED, 5>(J, th, zc, yb, i0, band); break;
-    default: fw_body<PT, FUSED, 0>(J, th, zc, yb, i0, band); break;
-    }
-}
-
 // ------------------------------------------------------------------ forward, tilt groups
 
 // Uniform double in scalar registers.
@@ -681,7 +524,7 @@ unsigned forward_group_blocks(const JobDesc &h, int ntheta)
 // LDS stores per tilt by up to FG.  Unions wider than the LDS band (tilts
 // past about 60 degrees far from the tilt axis) are gathered straight from
 // L2 by every wave instead.  Per pixel, the sums and their order are those
-// of k_forward and k_forward_w.  HV (half storage) stages the half copy of
+// of k_forward.  HV (half storage) stages the half copy of
 // the volume, hvol, whose pair entries the waves apply with fw_gather_h.
 //
 // The union of layer z is [ulo, ulo + uw) with ulo the least first cell and
@@ -1521,35 +1364,26 @@ hipError_t launch_shadow(const JobDesc &h, const JobDesc *d, hipStream_t st)
     return hipGetLastError();
 }
 
-// Forward-projection kernel for value projections: 0 direct gather,
-// 1 one wave per tilt with an LDS band, 2 tilt groups sharing a band.
-int forward_kind()
+// Forward-projection kernel for value projections: tilt groups sharing an
+// LDS band, or (TREC_FP_DIRECT set) the direct gather, kept as a reference
+// path for comparisons.
+static bool forward_direct()
 {
-    static const int kind = [] {
-        const char *e = getenv("TREC_FP_KERNEL");
-        if (getenv("TREC_FP_DIRECT")) return 0;
-        const int v = e ? atoi(e) : 2;
-        return v >= 0 && v <= 2 ? v : 2;
-    }();
-    return kind;
+    static const bool direct = getenv("TREC_FP_DIRECT") != nullptr;
+    return direct;
 }
 
 unsigned fwd_value_blocks(const JobDesc &h, int ntheta)
 {
-    switch (forward_kind()) {
-    case 0: return forward_blocks(h, ntheta);
-    case 1: return forward_wave_blocks(h, ntheta);
-    default: return forward_group_blocks(h, ntheta);
-    }
+    return forward_direct() ? forward_blocks(h, ntheta) : forward_group_blocks(h, ntheta);
 }
 
 hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st)
 {
     // storage < 0 selects the splat of ones (Rwt)
-    const int kind = forward_kind();
     if (storage < 0) {
         hipLaunchKernelGGL((k_forward<float, false, true>), dim3(nblk), dim3(WG), 0, st, L, nblk);
-    } else if (kind == 2) {
+    } else if (!forward_direct()) {
         // half storage reads the half copy of the volume
         const dim3 b(FW_PIX * FG);
         const bool hv = storage == TREC_STORE_F16;
@@ -1563,19 +1397,12 @@ hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool 
             default: hipLaunchKernelGGL((k_forward_g<float, true, false>), dim3(nblk), b, 0, st, L, nblk); break;
             }
         }
-    } else if (kind == 0) {
+    } else {
         switch (fused ? storage : -1) {
         case TREC_STORE_F16: hipLaunchKernelGGL((k_forward<_Float16, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
         case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward<__bf16, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
         case TREC_STORE_F32: hipLaunchKernelGGL((k_forward<float, true, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
         default: hipLaunchKernelGGL((k_forward<float, false, false>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
-        }
-    } else {
-        switch (fused ? storage : -1) {
-        case TREC_STORE_F16: hipLaunchKernelGGL((k_forward_w<_Float16, true>), dim3(nblk), dim3(FW_PIX), 0, st, L, nblk); break;
-        case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward_w<__bf16, true>), dim3(nblk), dim3(FW_PIX), 0, st, L, nblk); break;
-        case TREC_STORE_F32: hipLaunchKernelGGL((k_forward_w<float, true>), dim3(nblk), dim3(FW_PIX), 0, st, L, nblk); break;
-        default: hipLaunchKernelGGL((k_forward_w<float, false>), dim3(nblk), dim3(FW_PIX), 0, st, L, nblk); break;
         }
     }
     return hipGetLastError();

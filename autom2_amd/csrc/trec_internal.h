@@ -22,20 +22,15 @@ constexpr int RB = 16;
 
 // Threads per work-group of the row-blocked kernels.
 constexpr int WG = 256;
-// Pixels per forward-projection work-group; the voxel band the group's
-// pixels see in one z layer is staged in LDS when it fits FP_W cells
-// (255 / a1 + ncand + 1 <= FP_W, i.e. |tilt| up to about 60 degrees).
+// Direct forward projection (ray weights, and the TREC_FP_DIRECT reference
+// path): FP_PIX pixels of one tilt per work-group, gathered straight from L2.
 constexpr int FP_PIX = 256;
-// The LDS-staged forward projection runs one wave per work-group: 64 pixels,
-// whose band in one layer is at most 64 / a1 + ncand + 1 <= FW_W cells for
-// a1 >= 0.5 (|tilt| <= 60 degrees); wider bands take the direct path.
+// Tilt-group forward projection: FG neighbouring tilts (one wave each) of
+// the same FW_PIX pixels share an LDS band of FGS cells per plane, loaded in
+// chunks of FW_G cells.  Unions wider than that (tilts past about 60 degrees
+// far from the tilt axis) take the direct path.
 constexpr int FW_PIX = 64;
-constexpr int FW_W = 144;
-constexpr int FW_G = 16;   // cells per band load
-constexpr int FW_S = 148;  // LDS plane stride in cells (2368 bytes: planes start in different banks)
-static_assert(FW_S >= ((FW_W + FW_G - 1) / FW_G) * FW_G, "band loads must fit the plane");
-// Tilt groups: FG tilts (one wave each) share an LDS band of FGS cells per
-// plane, a whole number of 16-cell load chunks.
+constexpr int FW_G = 16;
 constexpr int FG = 4;
 constexpr int FGS = 400;  // 25 KB of LDS per group: six groups (24 waves) fit a CU
 static_assert(FGS % FW_G == 0, "the group band is loaded in whole chunks");
@@ -133,7 +128,6 @@ hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipSt
 
 // Work-group counts of the multi-job kernels for one job.
 unsigned forward_blocks(const JobDesc &h, int ntheta);
-unsigned forward_wave_blocks(const JobDesc &h, int ntheta);
 unsigned forward_group_blocks(const JobDesc &h, int ntheta);
 // work-groups of a value forward projection for the kernel launch_forward picks
 unsigned fwd_value_blocks(const JobDesc &h, int ntheta);
