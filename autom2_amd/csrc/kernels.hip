@@ -147,9 +147,12 @@ struct BufSlice {
     }
 };
 
-// Exact TiltRec-mpi geometry (ValCoef, main-mpi.cpp:204-226) at pitch 0:
+// TiltRec-mpi geometry (ValCoef, main-mpi.cpp:204-226) at pitch 0:
 // u = ((a0 + a1 * X) + a3 * Z) + ox in double, taps floor(u) and floor(u)+1,
-// fraction rounded to float as Weight::x_min_del is.
+// fraction rounded to float as Weight::x_min_del is.  The compiler fuses the
+// products into the sums, so u may differ from the reference's by an ulp; a
+// tap index can then change only where u is within an ulp of an integer,
+// where both choices put the same weights (to 1e-16) on the same cells.
 template <typename TC>
 __device__ __forceinline__ void exact_tap(const TC &t, double Xc, double Zc, int &g, float &dx)
 {
