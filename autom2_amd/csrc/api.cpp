@@ -49,8 +49,8 @@ std::string tune_key(const trec_geom &g, int storage)
 namespace trec {
 
 // Event timing of launch groups (trec_profile_*).
-enum ProfKind { PROF_PREP = 0, PROF_FORWARD, PROF_COMBINE, PROF_BACK, PROF_FINISH, PROF_N };
-const char *const prof_names[PROF_N] = {"prep", "forward", "combine", "back", "finish"};
+enum ProfKind { PROF_PREP = 0, PROF_FORWARD, PROF_COMBINE, PROF_BACK, PROF_FINISH, PROF_XF_MEAN, PROF_XF_WARP, PROF_N };
+const char *const prof_names[PROF_N] = {"prep", "forward", "combine", "back", "finish", "xf_mean", "xf_warp"};
 
 struct Prof {
     bool on = false;
@@ -780,6 +780,78 @@ size_t request_bytes(const trec_geom &g, const trec_method &m)
 
 }  // namespace trec
 
+
+// ------------------------------------------------------------------ .xf application
+
+namespace trec {
+
+// OpenCV's 1-D interpolation kernels at fraction x (imgwarp.cpp
+// interpolateCubic / interpolateLanczos4), evaluated in the same float and
+// double operations on the host, so the tables the device reads are the
+// ones warpAffine builds.
+static void xf_cubic(float x, float *c)
+{
+#pragma clang fp contract(off)
+    const float A = -0.75f;
+    c[0] = ((A * (x + 1) - 5 * A) * (x + 1) + 8 * A) * (x + 1) - 4 * A;
+    c[1] = ((A + 2) * x - (A + 3)) * x * x + 1;
+    c[2] = ((A + 2) * (1 - x) - (A + 3)) * (1 - x) * (1 - x) + 1;
+    c[3] = 1.f - c[0] - c[1] - c[2];
+}
+
+// sin(y_k) / y_k^2 for the 8 taps through the angle-sum identity with one
+// sin/cos pair, normalised to sum 1; a tap at distance 0 weighs 1e30 before
+// normalisation.
+static void xf_lanczos4(float x, float *c)
+{
+#pragma clang fp contract(off)
+    static const double s45 = 0.70710678118654752440084436210485;
+    static const double cs[8][2] = {{1, 0}, {-s45, -s45}, {0, 1}, {s45, -s45},
+                                    {-1, 0}, {s45, s45}, {0, -1}, {-s45, s45}};
+    const double pi = 3.1415926535897932384626433832795;
+    float sum = 0;
+    const double y0 = -(double)(x + 3) * pi * 0.25, s0 = std::sin(y0), c0 = std::cos(y0);
+    for (int k = 0; k < 8; ++k) {
+        const float d = x + 3 - k;
+        if (std::fabs(d) >= 1e-6f) {
+            const double y = -(double)d * pi * 0.25;
+            c[k] = (float)((cs[k][0] * s0 + cs[k][1] * c0) / (y * y));
+        } else {
+            c[k] = 1e30f;
+        }
+        sum += c[k];
+    }
+    sum = 1.f / sum;
+    for (int k = 0; k < 8; ++k) c[k] *= sum;
+}
+
+// mrcstack's forward matrix in float (main.cpp:56-60), then warpAffine's
+// inversion in double.
+static XfConst xf_inverse(const float *xf, int W, int H)
+{
+#pragma clang fp contract(off)
+    const float a11 = xf[0], a12 = xf[1], a21 = xf[2], a22 = xf[3], dx = xf[4], dy = xf[5];
+    const float cx = W * .5, cy = H * .5;
+    const float f[6] = {a11, a12, cx + dx - a11 * cx - a12 * cy, a21, a22, cy + dy - a21 * cx - a22 * cy};
+    XfConst r;
+    double *M = r.m;
+    for (int k = 0; k < 6; ++k) M[k] = f[k];
+    double D = M[0] * M[4] - M[1] * M[3];
+    D = D != 0 ? 1. / D : 0;
+    const double A11 = M[4] * D, A22 = M[0] * D;
+    M[0] = A11;
+    M[1] *= -D;
+    M[3] *= -D;
+    M[4] = A22;
+    const double b1 = -M[0] * M[2] - M[1] * M[5];
+    const double b2 = -M[3] * M[2] - M[4] * M[5];
+    M[2] = b1;
+    M[5] = b2;
+    return r;
+}
+
+}  // namespace trec
+
 // ------------------------------------------------------------------ C ABI
 
 extern "C" {
@@ -938,6 +1010,111 @@ trec_status trec_filter_rweight(trec_context *ctx, float *rows, int nx, long lon
     HIPCHK(ctx, hipMemcpyAsync(ctx->arena, ctx->staging, sizeof(float) * N, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipEventRecord(ctx->staging_ev, st));
     HIPCHK(ctx, launch_rweight(rows, (const float *)ctx->arena, N, nx, nrows, 0, 1, 1, nx, st));
+    return TREC_OK;
+}
+
+trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const float *xf_host, int mode,
+                          const float *in, float *out, void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    if (nx < 1 || ny < 1 || nsec < 1 || nsec > 65535)
+        return fail(ctx, TREC_ERR_INVALID, "bad stack size %d x %d x %d", nx, ny, nsec);
+    if ((size_t)nx * ny > ((size_t)1 << 29))
+        return fail(ctx, TREC_ERR_INVALID, "a section of %d x %d pixels exceeds 2^29", nx, ny);
+    if (!xf_host || !in || !out) return fail(ctx, TREC_ERR_INVALID, "null argument");
+    const size_t n = (size_t)nx * ny;
+    if (out < in + n * nsec && in < out + n * nsec)
+        return fail(ctx, TREC_ERR_INVALID, "input and output stacks overlap");
+    std::vector<XfConst> xc(nsec);
+    // Row and column terms are below 2^30 when every source coordinate is
+    // below 2^20 pixels, so their int sum cannot wrap.
+    const double lim = (double)(1 << 20);
+    for (int i = 0; i < nsec; ++i) {
+        for (int k = 0; k < 6; ++k)
+            if (!std::isfinite(xf_host[6 * i + k]))
+                return fail(ctx, TREC_ERR_INVALID, "section %d: non-finite .xf value", i);
+        xc[i] = xf_inverse(xf_host + 6 * i, nx, ny);
+        const double *m = xc[i].m;
+        const double X = nx - 1, Y = ny - 1;
+        const double t[6] = {std::fabs(m[2]), std::fabs(m[1] * Y + m[2]), std::fabs(m[0] * X),
+                             std::fabs(m[5]), std::fabs(m[4] * Y + m[5]), std::fabs(m[3] * X)};
+        for (double v : t)
+            if (!(v < lim))
+                return fail(ctx, TREC_ERR_INVALID, "section %d: the .xf maps pixels 2^20 or more pixels away", i);
+    }
+    const int lanczos = mode == 2;
+    const int K = lanczos ? 8 : 4;
+    // LDS window of a tile: its source footprint spans |m0| (T-1) + |m1| (T-1)
+    // columns (and the same with m3, m4 rows) plus the 1/32-pixel rounding
+    // and the kernel's K taps; the kernel checks each tile against it.
+    double win = 0;
+    for (int i = 0; i < nsec; ++i) {
+        const double *m = xc[i].m;
+        const double span = XF_T - 1;
+        const double rw = std::floor((std::fabs(m[0]) + std::fabs(m[1])) * span + 2) + K;
+        const double rh = std::floor((std::fabs(m[3]) + std::fabs(m[4])) * span + 2) + K;
+        win = std::max(win, rw * rh);
+    }
+    const int win_cap = (int)std::min<double>(win, XF_LDS_MAX);
+    float tab[XF_TAB * 8];
+    for (int i = 0; i < XF_TAB; ++i) {
+        const float x = i * (1.f / XF_TAB);
+        if (lanczos) xf_lanczos4(x, tab + 8 * i);
+        else xf_cubic(x, tab + 4 * i);
+    }
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>((2048 + nsec - 1) / nsec, (n + 4095) / 4096));
+
+    const size_t b_xc = align_up(sizeof(XfConst) * nsec), b_tab = align_up(sizeof(float) * XF_TAB * K);
+    const size_t b_rows = align_up(sizeof(int2) * ny * nsec), b_cols = align_up(sizeof(int2) * nx * nsec);
+    const size_t b_part = align_up(sizeof(double) * nb * nsec), b_cval = align_up(sizeof(float) * nsec);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    trec_status s = ensure_arena(ctx, b_xc + b_tab + b_rows + b_cols + b_part + b_cval);
+    if (s != TREC_OK) return s;
+    XfLaunch a{};
+    a.W = nx;
+    a.H = ny;
+    a.P = nsec;
+    a.nb = nb;
+    a.lanczos = lanczos;
+    a.win_cap = win_cap;
+    char *base = ctx->arena;
+    a.xc = (const XfConst *)base;
+    a.tab = (const float *)(base + b_xc);
+    a.rows = (int2 *)(base + b_xc + b_tab);
+    a.cols = (int2 *)(base + b_xc + b_tab + b_rows);
+    a.part = (double *)(base + b_xc + b_tab + b_rows + b_cols);
+    a.cval = (float *)(base + b_xc + b_tab + b_rows + b_cols + b_part);
+    a.src = in;
+    a.dst = out;
+
+    const size_t up = b_xc + sizeof(float) * XF_TAB * K;
+    HIPCHK(ctx, hipEventSynchronize(ctx->staging_ev));
+    if (ctx->staging_size < up) {
+        if (ctx->staging) (void)hipHostFree(ctx->staging);
+        ctx->staging = nullptr;
+        ctx->staging_size = 0;
+        HIPCHK(ctx, hipHostMalloc((void **)&ctx->staging, up, hipHostMallocDefault));
+        ctx->staging_size = up;
+    }
+    memcpy(ctx->staging, xc.data(), sizeof(XfConst) * nsec);
+    memcpy(ctx->staging + b_xc, tab, sizeof(float) * XF_TAB * K);
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(ctx, hipMemcpyAsync(base, ctx->staging, up, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipEventRecord(ctx->staging_ev, st));
+    tl_prof = &ctx->prof;
+    hipError_t e;
+    {
+        ProfScope ps(PROF_XF_MEAN, st);
+        e = launch_xf_terms(a, st);
+        if (e == hipSuccess) e = launch_xf_mean(a, st);
+    }
+    if (e == hipSuccess) {
+        ProfScope ps(PROF_XF_WARP, st);
+        e = launch_xf_warp(a, st);
+    }
+    tl_prof = nullptr;
+    HIPCHK(ctx, e);
     return TREC_OK;
 }
 

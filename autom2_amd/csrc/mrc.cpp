@@ -85,7 +85,7 @@ bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err)
     return true;
 }
 
-bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &data, std::string &err)
+bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &data, std::string &err, ByteRule bytes)
 {
     if (!mrc_read_header(path, h, err)) return false;
     File fh;
@@ -108,7 +108,10 @@ bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &d
     data.resize(n);
     switch (h.mode) {
     case MRC_BYTE:
-        for (size_t i = 0; i < n; ++i) data[i] = (float)(signed char)raw[i];
+        if (bytes == BYTE_UNSIGNED_SCALED)
+            for (size_t i = 0; i < n; ++i) data[i] = raw[i] / 255.0f;
+        else
+            for (size_t i = 0; i < n; ++i) data[i] = (float)(signed char)raw[i];
         break;
     case MRC_SHORT:
         for (size_t i = 0; i < n; ++i) {
@@ -185,6 +188,30 @@ bool mrc_write_float(const std::string &path, MrcHeader h, const float *data, st
     }
     const size_t n = sec * h.nz;
     if (fwrite(&h, sizeof h, 1, fh.f) != 1 || fwrite(data, 4, n, fh.f) != n) {
+        err = "write failed for " + path;
+        return false;
+    }
+    if (fclose(fh.f) != 0) {
+        fh.f = nullptr;
+        err = "close failed for " + path;
+        return false;
+    }
+    fh.f = nullptr;
+    return true;
+}
+
+bool mrc_write_with_header(const std::string &path, const MrcHeader &h, const float *data, std::string &err)
+{
+    File fh;
+    fh.f = fopen(path.c_str(), "wb");
+    if (!fh.f) {
+        err = "cannot create " + path + ": " + strerror(errno);
+        return false;
+    }
+    const size_t n = (size_t)h.nx * h.ny * h.nz;
+    std::vector<char> ext(h.next > 0 ? h.next : 0, 0);
+    if (fwrite(&h, sizeof h, 1, fh.f) != 1 || fwrite(ext.data(), 1, ext.size(), fh.f) != ext.size() ||
+        fwrite(data, 4, n, fh.f) != n) {
         err = "write failed for " + path;
         return false;
     }

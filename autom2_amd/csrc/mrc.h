@@ -48,8 +48,18 @@ enum { MRC_BYTE = 0, MRC_SHORT = 1, MRC_FLOAT = 2, MRC_USHORT = 6, MRC_HALF = 12
 // Reads the header; returns false with a message on failure.
 bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err);
 
+// How mode 0 (bytes) becomes float: TiltRec's reader takes signed char
+// (mrcstack.cpp:255-287); mrctools' MrcStack takes unsigned char / 255
+// (src/mrctools/src/mrcimg/mrc2img.cpp:60-75).
+enum ByteRule { BYTE_SIGNED = 0, BYTE_UNSIGNED_SCALED = 1 };
+
 // Reads all sections as float, [nz][ny][nx].
-bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &data, std::string &err);
+bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &data, std::string &err,
+                    ByteRule bytes = BYTE_SIGNED);
+
+// Writes header h as given, h.next zero bytes of extended header, and
+// h.nx * h.ny * h.nz floats.
+bool mrc_write_with_header(const std::string &path, const MrcHeader &h, const float *data, std::string &err);
 
 // InitializeHeader (mrcstack.cpp:57-86) with the given size, mode 2.
 MrcHeader mrc_new_header(int nx, int ny, int nz);

@@ -157,6 +157,41 @@ hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, 
 hipError_t launch_general_back(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img,
                                hipStream_t st);
 
+// .xf application (xform.hip).  OpenCV warpAffine's fixed point: 10-bit row
+// and column terms, 1/32-pixel steps, a bias of half a step.
+constexpr int XF_TAB = 32;
+constexpr int XF_AB_SCALE = 1024;
+constexpr int XF_ROUND_DELTA = XF_AB_SCALE / XF_TAB / 2;
+// Output tile of a warp work-group (one wave), and the largest LDS window
+// (floats) a launch reserves: a 32 x 32 tile turned by any angle at unit
+// scale needs at most 50 x 50 cells for the cubic kernel and 54 x 54 for
+// Lanczos-4; larger windows read their taps from global memory.
+constexpr int XF_T = 32;
+constexpr int XF_LDS_MAX = 8192;
+
+// Inverse affine map of one section, warpAffine's m[0..5].
+struct XfConst {
+    double m[6];
+};
+
+struct XfLaunch {
+    int W, H, P;
+    int nb;       // partial sums per section
+    int lanczos;  // 0: 4 x 4 cubic, 1: 8 x 8 Lanczos-4
+    int win_cap;  // LDS window of a warp work-group, floats
+    const XfConst *xc;  // [P]
+    const float *tab;   // [32][K] 1-D kernel table
+    const float *src;   // [P][H][W]
+    float *dst;         // [P][H][W]
+    int2 *rows;         // [P][H] row terms (x, y)
+    int2 *cols;         // [P][W] column terms (x, y)
+    double *part;       // [P][nb]
+    float *cval;        // [P] border values
+};
+hipError_t launch_xf_terms(const XfLaunch &a, hipStream_t st);
+hipError_t launch_xf_mean(const XfLaunch &a, hipStream_t st);
+hipError_t launch_xf_warp(const XfLaunch &a, hipStream_t st);
+
 // Small helpers.
 hipError_t launch_set_scale(JobDesc *d, const double *sumsq, double n, hipStream_t st);
 hipError_t launch_unblock_proj(const float *src, int P, int X, int Y, int nyb, float *dst, hipStream_t st);

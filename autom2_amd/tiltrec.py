@@ -89,7 +89,10 @@ SIGNATURES = [
     ("trec_server_wait", _I, [_P, ctypes.c_uint64, ctypes.POINTER(Result)]),
     ("trec_request_footprint", _I, [ctypes.POINTER(Geom), ctypes.POINTER(Method), ctypes.POINTER(ctypes.c_size_t)]),
     ("trec_server_destroy", None, [_P]),
+    ("trec_apply_xf", _I, [_P, _I, _I, _I, _PF, _I, _P, _P, _P]),
 ]
+
+XF_CUBIC, XF_LANCZOS4 = 1, 2  # mrcstack -m: 2 selects Lanczos-4, anything else cubic
 
 _lib = None
 
@@ -291,6 +294,16 @@ class Context:
 
     def trim(self):
         _check(lib().trec_context_trim(self._h), self._h)
+
+    def apply_xf(self, stack, xf, out, mode=XF_CUBIC, stream=None):
+        """mrcstack's .xf application (trec_apply_xf): stack and out are
+        [nsec][ny][nx] float32 device tensors, xf a host sequence of nsec
+        rows a11 a12 a21 a22 dx dy."""
+        import numpy as np
+        nsec, ny, nx = stack.shape
+        x = np.ascontiguousarray(np.asarray(xf, np.float32).reshape(nsec, 6))
+        _check(lib().trec_apply_xf(self._h, nx, ny, nsec, x.ctypes.data_as(_PF), int(mode), _ptr(stack),
+                                   _ptr(out), _stream(stream)), self._h)
 
 
 class Server:

@@ -163,6 +163,55 @@ def serving_form(trec, torch, n, rate, seed=1004):
     return out
 
 
+def xf_stage(trec, ctx, torch, stack_d, xf, reps=5, cpu=True):
+    """The .xf application of the raw stack (mrcstack, cubic): per-call and
+    per-kernel times (HIP events on the launch stream), the warp kernel's
+    HBM roofline (4 bytes read + 4 written per pixel), and the CPU oracle on
+    a few sections."""
+    P, H, W = stack_d.shape
+    out = torch.empty_like(stack_d)
+    stream = torch.cuda.current_stream()
+    for _ in range(2):
+        ctx.apply_xf(stack_d, xf, out, stream=stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(stream)
+    for k in range(reps):
+        ctx.apply_xf(stack_d, xf, out, stream=stream)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    call_ms = sorted(ev[k].elapsed_time(ev[k + 1]) for k in range(reps))[reps // 2]
+    ctx.profile(True)
+    for _ in range(reps):
+        ctx.apply_xf(stack_d, xf, out, stream=stream)
+    torch.cuda.synchronize()
+    warp_ms, nw = ctx.profile_read("xf_warp")
+    mean_ms, nm = ctx.profile_read("xf_mean")
+    ctx.profile(False)
+    warp_ms /= max(nw, 1)
+    mean_ms /= max(nm, 1)
+    px = float(P) * H * W
+    achieved = 8 * px / (warp_ms / 1e3) / 1e9
+    res = {"workload": f"cubic .xf application of {P} sections of {W}x{H} float32 (the bench stack "
+                       f"and its true alignment)",
+           "value": px / (call_ms / 1e3), "unit": "pixels/s", "ms_per_call": round(call_ms, 4),
+           "kernels_ms": {"xf_warp": round(warp_ms, 4), "xf_mean": round(mean_ms, 4)},
+           "roofline": {"bound": "hbm", "kernel": "xf_warp", "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "bytes_per_launch": 8 * px, "traffic": None}}
+    if cpu:
+        from oracle import orc
+        n = 4
+        host = stack_d[:n].cpu().numpy()
+        t0 = time.perf_counter()
+        orc.apply_xf(host, xf[:n], mode=1)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n * H * W / dt, "unit": "pixels/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} of {P} sections ({dt:.2f} s)"}
+    del out
+    return res
+
+
 def counters_run(iters):
     """The run the counter profiles measure: `iters` SIRT iterations of the
     bench workload through the library, then a 1 GiB device copy whose
@@ -220,6 +269,9 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16/BF16 and serving forms")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--counters", action="store_true", help="only the run the counter profiles measure")
+    ap.add_argument("--no-xf", action="store_true", help="skip the .xf application stage")
+    ap.add_argument("--xf-only", action="store_true",
+                    help="only the .xf application stage (its own JSON line), as its profiles measure it")
     args = ap.parse_args()
     if args.counters:
         counters_run(2)
@@ -241,6 +293,9 @@ def main():
     stack_d = torch.from_numpy(stack).cuda()
     geom = trec.make_geom(X, Y, P, T)
     ctx = trec.Context(0)
+    if args.xf_only:
+        print(json.dumps({"xf_apply": xf_stage(trec, ctx, torch, stack_d, _xf, cpu=not args.no_cpu)}), flush=True)
+        return
     t0 = time.perf_counter()
     ctx.autotune(geom, trec.make_method(trec.SIRT, args.iters, 1.0))
     log(f"[bench] autotune {time.perf_counter() - t0:.2f} s")
@@ -295,6 +350,12 @@ def main():
         torch.cuda.empty_cache()
         forms["serving"] = serving_form(trec, torch, args.serve_requests, 2.0)
 
+    xf_res = None
+    if not args.no_xf:
+        xf_res = xf_stage(trec, ctx, torch, stack_d, _xf, cpu=not args.no_cpu)
+        log(f"[bench] .xf application: {xf_res['ms_per_call']:.3f} ms/call, warp "
+            f"{xf_res['kernels_ms']['xf_warp']:.3f} ms ({xf_res['roofline']['frac']:.3f} of HBM peak)")
+
     cpu = None
     if not args.no_cpu:
         rows = args.cpu_rows
@@ -327,6 +388,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "forms": forms,
+        "xf_apply": xf_res,
     }
     print(json.dumps(out), flush=True)
 

@@ -25,6 +25,8 @@
  *   one iteration of CuSIRT's loop (recon_y.cu:347-382) trec_sirt_iterate
  *   (none: AuTom2 launches one process per series)      trec_reconstruct_batch,
  *                                                        trec_server_*
+ *   mrcstack's per-section cv::warpAffine loop          trec_apply_xf
+ *     (src/mrctools/src/mrcstack/main.cpp:40-80)
  *
  * Conventions
  *   - Every array argument is a device pointer (hipMalloc'd memory on the
@@ -174,6 +176,23 @@ trec_status trec_sirt_iterate(trec_context *ctx, const trec_geom *geom,
                               const float *angles_deg, const float *stack,
                               float gamma, float *volume, void *stream);
 
+/* .xf application: what mrcstack does before TiltRec runs
+ * (src/mrctools/src/mrcstack/main.cpp:40-80).  Section i of `in` ([nsec][ny][nx]
+ * float) is warped by line i of an IMOD-style .xf table, xf_host[6 i ..
+ * 6 i + 5] = a11 a12 a21 a22 dx dy (host memory), about the centre
+ * (nx / 2, ny / 2):
+ *   out_i(x, y) = in_i(M^-1 (x, y)),  M = [[a11, a12, cx + dx - a11 cx - a12 cy],
+ *                                          [a21, a22, cy + dy - a21 cx - a22 cy]]
+ * with cv::warpAffine's arithmetic: 1/32-pixel fixed-point source
+ * coordinates, INTER_LANCZOS4 (8 x 8) when mode == 2 and INTER_CUBIC
+ * (4 x 4) otherwise, and the mean of in_i as the constant outside the
+ * section.  in and out must not overlap.  Returns TREC_ERR_INVALID for
+ * sections above 2^29 pixels, a non-finite .xf value or one that maps
+ * pixels 2^20 or more pixels away. */
+trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec,
+                          const float *xf_host, int mode, const float *in,
+                          float *out, void *stream);
+
 /* One job of a batch: a tilt series and where its volume goes. */
 typedef struct {
     trec_geom geom;
@@ -200,7 +219,9 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom,
 /* Per-kernel timing: while enabled, every launch group of the
  * reconstruction calls is bracketed by HIP events on the stream it runs on
  * (HIP graphs are not used meanwhile).  kernel is one of "prep" (layout
- * transposes, filter, weights), "forward", "combine", "back", "finish";
+ * transposes, filter, weights), "forward", "combine", "back", "finish",
+ * "xf_mean" (trec_apply_xf's coordinate terms and section means) and
+ * "xf_warp" (its warp);
  * trec_profile_read waits for the recorded events and returns the summed
  * milliseconds and the number of launch groups since enabling. */
 trec_status trec_profile_enable(trec_context *ctx, int on);

@@ -48,6 +48,13 @@ def lib():
         for n in ("orc_sirt_f64", "orc_sart_f64"):
             getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, ctypes.c_double, _D]
         L.orc_rweight_f64.argtypes = [_D, _I, _I]
+        L.orc_xf_table.argtypes = [_I, _F]
+        L.orc_xf_inverse.argtypes = [_F, _I, _I, _D]
+        L.orc_image_mean.argtypes = [_F, ctypes.c_long]
+        L.orc_image_mean.restype = ctypes.c_double
+        L.orc_xf_coord.argtypes = [_D, _I, _I, ctypes.POINTER(ctypes.c_int)]
+        L.orc_warp_affine.argtypes = [_F, _I, _I, _D, _I, ctypes.c_float, _F]
+        L.orc_apply_xf.argtypes = [_F, _I, _I, _I, _F, _I, _F, _F]
         _lib = L
     return _lib
 
@@ -188,3 +195,56 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64)
     nb = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))
+
+
+# ------------------------------------------------------------ .xf application
+# mrcstack (src/mrctools/src/mrcstack/main.cpp:40-80) with OpenCV's
+# warpAffine restated in xform_oracle.c.
+
+def xf_table(lanczos):
+    """1-D kernel table at fractions i/32: [32][4] cubic or [32][8] Lanczos-4."""
+    K = 8 if lanczos else 4
+    t = np.zeros((32, K), np.float32)
+    lib().orc_xf_table(int(bool(lanczos)), _fp(t))
+    return t
+
+
+def xf_inverse(xf_row, W, H):
+    """warpAffine's inverse of mrcstack's matrix for one .xf line."""
+    x = np.ascontiguousarray(xf_row, np.float32)
+    M = np.zeros(6, np.float64)
+    lib().orc_xf_inverse(_fp(x), W, H, _dp(M))
+    return M
+
+
+def image_mean(img):
+    a = np.ascontiguousarray(img, np.float32)
+    return lib().orc_image_mean(_fp(a), a.size)
+
+
+def xf_coord(M, x, y):
+    """(ix, iy, table index fy*32 + fx) of output pixel (x, y)."""
+    out = (ctypes.c_int * 3)()
+    Md = np.ascontiguousarray(M, np.float64)
+    lib().orc_xf_coord(_dp(Md), int(x), int(y), out)
+    return out[0], out[1], out[2]
+
+
+def warp_affine(src, M, lanczos, cval):
+    s = np.ascontiguousarray(src, np.float32)
+    H, W = s.shape
+    Md = np.ascontiguousarray(M, np.float64)
+    d = np.empty_like(s)
+    lib().orc_warp_affine(_fp(s), W, H, _dp(Md), int(bool(lanczos)), float(cval), _fp(d))
+    return d
+
+
+def apply_xf(stack, xf, mode=1):
+    """stack [P][H][W] float32, xf [P][6]; returns (out, border values)."""
+    s = np.ascontiguousarray(stack, np.float32)
+    P, H, W = s.shape
+    x = np.ascontiguousarray(xf, np.float32).reshape(P, 6)
+    out = np.empty_like(s)
+    cv = np.empty(P, np.float32)
+    lib().orc_apply_xf(_fp(s), W, H, P, _fp(x), int(mode), _fp(out), _fp(cv))
+    return out, cv
