@@ -1022,6 +1022,8 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
         return fail(ctx, TREC_ERR_INVALID, "bad stack size %d x %d x %d", nx, ny, nsec);
     if ((size_t)nx * ny > ((size_t)1 << 29))
         return fail(ctx, TREC_ERR_INVALID, "a section of %d x %d pixels exceeds 2^29", nx, ny);
+    if ((size_t)((nx + XF_TW - 1) / XF_TW) * ((ny + XF_TH - 1) / XF_TH) * nsec >= ((size_t)1 << 26))
+        return fail(ctx, TREC_ERR_INVALID, "stack of %d sections of %d x %d is too large", nsec, nx, ny);
     if (!xf_host || !in || !out) return fail(ctx, TREC_ERR_INVALID, "null argument");
     const size_t n = (size_t)nx * ny;
     if (out < in + n * nsec && in < out + n * nsec)
@@ -1045,27 +1047,32 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
     }
     const int lanczos = mode == 2;
     const int K = lanczos ? 8 : 4;
-    // LDS window of a tile: its source footprint spans |m0| (T-1) + |m1| (T-1)
-    // columns (and the same with m3, m4 rows) plus the 1/32-pixel rounding
-    // and the kernel's K taps; the kernel checks each tile against it.
-    double win = 0;
+    // LDS window of a tile: its source footprint spans |m0| (TW-1) + |m1| (TH-1)
+    // columns and |m3| (TW-1) + |m4| (TH-1) rows, plus the 1/32-pixel rounding
+    // and the kernel's K taps.  Inner windows are loaded with a row pitch of
+    // 72 or 80 floats in whole 256-float steps; the kernel checks each tile
+    // against the reserve and takes a slower path where it does not fit.
+    double rw_max = 0, rh_max = 0, rwh_max = 0;
     for (int i = 0; i < nsec; ++i) {
         const double *m = xc[i].m;
-        const double span = XF_T - 1;
-        const double rw = std::floor((std::fabs(m[0]) + std::fabs(m[1])) * span + 2) + K;
-        const double rh = std::floor((std::fabs(m[3]) + std::fabs(m[4])) * span + 2) + K;
-        win = std::max(win, rw * rh);
+        const double rw = std::floor(std::fabs(m[0]) * (XF_TW - 1) + std::fabs(m[1]) * (XF_TH - 1) + 2) + K;
+        const double rh = std::floor(std::fabs(m[3]) * (XF_TW - 1) + std::fabs(m[4]) * (XF_TH - 1) + 2) + K;
+        rw_max = std::max(rw_max, rw);
+        rh_max = std::max(rh_max, rh);
+        rwh_max = std::max(rwh_max, rw * rh);
     }
+    const int pitch = rw_max <= 72 ? 72 : 80;
+    const double win = std::max(std::ceil(pitch * rh_max / 256) * 256, rwh_max);
     const int win_cap = (int)std::min<double>(win, XF_LDS_MAX);
-    float tab[XF_TAB * 8];
+    float tab[XF_TAB * XF_TP] = {};
     for (int i = 0; i < XF_TAB; ++i) {
         const float x = i * (1.f / XF_TAB);
-        if (lanczos) xf_lanczos4(x, tab + 8 * i);
-        else xf_cubic(x, tab + 4 * i);
+        if (lanczos) xf_lanczos4(x, tab + XF_TP * i);
+        else xf_cubic(x, tab + XF_TP * i);
     }
     const int nb = (int)std::max<size_t>(1, std::min<size_t>((2048 + nsec - 1) / nsec, (n + 4095) / 4096));
 
-    const size_t b_xc = align_up(sizeof(XfConst) * nsec), b_tab = align_up(sizeof(float) * XF_TAB * K);
+    const size_t b_xc = align_up(sizeof(XfConst) * nsec), b_tab = align_up(sizeof(float) * XF_TAB * XF_TP);
     const size_t b_rows = align_up(sizeof(int2) * ny * nsec), b_cols = align_up(sizeof(int2) * nx * nsec);
     const size_t b_part = align_up(sizeof(double) * nb * nsec), b_cval = align_up(sizeof(float) * nsec);
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1078,6 +1085,7 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
     a.nb = nb;
     a.lanczos = lanczos;
     a.win_cap = win_cap;
+    a.pitch = pitch;
     char *base = ctx->arena;
     a.xc = (const XfConst *)base;
     a.tab = (const float *)(base + b_xc);
@@ -1088,7 +1096,7 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
     a.src = in;
     a.dst = out;
 
-    const size_t up = b_xc + sizeof(float) * XF_TAB * K;
+    const size_t up = b_xc + sizeof(float) * XF_TAB * XF_TP;
     HIPCHK(ctx, hipEventSynchronize(ctx->staging_ev));
     if (ctx->staging_size < up) {
         if (ctx->staging) (void)hipHostFree(ctx->staging);
@@ -1098,7 +1106,7 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
         ctx->staging_size = up;
     }
     memcpy(ctx->staging, xc.data(), sizeof(XfConst) * nsec);
-    memcpy(ctx->staging + b_xc, tab, sizeof(float) * XF_TAB * K);
+    memcpy(ctx->staging + b_xc, tab, sizeof(float) * XF_TAB * XF_TP);
     hipStream_t st = (hipStream_t)stream;
     HIPCHK(ctx, hipMemcpyAsync(base, ctx->staging, up, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipEventRecord(ctx->staging_ev, st));

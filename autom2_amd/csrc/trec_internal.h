@@ -162,12 +162,15 @@ hipError_t launch_general_back(const GeneralJob &g, int mode, int theta0, int nt
 constexpr int XF_TAB = 32;
 constexpr int XF_AB_SCALE = 1024;
 constexpr int XF_ROUND_DELTA = XF_AB_SCALE / XF_TAB / 2;
-// Output tile of a warp work-group (one wave), and the largest LDS window
-// (floats) a launch reserves: a 32 x 32 tile turned by any angle at unit
-// scale needs at most 50 x 50 cells for the cubic kernel and 54 x 54 for
-// Lanczos-4; larger windows read their taps from global memory.
-constexpr int XF_T = 32;
-constexpr int XF_LDS_MAX = 8192;
+// Output tile of a warp work-group (one wave: lane = column), the table
+// entry pitch (floats; 32-byte entries) and the largest LDS window (floats)
+// a launch reserves.  Windows of inner tiles have a fixed row pitch of 72
+// or 80 floats (the launch's `pitch`): a 64 x 16 tile at unit scale spans
+// 68 columns unturned, 62 at 45 degrees.
+constexpr int XF_TW = 64;
+constexpr int XF_TH = 16;
+constexpr int XF_TP = 8;
+constexpr int XF_LDS_MAX = 15360;
 
 // Inverse affine map of one section, warpAffine's m[0..5].
 struct XfConst {
@@ -179,8 +182,9 @@ struct XfLaunch {
     int nb;       // partial sums per section
     int lanczos;  // 0: 4 x 4 cubic, 1: 8 x 8 Lanczos-4
     int win_cap;  // LDS window of a warp work-group, floats
+    int pitch;    // LDS row pitch of inner windows: 72 or 80
     const XfConst *xc;  // [P]
-    const float *tab;   // [32][K] 1-D kernel table
+    const float *tab;   // [32][XF_TP] 1-D kernel table
     const float *src;   // [P][H][W]
     float *dst;         // [P][H][W]
     int2 *rows;         // [P][H] row terms (x, y)

@@ -22,13 +22,14 @@
 //                 int2, as warpAffine's own per-row / per-column tables;
 //   k_xf_sum      per-section partial sums in double (fixed order), and
 //   k_xf_mean     the border value (float)(sum * (1 / n));
-//   k_xf_warp     32 x 32 output pixels per one-wave work-group.  The source window
-//                 of a tile is exact from its corners (both terms are
-//                 monotone in x and y), is staged once in LDS with
-//                 coalesced loads, and every tap is then an LDS read.  A
-//                 window beyond the LDS budget (magnification above about 2
-//                 along the axes, 1.2 at 45 degrees) reads its taps from
-//                 global memory instead.
+//   k_xf_warp     64 x 16 output pixels per one-wave work-group.  The source
+//                 window of a tile is exact from its corners (both terms
+//                 are monotone in x and y).  Inside the section it is loaded
+//                 by the buffer unit straight into LDS with a fixed row
+//                 pitch, so every tap of a pixel is an immediate offset from
+//                 one LDS address; at the section's edges it is staged
+//                 through registers; a window beyond the LDS budget reads
+//                 its taps from global memory.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -149,44 +150,60 @@ __global__ __launch_bounds__(64) void k_xf_mean(const double *__restrict__ part,
 
 namespace {
 
-// One output pixel.  Taps come from the staged window (LDS) or from global
-// memory; `sbase` is the window offset of the pixel's first tap (LDS) or
-// its section offset (global), `stride` the row pitch of that source.
+// Weights of one pixel: the float products of the row and column kernels.
 template <int K>
-__device__ __forceinline__ float warp_pixel(const float *__restrict__ srcp, int sbase, int stride, int sx, int sy,
-                                            int W, int H, const float *ty, const float *tx, float cval)
+__device__ __forceinline__ void tap_weights(const float *ty, const float *tx, float (&w)[K][K])
 {
 #pragma clang fp contract(off)
-    float w[K][K];
 #pragma unroll
     for (int a = 0; a < K; ++a)
 #pragma unroll
         for (int b = 0; b < K; ++b) w[a][b] = ty[a] * tx[b];
-    constexpr int width_pad = K - 1;
-    const bool inside = (unsigned)sx < (unsigned)max(W - width_pad, 0) && (unsigned)sy < (unsigned)max(H - width_pad, 0);
-    if (inside) {
-        float row[K];
-        float sum;
-        if constexpr (K == 4) {
+}
+
+// A pixel whose K x K taps all lie inside the section: the row-by-row float
+// sums of tap * weight.  `base` indexes the first tap in `srcp`, `stride`
+// is the row pitch.
+template <int K>
+__device__ __forceinline__ float interior_sum(const float *__restrict__ srcp, int base, int stride,
+                                              const float (&w)[K][K])
+{
+#pragma clang fp contract(off)
+    float row[K];
+    float sum;
+    if constexpr (K == 4) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
+        for (int a = 0; a < 4; ++a) {
 #pragma unroll
-                for (int b = 0; b < 4; ++b) row[b] = srcp[sbase + a * stride + b];
-                const float r = row[0] * w[a][0] + row[1] * w[a][1] + row[2] * w[a][2] + row[3] * w[a][3];
-                sum = a == 0 ? r : sum + r;
-            }
-        } else {
-            sum = 0.f;
-#pragma unroll
-            for (int a = 0; a < 8; ++a) {
-#pragma unroll
-                for (int b = 0; b < 8; ++b) row[b] = srcp[sbase + a * stride + b];
-                sum += row[0] * w[a][0] + row[1] * w[a][1] + row[2] * w[a][2] + row[3] * w[a][3] +
-                       row[4] * w[a][4] + row[5] * w[a][5] + row[6] * w[a][6] + row[7] * w[a][7];
-            }
+            for (int b = 0; b < 4; ++b) row[b] = srcp[base + a * stride + b];
+            const float r = row[0] * w[a][0] + row[1] * w[a][1] + row[2] * w[a][2] + row[3] * w[a][3];
+            sum = a == 0 ? r : sum + r;
         }
-        return sum;
+    } else {
+        sum = 0.f;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+#pragma unroll
+            for (int b = 0; b < 8; ++b) row[b] = srcp[base + a * stride + b];
+            sum += row[0] * w[a][0] + row[1] * w[a][1] + row[2] * w[a][2] + row[3] * w[a][3] + row[4] * w[a][4] +
+                   row[5] * w[a][5] + row[6] * w[a][6] + row[7] * w[a][7];
+        }
     }
+    return sum;
+}
+
+// Any pixel: interior, entirely outside (the border value), or touching the
+// border (border + sum of (tap - border) * weight over the taps inside).
+// `base` indexes the pixel's first tap, inside the section or not.
+template <int K>
+__device__ __forceinline__ float warp_pixel(const float *__restrict__ srcp, int base, int stride, int sx, int sy,
+                                            int W, int H, const float *ty, const float *tx, float cval)
+{
+#pragma clang fp contract(off)
+    float w[K][K];
+    tap_weights<K>(ty, tx, w);
+    const bool inside = (unsigned)sx < (unsigned)max(W - (K - 1), 0) && (unsigned)sy < (unsigned)max(H - (K - 1), 0);
+    if (inside) return interior_sum<K>(srcp, base, stride, w);
     if (sx >= W || sx + K <= 0 || sy >= H || sy + K <= 0) return cval;
     float sum = cval;
 #pragma unroll
@@ -195,7 +212,7 @@ __device__ __forceinline__ float warp_pixel(const float *__restrict__ srcp, int 
 #pragma unroll
         for (int b = 0; b < K; ++b) {
             if ((unsigned)(sx + b) >= (unsigned)W) continue;
-            sum += (srcp[sbase + a * stride + b] - cval) * w[a][b];
+            sum += (srcp[base + a * stride + b] - cval) * w[a][b];
         }
     }
     return sum;
@@ -203,125 +220,233 @@ __device__ __forceinline__ float warp_pixel(const float *__restrict__ srcp, int 
 
 }  // namespace
 
-// Grid: x = tile (tiles_x * tiles_y), y = section.  Wave v owns rows
-// y0 + v, y0 + v + 4, ... of the tile; lane l owns column x0 + l.
 constexpr int XF_SB = 12;                  // window cells per lane per staging round
 constexpr int XF_BUF_DWORD3 = 0x00020000;  // raw buffer, gfx9 family
 
-// The pixel loop of one tile: lane l owns column x0 + (l & 31) of rows
-// y0 + 2 j + (l >> 5).  Row terms are uniform per half-wave (two scalar
-// loads per row pair), the column term is loaded once.
-template <int K, bool STAGED>
-__device__ __forceinline__ void warp_rows(const float *__restrict__ taps, int stride, int rx0, int ry0,
-                                          const float *__restrict__ tab, const int2 *rw, int2 cx, int x, int ty0,
-                                          int y1, int W, int H, float cval, float *__restrict__ d)
+// How a tile's window reaches its taps.  XF_BY_DMA: the window lies inside
+// the section (so every pixel is interior) and is loaded straight into LDS
+// with a fixed row pitch.  XF_BY_LOADS: loaded through registers with its
+// own pitch, pixels classified one by one.  XF_BY_GLOBAL: taps read from
+// global memory (windows beyond the LDS budget).
+enum { XF_BY_DMA = 0, XF_BY_LOADS = 1, XF_BY_GLOBAL = 2 };
+
+// One tile: section, pixels, source window and path.  All fields uniform.
+struct XfTile {
+    int i, tx0, ty0, x1, y1;
+    int rx0, ry0, RW, RH;
+    int how;
+};
+
+// S is the LDS row pitch of DMA windows (floats, a multiple of 4).
+template <int K, int S>
+__device__ __forceinline__ XfTile xf_tile(unsigned t_all, unsigned ntiles, int tiles_x, int W, int H,
+                                          const int2 *__restrict__ rows, const int2 *__restrict__ cols, int win_cap)
 {
     constexpr int o = K / 2 - 1;
-    const int rh = (threadIdx.x >> 5) & 1;
-    for (int j = 0; j < XF_T / 2; ++j) {
-        const int ya = ty0 + 2 * j;
-        if (ya > y1) break;
-        const int2 r0 = ld_term(&rw[ya]), r1 = ld_term(&rw[min(ya + 1, y1)]);
-        const int y = ya + rh;
-        if (x >= W || y > y1) continue;
-        const int2 ry = rh ? r1 : r0;
-        // two's-complement sums, as warpAffine's int arithmetic wraps
-        const int X = (int)((unsigned)ry.x + (unsigned)cx.x) >> 5;
-        const int Y = (int)((unsigned)ry.y + (unsigned)cx.y) >> 5;
-        const int ix = sat_short(X >> 5), iy = sat_short(Y >> 5);
-        const float *tyk = &tab[(Y & (XF_TAB - 1)) * K];
-        const float *txk = &tab[(X & (XF_TAB - 1)) * K];
+    XfTile T;
+    T.i = (int)(t_all / ntiles);
+    const int t = (int)(t_all % ntiles);
+    T.tx0 = (t % tiles_x) * XF_TW;
+    T.ty0 = (t / tiles_x) * XF_TH;
+    T.x1 = min(T.tx0 + XF_TW, W) - 1;
+    T.y1 = min(T.ty0 + XF_TH, H) - 1;
+    const int2 *rw = rows + (size_t)T.i * H;
+    const int2 *cl = cols + (size_t)T.i * W;
+    // The row term is monotone in y and the column term in x, so the
+    // extremes of the source coordinate sit at the tile's corners.
+    const int2 ra = ld_term(&rw[T.ty0]), rb = ld_term(&rw[T.y1]), ca = ld_term(&cl[T.tx0]), cb = ld_term(&cl[T.x1]);
+    const int ixlo = sat_short((min(ra.x, rb.x) + min(ca.x, cb.x)) >> 10);
+    const int ixhi = sat_short((max(ra.x, rb.x) + max(ca.x, cb.x)) >> 10);
+    const int iylo = sat_short((min(ra.y, rb.y) + min(ca.y, cb.y)) >> 10);
+    const int iyhi = sat_short((max(ra.y, rb.y) + max(ca.y, cb.y)) >> 10);
+    T.rx0 = ixlo - o;
+    T.ry0 = iylo - o;
+    T.RW = ixhi - ixlo + K;
+    T.RH = iyhi - iylo + K;
+    const bool inner =
+        T.rx0 >= 0 && T.ry0 >= 0 && T.rx0 + T.RW <= W && T.ry0 + T.RH <= H && ixhi < 32767 && iyhi < 32767;
+    if (inner && T.RW <= S && T.RH <= win_cap / S && ((S * T.RH + 255) & ~255) <= win_cap)
+        T.how = XF_BY_DMA;
+    else if (T.RW <= win_cap && T.RH <= win_cap && T.RW * T.RH <= win_cap)
+        T.how = XF_BY_LOADS;
+    else
+        T.how = XF_BY_GLOBAL;
+    return T;
+}
+
+// Inner window into LDS by the buffer unit: rows of S cells (the window
+// and whatever follows it), 16 bytes per lane, 64 quads per instruction;
+// the last instruction's quads past the window are out of range and fill
+// padding.  Completes on the vector-memory count.
+template <int S>
+__device__ __forceinline__ void xf_window_dma(const XfTile &T, const float *s, int W, size_t sec, float *buf)
+{
+    constexpr int q4 = S / 4, step_r = 64 / q4, step_c = 64 % q4;
+    const int lane = threadIdx.x;
+    const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)s, (short)0, (int)(sec * 4), XF_BUF_DWORD3);
+    const int n4 = q4 * T.RH;
+    int r = lane / q4, cq = lane % q4;
+    for (int q0 = 0; q0 < n4; q0 += 64) {
+        const unsigned off = q0 + lane < n4
+                                 ? 4u * ((unsigned)(T.ry0 + r) * (unsigned)W + (unsigned)(T.rx0 + 4 * cq))
+                                 : 0x80000000u;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void *)(buf + 4 * q0), 16,
+                                                 off, 0, 0, 0);
+        r += step_r;
+        cq += step_c;
+        if (cq >= q4) {
+            cq -= q4;
+            ++r;
+        }
+    }
+}
+
+// Any window into LDS through registers, XF_SB cells per lane per round:
+// cells outside the section read as 0 through the buffer's range check.
+// Cell idx = lane + 64 m sits at row r, column c, stepped without division.
+__device__ __forceinline__ void xf_window_loads(const XfTile &T, const float *s, int W, int H, size_t sec,
+                                                float *buf)
+{
+    const int lane = threadIdx.x;
+    const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)s, (short)0, (int)(sec * 4), XF_BUF_DWORD3);
+    const int RW = T.RW, n = T.RW * T.RH;
+    const int q64 = 64 / RW, r64 = 64 % RW;
+    int r = lane / RW, c = lane % RW;
+    for (int base = 0; base < n; base += 64 * XF_SB) {
+        float v[XF_SB];
+#pragma unroll
+        for (int u = 0; u < XF_SB; ++u) {
+            const int gy = T.ry0 + r, gx = T.rx0 + c;
+            const bool in = base + u * 64 + lane < n && (unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W;
+            const unsigned off = in ? 4u * ((unsigned)gy * (unsigned)W + (unsigned)gx) : 0x80000000u;
+            v[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
+            r += q64;
+            c += r64;
+            if (c >= RW) {
+                c -= RW;
+                ++r;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < XF_SB; ++u) {
+            const int idx = base + u * 64 + lane;
+            if (idx < n) buf[idx] = v[u];
+        }
+    }
+}
+
+// The rows of an inner tile: lane l owns column x0 + l, the row term of
+// each row is uniform.  With the fixed pitch S every tap of a pixel is an
+// immediate offset from one LDS address.
+template <int K, int S>
+__device__ __forceinline__ void warp_inner(const XfTile &T, const float *buf, const float *__restrict__ tab,
+                                           const int2 *rw, int2 cx, float *__restrict__ d, int W)
+{
+    constexpr int o = K / 2 - 1;
+    const int x = T.tx0 + (int)threadIdx.x;
+    // window origin folded in: cell (iy, ix) of the first tap sits at
+    // wb[iy * S + ix]
+    const float *wb = buf - ((T.ry0 + o) * S + T.rx0 + o);
+    if (x > T.x1) return;
+#pragma unroll
+    for (int j = 0; j < XF_TH; ++j) {
+        const int y = T.ty0 + j;
+        if (y > T.y1) break;
+        const int2 r = ld_term(&rw[y]);
+        // two's-complement sums, as warpAffine's int arithmetic wraps; bits
+        // 5..9 are the 1/32 fraction, the rest the integer part
+        const int Sx = (int)((unsigned)r.x + (unsigned)cx.x), Sy = (int)((unsigned)r.y + (unsigned)cx.y);
+        const float *txk = &tab[(Sx & 0x3e0) >> 2];
+        const float *tyk = &tab[(Sy & 0x3e0) >> 2];
         float ty[K], txv[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             ty[k] = tyk[k];
             txv[k] = txk[k];
         }
-        const int sx = ix - o, sy = iy - o;
-        float v;
-        if constexpr (STAGED)
-            v = warp_pixel<K>(taps, (sy - ry0) * stride + (sx - rx0), stride, sx, sy, W, H, ty, txv, cval);
-        else
-            // wraps only for pixels whose taps all lie outside, which read nothing
-            v = warp_pixel<K>(taps, (int)((unsigned)sy * (unsigned)W + (unsigned)sx), stride, sx, sy, W, H, ty,
-                              txv, cval);
+        float w[K][K];
+        tap_weights<K>(ty, txv, w);
+        // inner: both coordinates are small and non-negative
+        const float v = interior_sum<K>(wb, __mul24(Sy >> 10, S) + (Sx >> 10), S, w);
         d[(size_t)y * W + x] = v;
     }
 }
 
-// Grid: x = tile (tiles_x * tiles_y of XF_T x XF_T pixels), y = section; one
-// wave per work-group.  win_cap is the dynamic LDS window in floats.
-template <int K>
+// The rows of any other tile, each pixel classified.  BY_LOADS: taps in
+// the window `buf` of pitch RW; BY_GLOBAL: taps in the section `s`.
+template <int K, bool STAGED>
+__device__ __forceinline__ void warp_general(const XfTile &T, const float *taps, const float *__restrict__ tab,
+                                             const int2 *rw, int2 cx, float cval, float *__restrict__ d, int W,
+                                             int H)
+{
+    constexpr int o = K / 2 - 1;
+    const int x = T.tx0 + (int)threadIdx.x;
+    if (x > T.x1) return;
+    for (int y = T.ty0; y <= T.y1; ++y) {
+        const int2 r = ld_term(&rw[y]);
+        const int Sx = (int)((unsigned)r.x + (unsigned)cx.x), Sy = (int)((unsigned)r.y + (unsigned)cx.y);
+        const float *txk = &tab[(Sx & 0x3e0) >> 2];
+        const float *tyk = &tab[(Sy & 0x3e0) >> 2];
+        float ty[K], txv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            ty[k] = tyk[k];
+            txv[k] = txk[k];
+        }
+        const int sx = sat_short(Sx >> 10) - o, sy = sat_short(Sy >> 10) - o;
+        float v;
+        if constexpr (STAGED)
+            v = warp_pixel<K>(taps, (sy - T.ry0) * T.RW + (sx - T.rx0), T.RW, sx, sy, W, H, ty, txv, cval);
+        else
+            // wraps only for pixels whose taps all lie outside, which read nothing
+            v = warp_pixel<K>(taps, (int)((unsigned)sy * (unsigned)W + (unsigned)sx), W, sx, sy, W, H, ty, txv,
+                              cval);
+        d[(size_t)y * W + x] = v;
+    }
+}
+
+// Grid: one wave per work-group and per tile of XF_TW x XF_TH pixels, over
+// all tiles of every section (raster order within a section).  Work-groups
+// reach the 8 XCDs round-robin; each XCD takes one contiguous run of the
+// list, so the windows of neighbouring tiles, which overlap, meet in one
+// L2.  tab1 is the kernel table, [32][XF_TP] floats; win_cap the dynamic LDS
+// window in floats.
+template <int K, int S>
 __global__ __launch_bounds__(64) void k_xf_warp(const float *__restrict__ src, float *__restrict__ dst, int W, int H,
                                                 const int2 *__restrict__ rows, const int2 *__restrict__ cols,
                                                 const float *__restrict__ cvals, const float *__restrict__ tab1,
-                                                int tiles_x, int win_cap)
+                                                int tiles_x, unsigned ntiles, int win_cap)
 {
-    constexpr int o = K / 2 - 1;
-    __shared__ __align__(16) float tab[XF_TAB * K];
+    __shared__ __align__(16) float tab[XF_TAB * XF_TP];
     extern __shared__ float win[];
-    const int i = blockIdx.y;
-    const int tx0 = (blockIdx.x % tiles_x) * XF_T, ty0 = (blockIdx.x / tiles_x) * XF_T;
-    const int x1 = min(tx0 + XF_T, W) - 1, y1 = min(ty0 + XF_T, H) - 1;
+    const unsigned nb = gridDim.x, q = nb >> 3, rem = nb & 7u;
+    const unsigned xcd = blockIdx.x & 7u;
+    const unsigned b = xcd * q + (xcd < rem ? xcd : rem) + (blockIdx.x >> 3);
     const int lane = threadIdx.x;
     const size_t sec = (size_t)W * H;
-    const float *s = src + (size_t)i * sec;
-    const int2 *rw = rows + (size_t)i * H;
-    const int2 *cl = cols + (size_t)i * W;
 
-    for (int k = lane; k < XF_TAB * K; k += 64) tab[k] = tab1[k];
-
-    // Source window of the tile from its corners: the row term is monotone
-    // in y and the column term in x, so the extremes sit at the corners.
-    const int2 ra = ld_term(&rw[ty0]), rb = ld_term(&rw[y1]), ca = ld_term(&cl[tx0]), cb = ld_term(&cl[x1]);
-    const int ixlo = sat_short((min(ra.x, rb.x) + min(ca.x, cb.x)) >> 10);
-    const int ixhi = sat_short((max(ra.x, rb.x) + max(ca.x, cb.x)) >> 10);
-    const int iylo = sat_short((min(ra.y, rb.y) + min(ca.y, cb.y)) >> 10);
-    const int iyhi = sat_short((max(ra.y, rb.y) + max(ca.y, cb.y)) >> 10);
-    const int rx0 = ixlo - o, ry0 = iylo - o;
-    const int RW = ixhi - ixlo + K, RH = iyhi - iylo + K;
-    const bool staged = RW <= win_cap && RH <= win_cap && RW * RH <= win_cap;
-    if (staged) {
-        // The window in linear order, XF_SB cells per lane per round: all
-        // loads of a round are in flight together.  Cell idx = lane + 64 m
-        // sits at row r, column c, stepped without division.  Cells outside
-        // the section read as 0 through the buffer's range check.
-        const int n = RW * RH;
-        const int q64 = 64 / RW, r64 = 64 % RW;
-        int r = lane / RW, c = lane % RW;
-        const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)s, (short)0, (int)(sec * 4), XF_BUF_DWORD3);
-        for (int base = 0; base < n; base += 64 * XF_SB) {
-            float v[XF_SB];
-#pragma unroll
-            for (int u = 0; u < XF_SB; ++u) {
-                const int gy = ry0 + r, gx = rx0 + c;
-                const bool in = base + u * 64 + lane < n && (unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W;
-                const unsigned off = in ? 4u * ((unsigned)gy * (unsigned)W + (unsigned)gx) : 0x80000000u;
-                v[u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, 0));
-                r += q64;
-                c += r64;
-                if (c >= RW) {
-                    c -= RW;
-                    ++r;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < XF_SB; ++u) {
-                const int idx = base + u * 64 + lane;
-                if (idx < n) win[idx] = v[u];
-            }
-        }
-    }
+    for (int k = lane; k < XF_TAB * XF_TP; k += 64) tab[k] = tab1[k];
+    const XfTile T = xf_tile<K, S>(b, ntiles, tiles_x, W, H, rows, cols, win_cap);
+    const float *s = src + (size_t)T.i * sec;
+    if (T.how == XF_BY_DMA)
+        xf_window_dma<S>(T, s, W, sec, win);
+    else if (T.how == XF_BY_LOADS)
+        xf_window_loads(T, s, W, H, sec, win);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
-    const float cval = *kc(&cvals[i]);
-    const int x = tx0 + (lane & 31);
-    const int2 cx = cl[min(x, x1)];
-    float *d = dst + (size_t)i * sec;
-    if (staged)
-        warp_rows<K, true>(win, RW, rx0, ry0, tab, rw, cx, x, ty0, y1, W, H, cval, d);
-    else
-        warp_rows<K, false>(s, W, 0, 0, tab, rw, cx, x, ty0, y1, W, H, cval, d);
+    const int2 *rw = rows + (size_t)T.i * H;
+    const int2 cx = cols[(size_t)T.i * W + min(T.tx0 + lane, T.x1)];
+    float *d = dst + (size_t)T.i * sec;
+    if (T.how == XF_BY_DMA) {
+        warp_inner<K, S>(T, win, tab, rw, cx, d, W);
+    } else {
+        const float cval = *kc(&cvals[T.i]);
+        if (T.how == XF_BY_LOADS)
+            warp_general<K, true>(T, win, tab, rw, cx, cval, d, W, H);
+        else
+            warp_general<K, false>(T, s, tab, rw, cx, cval, d, W, H);
+    }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -347,15 +472,21 @@ hipError_t launch_xf_mean(const XfLaunch &a, hipStream_t st)
 
 hipError_t launch_xf_warp(const XfLaunch &a, hipStream_t st)
 {
-    const int tiles_x = (a.W + XF_T - 1) / XF_T, tiles_y = (a.H + XF_T - 1) / XF_T;
-    const dim3 grid((unsigned)(tiles_x * tiles_y), (unsigned)a.P);
+    const int tiles_x = (a.W + XF_TW - 1) / XF_TW, tiles_y = (a.H + XF_TH - 1) / XF_TH;
+    const unsigned ntiles = (unsigned)tiles_x * (unsigned)tiles_y;
+    const dim3 grid(ntiles * (unsigned)a.P);
     const size_t lds = sizeof(float) * (size_t)a.win_cap;
-    if (a.lanczos)
-        hipLaunchKernelGGL(k_xf_warp<8>, grid, dim3(64), lds, st, a.src, a.dst, a.W, a.H, a.rows, a.cols, a.cval,
-                           a.tab, tiles_x, a.win_cap);
-    else
-        hipLaunchKernelGGL(k_xf_warp<4>, grid, dim3(64), lds, st, a.src, a.dst, a.W, a.H, a.rows, a.cols, a.cval,
-                           a.tab, tiles_x, a.win_cap);
+#define XF_LAUNCH(K, S)                                                                                        \
+    hipLaunchKernelGGL((k_xf_warp<K, S>), grid, dim3(64), lds, st, a.src, a.dst, a.W, a.H, a.rows, a.cols, a.cval, \
+                       a.tab, tiles_x, ntiles, a.win_cap)
+    if (a.lanczos) {
+        if (a.pitch <= 72) XF_LAUNCH(8, 72);
+        else XF_LAUNCH(8, 80);
+    } else {
+        if (a.pitch <= 72) XF_LAUNCH(4, 72);
+        else XF_LAUNCH(4, 80);
+    }
+#undef XF_LAUNCH
     return hipGetLastError();
 }
 
