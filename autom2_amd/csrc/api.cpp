@@ -69,8 +69,10 @@ struct Prof {
             pool.pop_back();
             return e;
         }
+        // timing only: no system-scope fence, so recording does not write
+        // back and invalidate the caches around the bracketed launches
         hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
         return e;
     }
 };
@@ -1070,7 +1072,9 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
         if (lanczos) xf_lanczos4(x, tab + XF_TP * i);
         else xf_cubic(x, tab + XF_TP * i);
     }
-    const int nb = (int)std::max<size_t>(1, std::min<size_t>((2048 + nsec - 1) / nsec, (n + 4095) / 4096));
+    // partial sums per section: at most 2048 work-groups in all (8 per CU),
+    // so the sum pass runs in a single round
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>(2048 / nsec, (n + 4095) / 4096));
 
     const size_t b_xc = align_up(sizeof(XfConst) * nsec), b_tab = align_up(sizeof(float) * XF_TAB * XF_TP);
     const size_t b_rows = align_up(sizeof(int2) * ny * nsec), b_cols = align_up(sizeof(int2) * nx * nsec);

@@ -198,7 +198,9 @@ def xf_stage(trec, ctx, torch, stack_d, xf, reps=5, cpu=True):
            "kernels_ms": {"xf_warp": round(warp_ms, 4), "xf_mean": round(mean_ms, 4)},
            "roofline": {"bound": "hbm", "kernel": "xf_warp", "achieved": round(achieved, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                        "bytes_per_launch": 8 * px, "traffic": None}}
+                        "bytes_per_launch": 8 * px,
+                        "traffic": profiled_traffic("xf_warp<4", "xf1024_cubic_counters.md")
+                        if (P, H, W) == (61, 1024, 1024) else None}}
     if cpu:
         from oracle import orc
         n = 4
@@ -237,11 +239,11 @@ def counters_run(iters):
     log("[bench] counters run done")
 
 
-def profiled_traffic(kernel):
+def profiled_traffic(kernel, summary="sirt1024_f32_counters.md"):
     """HBM bytes per dispatch of the kernel whose name starts with `kernel`,
-    from the committed counter summary (profiles/sirt1024_f32_counters.md):
-    its calibrated read GB plus write GB, or None."""
-    path = os.path.join(ROOT, "profiles", "sirt1024_f32_counters.md")
+    from a committed counter summary under profiles/: its calibrated read
+    GB plus write GB, or None."""
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         cols = None
         for line in open(path):
