@@ -215,6 +215,33 @@ std::vector<float> rweight_kernel(int X, int &N)
     return hk;
 }
 
+// The FBP spatial filter's taps (SheppLogan / RamLak, filter_prj.cpp:137-178,
+// in the same float / double operations) and N = cvGetOptimalDFTSize(X + 2 +
+// taps - 1), the smallest 2^a 3^b 5^c not below it (filter_prj.cpp:277-278).
+std::vector<float> spatial_taps(int mode, int X, int &N)
+{
+    const int L = FBP_TAPS;
+    std::vector<float> h(L);
+    if (mode == 0) {
+        float n = -((float)L - 1.0) / 2.0;
+        for (int i = 0; i < L; ++i) {
+            h[i] = (float)(-2.0 / (M_PI * (4.0 * n * n - 1.0)));
+            n = n + 1.0;
+        }
+    } else {
+        int n = -(L - 1) / 2;
+        for (int i = 0; i < L; ++i, ++n)
+            h[i] = n == 0 ? (float)(M_PI / 4) : (std::abs(n % 2) == 1 ? (float)(-1.0 / (M_PI * n * n)) : 0.f);
+    }
+    for (N = X + 2 + L - 1;; ++N) {
+        int r = N;
+        for (int f : {2, 3, 5})
+            while (r % f == 0) r /= f;
+        if (r == 1) break;
+    }
+    return h;
+}
+
 }  // namespace trec
 
 // ------------------------------------------------------------------ plans
@@ -222,6 +249,17 @@ std::vector<float> rweight_kernel(int X, int &N)
 namespace trec {
 
 bool is_iterative(int k) { return k == TREC_SIRT || k == TREC_SART; }
+// FBP and WBP filter the projections, then back-project like BPT.
+bool is_filtered(int k) { return k == TREC_FBP || k == TREC_WBP; }
+
+// Device taps of a filtered method: the R-weighting's circulant kernel of
+// length N (WBP), or the 251 Shepp-Logan taps with N the length of the
+// reference's padded DFT (FBP).
+static void filter_taps(JobPlan &p, int k, int X)
+{
+    if (k == TREC_WBP) p.hk_host = rweight_kernel(X, p.N);
+    else if (k == TREC_FBP) p.hk_host = spatial_taps(0, X, p.N);
+}
 
 trec_status validate(trec_context *ctx, const trec_geom *g, const trec_method *m, const float *angles)
 {
@@ -234,9 +272,9 @@ trec_status validate(trec_context *ctx, const trec_geom *g, const trec_method *m
     if (!std::isfinite(g->offset) || !std::isfinite(g->pitch) || !std::isfinite(g->zshift))
         return fail(ctx, TREC_ERR_INVALID, "non-finite geometry");
     if (g->rules != TREC_RULES_MPI && g->rules != TREC_RULES_CUDA) return fail(ctx, TREC_ERR_INVALID, "bad rules");
-    if (m->method == TREC_FBP || m->method == TREC_ADMM)
-        return fail(ctx, TREC_ERR_UNSUPPORTED, "method %s is not provided", m->method == TREC_FBP ? "FBP" : "ADMM");
-    if (m->method != TREC_BPT && m->method != TREC_WBP && m->method != TREC_SIRT && m->method != TREC_SART)
+    if (m->method == TREC_ADMM) return fail(ctx, TREC_ERR_UNSUPPORTED, "method ADMM is not provided");
+    if (m->method != TREC_BPT && m->method != TREC_FBP && m->method != TREC_WBP && m->method != TREC_SIRT &&
+        m->method != TREC_SART)
         return fail(ctx, TREC_ERR_INVALID, "unknown method %d", m->method);
     if (is_iterative(m->method) && (m->iterations < 0 || m->iterations > 100000))
         return fail(ctx, TREC_ERR_INVALID, "bad iteration count %d", m->iterations);
@@ -286,7 +324,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
     p.stack = stack;
     p.initial = initial;
     p.volume = volume;
-    if (m.method == TREC_BPT || m.method == TREC_WBP) p.m.storage = TREC_STORE_F32;
+    if (m.method == TREC_BPT || is_filtered(m.method)) p.m.storage = TREC_STORE_F32;
     p.coef.resize(8 * (size_t)g.nproj);
     coefficients(g, angles, p.coef.data());
     double ox, oy, oz;
@@ -305,7 +343,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
     float thr_upd = 0.f, thr_res = 0.f;
     if (g.rules == TREC_RULES_CUDA) {
         if (m.method == TREC_SIRT) thr_upd = 1e-5f;
-        if (m.method == TREC_BPT || m.method == TREC_WBP) thr_upd = 1e-5f;
+        if (m.method == TREC_BPT || is_filtered(m.method)) thr_upd = 1e-5f;
         if (m.method == TREC_SART) {
             thr_upd = 1e-6f;
             thr_res = 1e-7f;
@@ -351,7 +389,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             // candidate whose weight is below 1.2e-7
             t.ncand = (int)std::ceil(2.0 / c[1] - 1e-6);
         }
-        if (m.method == TREC_WBP) p.hk_host = rweight_kernel(g.nx, p.N);
+        filter_taps(p, m.method, g.nx);
     } else {
         GeneralJob &q = p.gj;
         q.X = g.nx;
@@ -376,7 +414,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             t.oy = oy;
             t.oz = oz;
         }
-        if (m.method == TREC_WBP) p.hk_host = rweight_kernel(g.nx, p.N);
+        filter_taps(p, m.method, g.nx);
     }
     return TREC_OK;
 }
@@ -415,15 +453,15 @@ size_t carve(JobPlan &p, char *base)
         h.inv_c = k == TREC_SIRT ? (float *)take(4 * T * X) : nullptr;
         h.rwt = p.want_rwt ? (float *)take(4 * P * X) : nullptr;
         p.sumsq = p.m.storage != TREC_STORE_F32 ? (double *)take(sizeof(double)) : nullptr;
-        p.hk = k == TREC_WBP ? (float *)take(4 * (size_t)p.N) : nullptr;
+        p.hk = is_filtered(k) ? (float *)take(4 * p.hk_host.size()) : nullptr;
     } else {
         GeneralJob &q = p.gj;
         p.d_tg = (TiltGeneral *)take(sizeof(TiltGeneral) * P);
         q.tg = p.d_tg;
         q.rval = is_iterative(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
         q.rwt = is_iterative(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
-        p.stack_copy = k == TREC_WBP ? (float *)take(4 * P * Y * X) : nullptr;
-        p.hk = k == TREC_WBP ? (float *)take(4 * (size_t)p.N) : nullptr;
+        p.stack_copy = is_filtered(k) ? (float *)take(4 * P * Y * X) : nullptr;
+        p.hk = is_filtered(k) ? (float *)take(4 * p.hk_host.size()) : nullptr;
     }
     p.bytes = off;
     return off;
@@ -528,10 +566,13 @@ hipError_t issue_prep(JobPlan &p, hipStream_t st)
         if (p.initial) LCHK(launch_block_volume(h, p.d, p.initial, p.m.layout, st));
         else LCHK(launch_zero(h.vol, (size_t)h.nyb * T * X * RB, st));
         if (h.hvol) LCHK(launch_shadow(h, p.d, st));
+        // rows of the float slices: 4-row planes of X cells, row stride 1, cell stride 4
         if (k == TREC_WBP)
-            // rows of the float slices: 4-row planes of X cells, row stride 1, cell stride 4
             LCHK(launch_rweight((float *)h.proj, p.hk, p.N, (int)X, (long long)h.nyb * P * RB, 1, 4, 4,
                                 (long long)X * 4, st));
+        if (k == TREC_FBP)
+            LCHK(launch_fbp_filter((float *)h.proj, p.hk, p.N, (int)X, (long long)h.nyb * P * RB, 1, 4, 4,
+                                   (long long)X * 4, st));
         if (is_iterative(k)) {
             LaunchList L{};
             L.n = 1;
@@ -552,9 +593,13 @@ hipError_t issue_prep(JobPlan &p, hipStream_t st)
         const size_t nv = X * Y * T;
         if (p.initial && p.initial != p.volume) LCHK(launch_copy(p.volume, p.initial, nv, st));
         else if (!p.initial) LCHK(launch_zero(p.volume, nv, st));
-        if (k == TREC_WBP) {
+        if (is_filtered(k)) {
             LCHK(launch_copy(p.stack_copy, p.stack, P * Y * X, st));
-            LCHK(launch_rweight(p.stack_copy, p.hk, p.N, (int)X, (long long)P * Y, 0, 1, 1, (long long)X, st));
+            if (k == TREC_WBP)
+                LCHK(launch_rweight(p.stack_copy, p.hk, p.N, (int)X, (long long)P * Y, 0, 1, 1, (long long)X, st));
+            else
+                LCHK(launch_fbp_filter(p.stack_copy, p.hk, p.N, (int)X, (long long)P * Y, 0, 1, 1, (long long)X,
+                                       st));
         }
         if (is_iterative(k)) {
             LCHK(launch_zero(q.rwt, P * Y * X, st));
@@ -590,7 +635,7 @@ hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
         return launch_general_back(q, BACK_SART, th, 1, q.rval, st);
     }
     default:
-        return launch_general_back(q, BACK_BPT, 0, (int)P, p.m.method == TREC_WBP ? p.stack_copy : p.stack, st);
+        return launch_general_back(q, BACK_BPT, 0, (int)P, is_filtered(p.m.method) ? p.stack_copy : p.stack, st);
     }
 }
 
@@ -1127,6 +1172,55 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
     }
     tl_prof = nullptr;
     HIPCHK(ctx, e);
+    return TREC_OK;
+}
+
+trec_status trec_float_to_byte(trec_context *ctx, const float *in, size_t n, unsigned char *out,
+                               float *stats_host, void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    if (!in || !out || n == 0) return fail(ctx, TREC_ERR_INVALID, "bad conversion arguments");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    trec_status s = ensure_arena(ctx, align_up(sizeof(double) * 2048) + align_up(2 * sizeof(float)));
+    if (s != TREC_OK) return s;
+    double *part = (double *)ctx->arena;
+    float *stats = (float *)(ctx->arena + align_up(sizeof(double) * 2048));
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(ctx, launch_to_byte(in, n, part, stats, out, st));
+    if (stats_host) {
+        HIPCHK(ctx, hipMemcpyAsync(stats_host, stats, 2 * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+    }
+    return TREC_OK;
+}
+
+trec_status trec_filter_fbp(trec_context *ctx, float *rows, int nx, long long nrows, int mode, void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    if (!rows || nx < 1 || nrows < 0 || (mode != 0 && mode != 1))
+        return fail(ctx, TREC_ERR_INVALID, "bad filter arguments");
+    if (nrows == 0) return TREC_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int N = 0;
+    std::vector<float> h = spatial_taps(mode, nx, N);
+    const size_t bytes = sizeof(float) * h.size();
+    trec_status s = ensure_arena(ctx, align_up(bytes));
+    if (s != TREC_OK) return s;
+    HIPCHK(ctx, hipEventSynchronize(ctx->staging_ev));
+    if (ctx->staging_size < bytes) {
+        if (ctx->staging) (void)hipHostFree(ctx->staging);
+        ctx->staging = nullptr;
+        ctx->staging_size = 0;
+        HIPCHK(ctx, hipHostMalloc((void **)&ctx->staging, bytes, hipHostMallocDefault));
+        ctx->staging_size = bytes;
+    }
+    memcpy(ctx->staging, h.data(), bytes);
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->arena, ctx->staging, bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipEventRecord(ctx->staging_ev, st));
+    HIPCHK(ctx, launch_fbp_filter(rows, (const float *)ctx->arena, N, nx, nrows, 0, 1, 1, nx, st));
     return TREC_OK;
 }
 

@@ -1120,6 +1120,133 @@ __global__ __launch_bounds__(WG) void k_rweight(float *__restrict__ data, const 
     }
 }
 
+// FBP spatial filter (ApplyFilterInplace modes 0 / 1, filter_prj.cpp:266-446),
+// on four rows per work-group, in place, with the addressing of k_rweight.
+// The reference convolves through OpenCV's DFT; the same circular
+// convolution is summed here directly:
+//   pad[p]  = row[clamp(p - 1, 0, X - 1)]          p in [0, X + 2)
+//   ext[j]  = pad[j] (j < X + 2), pad[2 (X+2) - 1 - j] (j < 2 (X+2)), else 0,
+//             for j in [0, N), N = the reference's optimal DFT size
+//   out[i]  = sum_k h[k] ext[(i + L/2 + 1 - k) mod N],  L = FBP_TAPS
+// The taps are uniform (scalar loads); each thread produces two neighbouring
+// outputs, which share all but one of their ext reads.
+__global__ __launch_bounds__(WG) void k_fbp_filter(float *__restrict__ data, const float *__restrict__ taps, int N,
+                                                   int X, long long nrows, long long rstride, int xstride, int rpg,
+                                                   long long gstride)
+{
+    extern __shared__ float smem[];
+    float4 *e = reinterpret_cast<float4 *>(smem);  // [N]
+    const long long row0 = (long long)blockIdx.x * 4;
+    long long base[4];
+    bool live[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const long long k = row0 + q;
+        live[q] = k < nrows;
+        const long long kk = live[q] ? k : row0;
+        base[q] = (kk / rpg) * gstride + (kk % rpg) * rstride;
+    }
+    const int npad = X + 2;
+    for (int j = threadIdx.x; j < N; j += WG) {
+        const int p = j < npad ? j : (j < 2 * npad ? 2 * npad - 1 - j : -1);
+        float v[4];
+        if (p < 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = 0.f;
+        } else {
+            const int x = min(max(p - 1, 0), X - 1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = data[base[q] + (long long)x * xstride];
+        }
+        e[j] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+    __syncthreads();
+    const TREC_KCONST float *h = kconst(taps);
+    for (int i0 = 2 * (int)threadIdx.x; i0 < X; i0 += 2 * WG) {
+        // output i0 reads ext[i0 + L/2 + 1 - k], output i0 + 1 the cell one
+        // further on, which output i0 read at tap k - 1
+        float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0;
+        const int top = i0 + FBP_TAPS / 2 + 1;
+        float4 prev = e[top + 1 < N ? top + 1 : top + 1 - N];
+        for (int k = 0; k < FBP_TAPS; ++k) {
+            int j = top - k;
+            j += j < 0 ? N : 0;
+            const float4 cur = e[j];
+            const float hk = h[k];
+            a0.x = fmaf(hk, cur.x, a0.x); a0.y = fmaf(hk, cur.y, a0.y);
+            a0.z = fmaf(hk, cur.z, a0.z); a0.w = fmaf(hk, cur.w, a0.w);
+            a1.x = fmaf(hk, prev.x, a1.x); a1.y = fmaf(hk, prev.y, a1.y);
+            a1.z = fmaf(hk, prev.z, a1.z); a1.w = fmaf(hk, prev.w, a1.w);
+            prev = cur;
+        }
+        const float o0[4] = {a0.x, a0.y, a0.z, a0.w}, o1[4] = {a1.x, a1.y, a1.z, a1.w};
+        // every read of this work-group's rows happened before the barrier
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!live[q]) continue;
+            data[base[q] + (long long)i0 * xstride] = o0[q];
+            if (i0 + 1 < X) data[base[q] + (long long)(i0 + 1) * xstride] = o1[q];
+        }
+    }
+}
+
+// --f2b byte output (recon_z.cu:253-262, CufloatToByteKernel basic.cu:442-474).
+// Partial sums in double of x (pass 0) or of (x - mean)^2 with the
+// difference taken in float (pass 1), one per work-group, in a fixed order.
+__global__ __launch_bounds__(WG) void k_moment_part(const float *__restrict__ a, size_t n, int pass,
+                                                    const float *__restrict__ stats, double *__restrict__ part)
+{
+    const float mean = pass ? stats[0] : 0.f;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n; i += (size_t)gridDim.x * WG) {
+        const float v = a[i];
+        if (pass) {
+            const float d = v - mean;
+            acc += (double)d * (double)d;
+        } else {
+            acc += (double)v;
+        }
+    }
+    __shared__ double red[WG];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+#pragma unroll
+    for (int w = WG / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// stats[0] = mean (pass 0) or stats[1] = standard deviation (pass 1), float.
+__global__ __launch_bounds__(64) void k_moment_finish(const double *__restrict__ part, int nb, size_t n, int pass,
+                                                      float *__restrict__ stats)
+{
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nb; k += 64) acc += part[k];
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) acc += __shfl_down(acc, w, 64);
+    if (threadIdx.x == 0) {
+        if (pass) stats[1] = (float)sqrt(acc / (double)n);
+        else stats[0] = (float)(acc / (double)n);
+    }
+}
+
+// byte = (unsigned char)(((clamp((v - mean) / sd, -10, 10) + 10) / 20) * 255)
+__global__ __launch_bounds__(WG) void k_to_byte(const float *__restrict__ a, size_t n,
+                                                const float *__restrict__ stats, unsigned char *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const float mean = stats[0], sd = stats[1];
+    const float scale = 10.0f;
+    for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n; i += (size_t)gridDim.x * WG) {
+        float v = (a[i] - mean) / sd;
+        v = fmaxf(-scale, fminf(scale, v));
+        v = ((v + scale) / (2.0f * scale)) * 255.0f;
+        out[i] = (unsigned char)v;
+    }
+}
+
 // Sum of squares in double: the stack's scale for the 16-bit forms and the
 // residual norms (one atomic per work-group).
 template <typename T>
@@ -1454,6 +1581,38 @@ hipError_t launch_rweight(float *data, const float *hk, int N, int X, long long 
     const unsigned blocks = (unsigned)((nrows + 3) / 4);
     hipLaunchKernelGGL(k_rweight, dim3(blocks), dim3(WG), shmem, st, data, hk, N, X, nrows, row_stride, x_stride,
                        rows_per_group, group_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_fbp_filter(float *data, const float *taps, int N, int X, long long nrows, long long row_stride,
+                             int x_stride, int rows_per_group, long long group_stride, hipStream_t st)
+{
+    const size_t shmem = sizeof(float4) * (size_t)N;
+    if (shmem > 160 * 1024) return hipErrorInvalidValue;  // rows beyond X ~ 9900 do not fit one CU's LDS
+    if (shmem > 64 * 1024) {
+        static hipError_t attr = hipFuncSetAttribute((const void *)k_fbp_filter,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (attr != hipSuccess) return attr;
+    }
+    const unsigned blocks = (unsigned)((nrows + 3) / 4);
+    hipLaunchKernelGGL(k_fbp_filter, dim3(blocks), dim3(WG), shmem, st, data, taps, N, X, nrows, row_stride,
+                       x_stride, rows_per_group, group_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_byte(const float *a, size_t n, double *part, float *stats, unsigned char *out, hipStream_t st)
+{
+    const int nb = (int)std::min<size_t>(std::max<size_t>(cdiv(n, WG), 1), 2048);
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_moment_part, dim3(nb), dim3(WG), 0, st, a, n, pass, stats, part);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_moment_finish, dim3(1), dim3(64), 0, st, part, nb, n, pass, stats);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const unsigned blocks = (unsigned)std::min<size_t>(std::max<size_t>(cdiv(n, WG), 1), 8192);
+    hipLaunchKernelGGL(k_to_byte, dim3(blocks), dim3(WG), 0, st, a, n, stats, out);
     return hipGetLastError();
 }
 

@@ -200,6 +200,45 @@ bool mrc_write_float(const std::string &path, MrcHeader h, const float *data, st
     return true;
 }
 
+bool mrc_write_bytes(const std::string &path, MrcHeader h, const unsigned char *data, std::string &err)
+{
+    const size_t sec = (size_t)h.nx * h.ny;
+    const unsigned char *mid = data + sec * (size_t)(int)(h.nz * .5);
+    float amin = FLT_MAX, amax = FLT_MIN;
+    long double amean = 0;
+    for (size_t i = sec; i--;) {
+        const float v = mid[i];
+        if (v > amax) amax = v;
+        if (v < amin) amin = v;
+        amean += v;
+    }
+    amean /= sec;
+    h.amin = amin;
+    h.amax = amax;
+    h.amean = (float)amean;
+    h.nlabl = 0;
+    h.mode = MRC_BYTE;
+    h.next = 0;
+    File fh;
+    fh.f = fopen(path.c_str(), "wb");
+    if (!fh.f) {
+        err = "cannot create " + path + ": " + strerror(errno);
+        return false;
+    }
+    const size_t n = sec * h.nz;
+    if (fwrite(&h, sizeof h, 1, fh.f) != 1 || fwrite(data, 1, n, fh.f) != n) {
+        err = "write failed for " + path;
+        return false;
+    }
+    if (fclose(fh.f) != 0) {
+        fh.f = nullptr;
+        err = "close failed for " + path;
+        return false;
+    }
+    fh.f = nullptr;
+    return true;
+}
+
 bool mrc_write_with_header(const std::string &path, const MrcHeader &h, const float *data, std::string &err)
 {
     File fh;

@@ -57,6 +57,12 @@ enum ByteRule { BYTE_SIGNED = 0, BYTE_UNSIGNED_SCALED = 1 };
 bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &data, std::string &err,
                     ByteRule bytes = BYTE_SIGNED);
 
+// --f2b output: mode 0 (InitializeHeader(0), mrcstack.cpp:57-86) and bytes
+// [nz][ny][nx]; amin/amax/amean from section nz/2 read as bytes (the
+// reference's UpdateHeader reads that section as floats, which for a byte
+// file takes the wrong bytes).
+bool mrc_write_bytes(const std::string &path, MrcHeader h, const unsigned char *data, std::string &err);
+
 // Writes header h as given, h.next zero bytes of extended header, and
 // h.nx * h.ny * h.nz floats.
 bool mrc_write_with_header(const std::string &path, const MrcHeader &h, const float *data, std::string &err);

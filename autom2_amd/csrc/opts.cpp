@@ -81,10 +81,10 @@ void usage()
                  "    MRC file as initial model for the reconstruction (optional)\n"
                  "[--geometry(-g) offset,pitch_angle,zshift,thickness]\n"
                  "[--method(-m) Method]\n"
-                 "    BPT | WBP | SIRT,iterations,relax | SART,iterations,relax\n"
-                 "    (FBP and ADMM,iterations,cgiter,relax,soft are not provided)\n"
+                 "    BPT | FBP | WBP | SIRT,iterations,relax | SART,iterations,relax\n"
+                 "    (ADMM,iterations,cgiter,relax,soft is not provided)\n"
                  "[--axis(-a) y|z]  volume orientation: y = TiltRec (X,thickness,Y), z = TiltRecZ (X,Y,thickness)\n"
-                 "[--f2b(-f)]  not provided\n"
+                 "[--f2b(-f)]  byte output: (v - mean) / sd clamped to +-10, mapped to 0..255\n"
                  "[--rules mpi|cuda]  thresholds and y origin of TiltRec-mpi or TiltRec-cuda\n"
                  "[--storage f32|f16|bf16]  projection/residual storage of SIRT and SART\n"
                  "[--device N]  HIP device\n"

@@ -104,9 +104,9 @@ int main(int argc, char **argv)
                rules == TREC_RULES_MPI ? "mpi" : "cuda");
         return 0;
     }
-    if (o.f2b) return die(3, "--f2b (byte output) is not provided; write float and convert afterwards");
     int kind;
     if (o.method == "BPT") kind = TREC_BPT;
+    else if (o.method == "FBP") kind = TREC_FBP;
     else if (o.method == "WBP") kind = TREC_WBP;
     else if (o.method == "SIRT") kind = TREC_SIRT;
     else if (o.method == "SART") kind = TREC_SART;
@@ -161,12 +161,35 @@ int main(int argc, char **argv)
         return die(6, "reconstruction failed: %s", msg.c_str());
     }
     const double t_rec = now_s();
-    trec_context_destroy(ctx);
-    std::vector<float> vol(nvol);
-    if (hipMemcpy(vol.data(), dvol.p, 4 * nvol, hipMemcpyDeviceToHost) != hipSuccess)
-        return die(5, "copy from device failed");
     MrcHeader hout = layout == TREC_LAYOUT_Y ? mrc_new_header(X, T, Y) : mrc_new_header(X, Y, T);
-    if (!mrc_write_float(o.output, hout, vol.data(), err)) return die(4, "%s", err.c_str());
+    if (o.f2b) {
+        // byte output over the whole volume's mean and deviation, as
+        // TiltRecZ-cuda writes it (recon_z.cu:253-262)
+        DevBuf dbytes;
+        float stats[2] = {0, 0};
+        if (hipMalloc(&dbytes.p, nvol) != hipSuccess) {
+            trec_context_destroy(ctx);
+            return die(5, "device allocation failed");
+        }
+        s = trec_float_to_byte(ctx, (const float *)dvol.p, nvol, (unsigned char *)dbytes.p, stats, nullptr);
+        if (s != TREC_OK) {
+            std::string msg = std::string(trec_status_string(s)) + ": " + trec_last_error(ctx);
+            trec_context_destroy(ctx);
+            return die(6, "byte conversion failed: %s", msg.c_str());
+        }
+        trec_context_destroy(ctx);
+        std::vector<unsigned char> bytes(nvol);
+        if (hipMemcpy(bytes.data(), dbytes.p, nvol, hipMemcpyDeviceToHost) != hipSuccess)
+            return die(5, "copy from device failed");
+        printf("f2b: mean %g, standard deviation %g\n", stats[0], stats[1]);
+        if (!mrc_write_bytes(o.output, hout, bytes.data(), err)) return die(4, "%s", err.c_str());
+    } else {
+        trec_context_destroy(ctx);
+        std::vector<float> vol(nvol);
+        if (hipMemcpy(vol.data(), dvol.p, 4 * nvol, hipMemcpyDeviceToHost) != hipSuccess)
+            return die(5, "copy from device failed");
+        if (!mrc_write_float(o.output, hout, vol.data(), err)) return die(4, "%s", err.c_str());
+    }
     const double t_end = now_s();
     printf("read %.3f s, upload %.3f s, reconstruct %.3f s, write %.3f s, total %.3f s\n", t_read - t0,
            t_up - t_read, t_rec - t_up, t_end - t_rec, t_end - t0);

@@ -47,6 +47,8 @@ trec_status fail(trec_context *ctx, trec_status s, const char *fmt, ...);
 void coefficients(const trec_geom &g, const float *angles, double *coef);
 // Circulant R-weighting kernel of length N = X + X/10.
 std::vector<float> rweight_kernel(int X, int &N);
+// FBP taps (mode 0 Shepp-Logan, 1 Ram-Lak) and the padded length N.
+std::vector<float> spatial_taps(int mode, int X, int &N);
 
 trec_status validate(trec_context *ctx, const trec_geom *g, const trec_method *m, const float *angles);
 trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const trec_method &m, const float *angles,

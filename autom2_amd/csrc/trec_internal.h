@@ -123,6 +123,13 @@ hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode
 hipError_t launch_rweight(float *data, const float *hk, int N, int X, long long nrows,
                           long long row_stride, int x_stride, int rows_per_group,
                           long long group_stride, hipStream_t st);
+// FBP spatial filter in place (FBP_TAPS taps, ext length N), same addressing.
+constexpr int FBP_TAPS = 251;
+hipError_t launch_fbp_filter(float *data, const float *taps, int N, int X, long long nrows, long long row_stride,
+                             int x_stride, int rows_per_group, long long group_stride, hipStream_t st);
+// --f2b: mean and standard deviation of n floats into stats[0..1] (part
+// holds 2048 doubles), then the byte mapping into out.
+hipError_t launch_to_byte(const float *a, size_t n, double *part, float *stats, unsigned char *out, hipStream_t st);
 // out += sum of squares (in double) of n values of the storage type.
 hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipStream_t st);
 

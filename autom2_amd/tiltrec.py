@@ -76,6 +76,8 @@ SIGNATURES = [
     ("trec_workspace_size", _I, [ctypes.POINTER(Geom), ctypes.POINTER(Method), ctypes.POINTER(ctypes.c_size_t)]),
     ("trec_reconstruct", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, ctypes.POINTER(Method), _P, _P]),
     ("trec_filter_rweight", _I, [_P, _P, _I, ctypes.c_longlong, _P]),
+    ("trec_filter_fbp", _I, [_P, _P, _I, ctypes.c_longlong, _I, _P]),
+    ("trec_float_to_byte", _I, [_P, _P, ctypes.c_size_t, _P, _PF, _P]),
     ("trec_forward_project", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, _P, _P]),
     ("trec_backproject", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, _P]),
     ("trec_sirt_iterate", _I, [_P, ctypes.POINTER(Geom), _PF, _P, ctypes.c_float, _P, _P]),
@@ -263,6 +265,20 @@ class Context:
     def filter_rweight(self, rows, stream=None):
         nx = rows.shape[-1]
         _check(lib().trec_filter_rweight(self._h, _ptr(rows), nx, rows.numel() // nx, _stream(stream)), self._h)
+
+    def filter_fbp(self, rows, mode=0, stream=None):
+        """FBP's spatial filter in place on [..., nx] rows (0: Shepp-Logan, 1: Ram-Lak)."""
+        nx = rows.shape[-1]
+        _check(lib().trec_filter_fbp(self._h, _ptr(rows), nx, rows.numel() // nx, int(mode), _stream(stream)),
+               self._h)
+
+    def float_to_byte(self, values, out, stream=None):
+        """--f2b conversion of a float device tensor into a uint8 device
+        tensor of the same size; returns (mean, sd)."""
+        stats = (ctypes.c_float * 2)()
+        _check(lib().trec_float_to_byte(self._h, _ptr(values), values.numel(), _ptr(out),
+                                        ctypes.cast(stats, _PF), _stream(stream)), self._h)
+        return stats[0], stats[1]
 
     def forward_project(self, geom, angles, volume, rval, rwt=None, stream=None):
         pa, _keep = _angles(angles)

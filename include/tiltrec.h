@@ -19,6 +19,11 @@
  *     (TiltRec_cuda_z/recon_z.cuh:17-39)                 layout TREC_LAYOUT_Z
  *   ApplyFilterInplace(mode 2)                          trec_filter_rweight
  *     (src/TiltRec/src/filter/filter_prj.h:24)
+ *   ApplyFilterInplace(modes 0, 1)                      trec_filter_fbp
+ *     (filter_prj.cpp:266-446, 738-771)
+ *   --f2b: thrust moments + CufloatToByteKernel         trec_float_to_byte
+ *     (TiltRec_cuda_z/recon_z.cu:253-262,
+ *      TiltRec_cuda_y/basic.cu:442-474)
  *   CuReprojectKernel + CuReWeightKernel                trec_forward_project
  *     (TiltRec_cuda_y/basic.cu:237-264,357-382)
  *   CuBackProjKernel (basic.cu:383-415)                 trec_backproject
@@ -65,7 +70,7 @@ typedef enum {
 
 typedef enum {
     TREC_BPT = 0,  /* plain back-projection */
-    TREC_FBP = 1,  /* Shepp-Logan filtered back-projection (not provided) */
+    TREC_FBP = 1,  /* Shepp-Logan filtered back-projection */
     TREC_WBP = 2,  /* R-weighted back-projection */
     TREC_SIRT = 3,
     TREC_SART = 4,
@@ -157,6 +162,23 @@ trec_status trec_reconstruct(trec_context *ctx, const trec_geom *geom,
  * filter_prj.cpp:652-802). */
 trec_status trec_filter_rweight(trec_context *ctx, float *rows, int nx,
                                 long long nrows, void *stream);
+
+/* FBP's spatial filter of nrows rows of nx floats in place (ApplyFilterInplace
+ * modes 0 and 1, filter_prj.cpp:137-178, 266-446): mode 0 Shepp-Logan (what
+ * FBP uses), 1 Ram-Lak; 251 taps over the reference's symmetrised and
+ * mirror-padded row.  Rows up to about 9,900 pixels. */
+trec_status trec_filter_fbp(trec_context *ctx, float *rows, int nx,
+                            long long nrows, int mode, void *stream);
+
+/* --f2b byte output (TiltRecZ-cuda, recon_z.cu:253-262; CufloatToByteKernel,
+ * basic.cu:442-474): the mean and standard deviation of the n floats of `in`,
+ * then out[i] = (unsigned char)(((clamp((in[i] - mean) / sd, -10, 10) + 10)
+ * / 20) * 255).  The moments are summed in double and rounded to float.
+ * stats_host (may be NULL) receives {mean, sd}; the call then waits for the
+ * stream. */
+trec_status trec_float_to_byte(trec_context *ctx, const float *in, size_t n,
+                               unsigned char *out, float *stats_host,
+                               void *stream);
 
 /* Forward projection of a layout-Y volume: rval[p][y][x] = splat of the
  * volume, rwt[p][y][x] = splat of ones (Reproject, main-mpi.cpp:286-344).

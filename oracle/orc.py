@@ -48,6 +48,12 @@ def lib():
         for n in ("orc_sirt_f64", "orc_sart_f64"):
             getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, ctypes.c_double, _D]
         L.orc_rweight_f64.argtypes = [_D, _I, _I]
+        L.orc_optimal_dft_size.argtypes = [_I]
+        L.orc_optimal_dft_size.restype = _I
+        L.orc_spatial_taps.argtypes = [_I, _I, _F]
+        L.orc_fbp_filter_f64.argtypes = [_D, _I, _I, _I]
+        L.orc_bytes_from_stats.argtypes = [_F, ctypes.c_long, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
+        L.orc_float_to_byte.argtypes = [_F, ctypes.c_long, ctypes.c_void_p, _F]
         L.orc_xf_table.argtypes = [_I, _F]
         L.orc_xf_inverse.argtypes = [_F, _I, _I, _D]
         L.orc_image_mean.argtypes = [_F, ctypes.c_long]
@@ -196,6 +202,48 @@ def rel_l2(a, b):
     nb = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))
 
+
+def optimal_dft_size(n):
+    return lib().orc_optimal_dft_size(int(n))
+
+
+def spatial_taps(mode, L=251):
+    """Shepp-Logan (mode 0) or Ram-Lak (mode 1) taps, filter_prj.cpp:137-178."""
+    h = np.zeros(L, np.float32)
+    lib().orc_spatial_taps(int(mode), int(L), _fp(h))
+    return h
+
+
+def fbp_filter_f64(rows, mode=0):
+    """ApplyFilterInplace mode 0/1 on [..., X] rows, FP64 (filter_prj.cpp:266-446)."""
+    a = np.array(rows, dtype=np.float64, copy=True)
+    shp = a.shape
+    a = np.ascontiguousarray(a.reshape(-1, shp[-1]))
+    lib().orc_fbp_filter_f64(_dp(a), shp[-1], a.shape[0], int(mode))
+    return a.reshape(shp)
+
+
+def fbp_f32(coef, proj, T, org, mode=0, vol=None):
+    """TiltRec's FBP: the spatial filter (FP64, rounded to float as the
+    reference stores it) followed by the FP32 back-projection."""
+    filt = fbp_filter_f64(proj.astype(np.float64), mode).astype(np.float32)
+    return bpt_f32(coef, filt, T, org, vol)
+
+def float_to_byte(values):
+    """--f2b conversion (recon_z.cu:253-262): (bytes uint8, (mean, sd))."""
+    v = np.ascontiguousarray(values, np.float32).ravel()
+    out = np.empty(v.size, np.uint8)
+    st = np.zeros(2, np.float32)
+    lib().orc_float_to_byte(_fp(v), v.size, out.ctypes.data, _fp(st))
+    return out.reshape(np.shape(values)), (float(st[0]), float(st[1]))
+
+
+def bytes_from_stats(values, mean, sd):
+    """The byte mapping alone, for given float mean and deviation."""
+    v = np.ascontiguousarray(values, np.float32).ravel()
+    out = np.empty(v.size, np.uint8)
+    lib().orc_bytes_from_stats(_fp(v), v.size, float(mean), float(sd), out.ctypes.data)
+    return out.reshape(np.shape(values))
 
 # ------------------------------------------------------------ .xf application
 # mrcstack (src/mrctools/src/mrcstack/main.cpp:40-80) with OpenCV's

@@ -581,3 +581,129 @@ void orc_rweight_f64(double *data, int W, int rows)
     free(ct);
     free(st);
 }
+
+/* ------------------------------------------------------------------ FBP
+ * ApplyFilterInplace modes 0 (Shepp-Logan) and 1 (Ram-Lak), the filter of
+ * TiltRec's FBP (filter_prj.cpp:137-178 taps, 266-446 the filtering, 738-771
+ * the dispatch; main-mpi.cpp:835-840).  Rows are filtered independently (the
+ * reference transforms with CV_DXT_ROWS, so its y padding never mixes
+ * rows).  Per row of W pixels:
+ *   pad  = row with its first and last pixel repeated (W + 2 cells, the
+ *          symmetrised 1-pixel border);
+ *   ext  = pad, then its mirror image pad[2 (W+2) - 1 - j], up to N =
+ *          cvGetOptimalDFTSize(W + 2 + L - 1) cells or 2 (W+2), zeros beyond;
+ *   y    = circular convolution of ext with the L = 251 taps (DFT product,
+ *          inverse DFT scaled by 1/N);
+ *   out[i] = y[L/2 + 1 + i].
+ * The reference evaluates it with OpenCV's float DFT; this is the same
+ * convolution summed directly in double, the FP64 form. */
+
+/* cvGetOptimalDFTSize: the smallest 2^a 3^b 5^c >= n. */
+int orc_optimal_dft_size(int n)
+{
+    for (int m = n > 1 ? n : 1;; m++) {
+        int r = m;
+        while (r % 2 == 0) r /= 2;
+        while (r % 3 == 0) r /= 3;
+        while (r % 5 == 0) r /= 5;
+        if (r == 1) return m;
+    }
+}
+
+/* The L taps, centred: SheppLogan / RamLak of filter_prj.cpp:137-178. */
+void orc_spatial_taps(int mode, int L, float *h)
+{
+    if (mode == 0) {
+        float n = -((float)L - 1.0) / 2.0;
+        for (int i = 0; i < L; i++) {
+            h[i] = (float)(-2.0 / (M_PI * (4.0 * n * n - 1.0)));
+            n = n + 1.0;
+        }
+    } else {
+        int n = -(L - 1) / 2;
+        for (int i = 0; i < L; i++) {
+            if (n == 0)
+                h[i] = M_PI / 4;
+            else if (abs(n % 2) == 1)
+                h[i] = -1.0 / (M_PI * n * n);
+            else
+                h[i] = 0;
+            n++;
+        }
+    }
+}
+
+void orc_fbp_filter_f64(double *data, int W, int rows, int mode)
+{
+    const int L = 251, npad = W + 2;
+    const int N = orc_optimal_dft_size(npad + L - 1);
+    float h[251];
+    orc_spatial_taps(mode, L, h);
+    double *ext = (double *)malloc(sizeof(double) * N);
+    double *out = (double *)malloc(sizeof(double) * W);
+    for (int r = 0; r < rows; r++) {
+        double *row = data + (size_t)r * W;
+        for (int j = 0; j < N; j++) {
+            int p;
+            if (j < npad)
+                p = j;
+            else if (j < 2 * npad)
+                p = 2 * npad - 1 - j;
+            else
+                p = -1;
+            if (p < 0) {
+                ext[j] = 0;
+            } else {
+                int x = p - 1;
+                x = x < 0 ? 0 : (x > W - 1 ? W - 1 : x);
+                ext[j] = row[x];
+            }
+        }
+        for (int i = 0; i < W; i++) {
+            double acc = 0;
+            for (int k = 0; k < L; k++) {
+                int j = (i + L / 2 + 1 - k) % N;
+                if (j < 0) j += N;
+                acc += (double)h[k] * ext[j];
+            }
+            out[i] = acc;
+        }
+        for (int i = 0; i < W; i++) row[i] = out[i];
+    }
+    free(ext);
+    free(out);
+}
+
+/* ------------------------------------------------------------------ f2b
+ * --f2b byte output as TiltRecZ-cuda writes it (recon_z.cu:253-262,
+ * CufloatToByteKernel basic.cu:442-474): mean and standard deviation of the
+ * volume (here summed in double and rounded to float; the reference sums
+ * in float through thrust), then per value
+ *   z = (v - mean) / sd, clamped to [-10, 10], byte = (uchar)(((z + 10) / 20) * 255).
+ * stats receives {mean, sd}. */
+void orc_bytes_from_stats(const float *v, long n, float mean, float sd, unsigned char *out)
+{
+    const float scale = 10.0f;
+    for (long i = 0; i < n; i++) {
+        float z = (v[i] - mean) / sd;
+        z = fmaxf(-scale, fminf(scale, z));
+        z = ((z + scale) / (2.0f * scale)) * 255.0f;
+        out[i] = (unsigned char)z;
+    }
+}
+
+void orc_float_to_byte(const float *v, long n, unsigned char *out, float *stats)
+{
+    double s = 0;
+    for (long i = 0; i < n; i++) s += v[i];
+    const float mean = (float)(s / (double)n);
+    double q = 0;
+    for (long i = 0; i < n; i++) {
+        const float d = v[i] - mean;
+        q += (double)d * (double)d;
+    }
+    const float sd = (float)sqrt(q / (double)n);
+    orc_bytes_from_stats(v, n, mean, sd, out);
+    stats[0] = mean;
+    stats[1] = sd;
+}
