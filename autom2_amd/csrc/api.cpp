@@ -249,6 +249,8 @@ std::vector<float> spatial_taps(int mode, int X, int &N)
 namespace trec {
 
 bool is_iterative(int k) { return k == TREC_SIRT || k == TREC_SART; }
+// Methods that forward-project: they need the residual and ray weights.
+bool uses_forward(int k) { return is_iterative(k) || k == TREC_ADMM; }
 // FBP and WBP filter the projections, then back-project like BPT.
 bool is_filtered(int k) { return k == TREC_FBP || k == TREC_WBP; }
 
@@ -272,12 +274,14 @@ trec_status validate(trec_context *ctx, const trec_geom *g, const trec_method *m
     if (!std::isfinite(g->offset) || !std::isfinite(g->pitch) || !std::isfinite(g->zshift))
         return fail(ctx, TREC_ERR_INVALID, "non-finite geometry");
     if (g->rules != TREC_RULES_MPI && g->rules != TREC_RULES_CUDA) return fail(ctx, TREC_ERR_INVALID, "bad rules");
-    if (m->method == TREC_ADMM) return fail(ctx, TREC_ERR_UNSUPPORTED, "method ADMM is not provided");
     if (m->method != TREC_BPT && m->method != TREC_FBP && m->method != TREC_WBP && m->method != TREC_SIRT &&
-        m->method != TREC_SART)
+        m->method != TREC_SART && m->method != TREC_ADMM)
         return fail(ctx, TREC_ERR_INVALID, "unknown method %d", m->method);
-    if (is_iterative(m->method) && (m->iterations < 0 || m->iterations > 100000))
+    if ((is_iterative(m->method) || m->method == TREC_ADMM) && (m->iterations < 0 || m->iterations > 100000))
         return fail(ctx, TREC_ERR_INVALID, "bad iteration count %d", m->iterations);
+    if (m->method == TREC_ADMM &&
+        (m->cg_iterations < 0 || m->cg_iterations > 100000 || !std::isfinite(m->soft) || m->soft < 0))
+        return fail(ctx, TREC_ERR_INVALID, "bad ADMM parameters cgiter=%d soft=%g", m->cg_iterations, m->soft);
     if (!std::isfinite(m->gamma)) return fail(ctx, TREC_ERR_INVALID, "non-finite gamma");
     if (m->storage < TREC_STORE_F32 || m->storage > TREC_STORE_BF16) return fail(ctx, TREC_ERR_INVALID, "bad storage");
     if (m->layout != TREC_LAYOUT_Y && m->layout != TREC_LAYOUT_Z) return fail(ctx, TREC_ERR_INVALID, "bad layout");
@@ -324,7 +328,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
     p.stack = stack;
     p.initial = initial;
     p.volume = volume;
-    if (m.method == TREC_BPT || is_filtered(m.method)) p.m.storage = TREC_STORE_F32;
+    if (m.method == TREC_BPT || is_filtered(m.method) || m.method == TREC_ADMM) p.m.storage = TREC_STORE_F32;
     p.coef.resize(8 * (size_t)g.nproj);
     coefficients(g, angles, p.coef.data());
     double ox, oy, oz;
@@ -338,6 +342,10 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
         if (!(c[2] == 0.0 && c[4] == 0.0 && c[5] == 0.0 && c[6] == 1.0 && c[7] == 0.0 && c[1] >= 0.2)) rows = false;
     }
     p.path = rows ? PATH_ROWS : PATH_GENERAL;
+    if (m.method == TREC_ADMM && !rows)
+        return fail(ctx, TREC_ERR_UNSUPPORTED,
+                    "ADMM runs on the row-blocked path only: pitch 0 and every |tilt + offset| up to about 78 "
+                    "degrees");
 
     // numeric rules (see trec_rules)
     float thr_upd = 0.f, thr_res = 0.f;
@@ -349,9 +357,13 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             thr_res = 1e-7f;
         }
     }
+    // ADMM: a tap contributes where the ray weight exceeds 10e-6 (both
+    // reference binaries, main-mpiz.cpp:740-785, baiscz.cu:446-480)
+    if (m.method == TREC_ADMM) thr_res = 10e-6f;
     const int iters = is_iterative(m.method) ? m.iterations : 0;
     if (m.method == TREC_SIRT) p.stages = iters;
     else if (m.method == TREC_SART) p.stages = iters * g.nproj;
+    else if (m.method == TREC_ADMM) p.stages = 0;  // issued by issue_admm
     else p.stages = 1;
 
     if (p.path == PATH_ROWS) {
@@ -447,13 +459,24 @@ size_t carve(JobPlan &p, char *base)
                      ? take(64 * nyb * T * (X + 1) + 16 * (FGS + FW_G))
                      : nullptr;
         h.proj = take(es * nyb * P * X * RB);
-        h.res = is_iterative(k) ? take(es * nyb * P * X * RB) : nullptr;
+        h.res = uses_forward(k) ? take(es * nyb * P * X * RB) : nullptr;
         h.part = (k == TREC_SIRT && h.nzc > 1) ? (float *)take(4 * (size_t)h.nzc * nyb * P * X * RB) : nullptr;
-        h.inv_rwt = is_iterative(k) ? (float *)take(4 * P * X) : nullptr;
+        h.inv_rwt = uses_forward(k) ? (float *)take(4 * P * X) : nullptr;
         h.inv_c = k == TREC_SIRT ? (float *)take(4 * T * X) : nullptr;
         h.rwt = p.want_rwt ? (float *)take(4 * P * X) : nullptr;
         p.sumsq = p.m.storage != TREC_STORE_F32 ? (double *)take(sizeof(double)) : nullptr;
         p.hk = is_filtered(k) ? (float *)take(4 * p.hk_host.size()) : nullptr;
+        if (k == TREC_ADMM) {
+            // the CG and ADMM vectors, each a row-blocked volume, and the
+            // descriptors of the three back-projection passes
+            AdmmVecs &a = p.av;
+            a.n = nyb * T * X * RB;
+            float **vecs[7] = {&a.htb, &a.u, &a.d, &a.r, &a.p, &a.h, &a.ax};
+            for (float **v : vecs) *v = (float *)take(4 * a.n + 16 * FGS);
+            a.part = (double *)take(sizeof(double) * 2 * ADMM_DOT_BLOCKS);
+            a.S = (double *)take(sizeof(double) * 8);
+            for (int i = 0; i < 3; ++i) p.d_adm[i] = (JobDesc *)take(sizeof(JobDesc));
+        }
     } else {
         GeneralJob &q = p.gj;
         p.d_tg = (TiltGeneral *)take(sizeof(TiltGeneral) * P);
@@ -510,6 +533,25 @@ trec_status upload(trec_context *ctx, const std::vector<JobPlan *> &plans, hipSt
             ones[i].zc_len = p.h.T;
             pcs.push_back({p.d, &p.h, sizeof(JobDesc)});
             pcs.push_back({p.d_ones, &ones[i], sizeof(JobDesc)});
+            if (p.m.method == TREC_ADMM) {
+                // 0: htb = A^T b (the stack as the image, no x term);
+                // 1: ax = op(vol); 2: h = op(p).  The operator's forward
+                // projection reads the stack after it has been zeroed, so its
+                // residual is -R / w; out = -sum s + mu x.
+                for (int k = 0; k < 3; ++k) p.h_adm[k] = p.h;
+                p.h_adm[0].res = p.h.proj;
+                p.h_adm[0].out = p.av.htb;
+                p.h_adm[0].lin_a = 1.f;
+                p.h_adm[0].lin_b = 0.f;
+                p.h_adm[1].out = p.av.ax;
+                p.h_adm[2].vol = p.av.p;
+                p.h_adm[2].out = p.av.h;
+                for (int k = 1; k < 3; ++k) {
+                    p.h_adm[k].lin_a = -1.f;
+                    p.h_adm[k].lin_b = p.m.gamma;
+                }
+                for (int k = 0; k < 3; ++k) pcs.push_back({p.d_adm[k], &p.h_adm[k], sizeof(JobDesc)});
+            }
             pcs.push_back({const_cast<TiltConst *>(p.h.tc), p.tc.data(), sizeof(TiltConst) * p.tc.size()});
         } else {
             pcs.push_back({p.d_tg, p.tg.data(), sizeof(TiltGeneral) * p.tg.size()});
@@ -573,7 +615,7 @@ hipError_t issue_prep(JobPlan &p, hipStream_t st)
         if (k == TREC_FBP)
             LCHK(launch_fbp_filter((float *)h.proj, p.hk, p.N, (int)X, (long long)h.nyb * P * RB, 1, 4, 4,
                                    (long long)X * 4, st));
-        if (is_iterative(k)) {
+        if (uses_forward(k)) {
             LaunchList L{};
             L.n = 1;
             L.off[0] = 0;
@@ -736,9 +778,67 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
     return hipSuccess;
 }
 
+// TiltRecZ-mpi's ADMM (main-mpiz.cpp:958-1013) on one row-blocked job, after
+// issue_prep: htb = A^T b; then per outer iteration htb += (u - d) mu, the
+// CG of applycg (:835-935) on op(v) = htb from the current volume, the soft
+// threshold and the d update.
+hipError_t issue_admm(JobPlan &p, hipStream_t st)
+{
+    const JobDesc &h = p.h;
+    AdmmVecs a = p.av;
+    a.vol = h.vol;
+    a.mu = p.m.gamma;
+    a.soft = p.m.soft;
+    auto one = [&](int k, unsigned blocks) {
+        LaunchList L{};
+        L.n = 1;
+        L.off[0] = 0;
+        L.off[1] = blocks;
+        L.job[0] = p.d_adm[k];
+        L.theta0[0] = 0;
+        L.ntheta[0] = 0;
+        return L;
+    };
+    auto back = [&](int k) {
+        ProfScope ps(PROF_BACK, st);
+        const LaunchList L = one(k, back_blocks(p.h_adm[k]));
+        return launch_back(L, L.off[1], TREC_STORE_F32, BACK_LIN, st);
+    };
+    auto op = [&](int k) {
+        {
+            ProfScope ps(PROF_FORWARD, st);
+            const LaunchList L = one(k, fwd_blocks(p.h_adm[k], h.P));
+            LCHK(launch_forward(L, L.off[1], TREC_STORE_F32, true, st));
+        }
+        return back(k);
+    };
+    LCHK(back(0));
+    LCHK(launch_zero((float *)h.proj, (size_t)h.nyb * h.P * h.X * RB, st));
+    LCHK(launch_zero(a.u, a.n, st));
+    LCHK(launch_zero(a.d, a.n, st));
+    for (int it = 0; it < p.m.iterations; ++it) {
+        LCHK(launch_admm_vec(a, ADMM_HTB, st));
+        LCHK(op(1));
+        LCHK(launch_admm_vec(a, ADMM_R0, st));
+        for (int cg = 0; cg < p.m.cg_iterations; ++cg) {
+            LCHK(op(2));
+            LCHK(launch_admm_dots(a, 0, st));
+            LCHK(launch_admm_vec(a, ADMM_STEP, st));
+            LCHK(op(1));
+            LCHK(launch_admm_vec(a, ADMM_RES, st));
+            LCHK(launch_admm_dots(a, 1, st));
+            LCHK(launch_admm_vec(a, ADMM_DIR, st));
+        }
+        LCHK(launch_admm_vec(a, ADMM_SOFT, st));
+    }
+    return hipSuccess;
+}
+
 hipError_t issue_all(const std::vector<JobPlan *> &plans, hipStream_t st)
 {
     for (auto *p : plans) LCHK(issue_prep(*p, st));
+    for (auto *p : plans)
+        if (p->m.method == TREC_ADMM) LCHK(issue_admm(*p, st));
     std::vector<int> stage(plans.size(), 0);
     int S = 0;
     for (auto *p : plans) S = std::max(S, p->stages);
@@ -755,10 +855,10 @@ std::string graph_key(trec_context *ctx, const std::vector<JobPlan *> &plans)
     k += b;
     for (auto *pp : plans) {
         const JobPlan &p = *pp;
-        snprintf(b, sizeof b, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%a,%p,%p,%p,%d,%d|", p.g.nx, p.g.ny, p.g.nproj,
+        snprintf(b, sizeof b, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%a,%p,%p,%p,%d,%d,%d,%a|", p.g.nx, p.g.ny, p.g.nproj,
                  p.g.thickness, p.g.rules, p.m.method, p.m.iterations, p.m.storage, p.m.layout, p.path,
                  (double)p.m.gamma, (const void *)p.stack, (const void *)p.initial, (void *)p.volume,
-                 p.path == PATH_ROWS ? p.h.nzc : 0, (int)p.want_rwt);
+                 p.path == PATH_ROWS ? p.h.nzc : 0, (int)p.want_rwt, p.m.cg_iterations, (double)p.m.soft);
         k += b;
     }
     return k;

@@ -915,7 +915,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
                     const half2v w = {(_Float16)w0, (_Float16)w1f};
                     float ic = 1.f;
                     bool upd = true;
-                    if constexpr (MODE != BACK_SIRT) {
+                    if constexpr (MODE != BACK_SIRT && MODE != BACK_LIN) {
                         float cw = 0.f;
                         cw += ((unsigned)g < (unsigned)X) ? w0 : 0.f;
                         cw += ((unsigned)(g + 1) < (unsigned)X) ? w1 : 0.f;
@@ -931,7 +931,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
 #pragma unroll
                             for (int h = 0; h < 4; ++h) {
                                 const half2v pr = __builtin_bit_cast(half2v, ev[h]);
-                                if constexpr (MODE == BACK_SIRT)
+                                if constexpr (MODE == BACK_SIRT || MODE == BACK_LIN)
                                     val[k][4 * q + h] = __builtin_amdgcn_fdot2(pr, w, val[k][4 * q + h], false);
                                 else
                                     val[k][4 * q + h] =
@@ -948,7 +948,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
                     r0[4 * q] = a.x; r0[4 * q + 1] = a.y; r0[4 * q + 2] = a.z; r0[4 * q + 3] = a.w;
                     r1[4 * q] = bq.x; r1[4 * q + 1] = bq.y; r1[4 * q + 2] = bq.z; r1[4 * q + 3] = bq.w;
                 }
-                if constexpr (MODE == BACK_SIRT) {
+                if constexpr (MODE == BACK_SIRT || MODE == BACK_LIN) {
 #pragma unroll
                     for (int r = 0; r < RB; ++r) val[k][r] = fmaf(w1, r1[r], fmaf(w0, r0[r], val[k][r]));
                 } else {
@@ -977,6 +977,21 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
         float v[RB];
 #pragma unroll
         for (int r = 0; r < RB; ++r) v[r] = 0.f;
+        if constexpr (MODE == BACK_LIN) {
+            // out = a * sum s + b * x, each product rounded on its own
+            if (live) {
+                Slice<float>::load(vslice, X, x, v);
+                float o[RB];
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    float pa = J.lin_a * val[k][r], pb = J.lin_b * v[r];
+                    asm volatile("" : "+v"(pa), "+v"(pb));
+                    o[r] = pa + pb;
+                }
+                Slice<float>::store(J.out + ((size_t)yb * J.T + z) * X * RB, X, x, o);
+            }
+            continue;
+        }
         if (live) {
             Slice<float>::load(vslice, X, x, v);
             bool upd = true;
@@ -996,7 +1011,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
             }
             if (upd) Slice<float>::store(vslice, X, x, v);
         }
-        if constexpr (HALF && MODE != BACK_BPT) {
+        if constexpr (HALF && MODE != BACK_BPT && MODE != BACK_LIN) {
             // Keep the half copy in step: lane xl writes pair entry x (cells
             // x and x+1, the latter from lane xl+1); the entry straddling two
             // tiles is written in halves, its first cell by this tile's last
@@ -1563,6 +1578,9 @@ hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode
     switch (mode) {
     case BACK_SIRT: launch_back_mode<BACK_SIRT>(L, nblk, storage, st); break;
     case BACK_BPT: launch_back_mode<BACK_BPT>(L, nblk, storage, st); break;
+    case BACK_LIN:
+        hipLaunchKernelGGL((k_back<float, BACK_LIN>), dim3(nblk), dim3(WG), 0, st, L, nblk);
+        break;
     default: launch_back_mode<BACK_SART>(L, nblk, storage, st); break;
     }
     return hipGetLastError();

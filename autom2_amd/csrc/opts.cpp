@@ -81,8 +81,8 @@ void usage()
                  "    MRC file as initial model for the reconstruction (optional)\n"
                  "[--geometry(-g) offset,pitch_angle,zshift,thickness]\n"
                  "[--method(-m) Method]\n"
-                 "    BPT | FBP | WBP | SIRT,iterations,relax | SART,iterations,relax\n"
-                 "    (ADMM,iterations,cgiter,relax,soft is not provided)\n"
+                 "    BPT | FBP | WBP | SIRT,iterations,relax | SART,iterations,relax |\n"
+                 "    ADMM,iterations,cgiter,relax,soft\n"
                  "[--axis(-a) y|z]  volume orientation: y = TiltRec (X,thickness,Y), z = TiltRecZ (X,Y,thickness)\n"
                  "[--f2b(-f)]  byte output: (v - mean) / sd clamped to +-10, mapped to 0..255\n"
                  "[--rules mpi|cuda]  thresholds and y origin of TiltRec-mpi or TiltRec-cuda\n"
@@ -216,6 +216,7 @@ std::string opts_json(const Options &o)
       << esc(o.initial) << "\",\"tiltfile\":\"" << esc(o.angle) << "\",\"offset\":" << o.offset
       << ",\"pitch\":" << o.pitch << ",\"zshift\":" << o.zshift << ",\"thickness\":" << o.thickness
       << ",\"method\":\"" << esc(o.method) << "\",\"iterations\":" << o.iteration << ",\"gamma\":" << o.gamma
+      << ",\"cgiter\":" << o.cgiter << ",\"soft\":" << o.soft
       << ",\"axis\":\"" << esc(o.axis) << "\",\"f2b\":" << (o.f2b ? "true" : "false") << ",\"rules\":\""
       << esc(o.rules) << "\",\"storage\":\"" << esc(o.storage) << "\",\"device\":" << o.device << "}";
     return s.str();

@@ -457,6 +457,9 @@ trec_status trec_server_submit(trec_server *srv, const trec_request *req, uint64
     if (!req->stack || !req->volume_out) return TREC_ERR_INVALID;
     trec_status s = validate(nullptr, &req->geom, &req->method, req->angles_deg);
     if (s != TREC_OK) return s;
+    // the serving loop advances requests stage by stage; ADMM's nested CG
+    // runs as one sequence (trec_reconstruct)
+    if (req->method.method == TREC_ADMM) return TREC_ERR_UNSUPPORTED;
     auto r = std::make_unique<Request>();
     r->g = req->geom;
     r->m = req->method;

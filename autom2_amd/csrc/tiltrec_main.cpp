@@ -110,6 +110,7 @@ int main(int argc, char **argv)
     else if (o.method == "WBP") kind = TREC_WBP;
     else if (o.method == "SIRT") kind = TREC_SIRT;
     else if (o.method == "SART") kind = TREC_SART;
+    else if (o.method == "ADMM") kind = TREC_ADMM;
     else return die(3, "method %s is not provided", o.method.c_str());
 
     const double t0 = now_s();
@@ -138,7 +139,7 @@ int main(int argc, char **argv)
 
     trec_geom g{X, Y, P, T, o.offset, o.pitch, o.zshift, rules};
     const int storage = o.storage == "f16" ? TREC_STORE_F16 : (o.storage == "bf16" ? TREC_STORE_BF16 : TREC_STORE_F32);
-    trec_method m{kind, o.iteration, o.gamma, storage, layout};
+    trec_method m{kind, o.iteration, o.gamma, storage, layout, o.cgiter, o.soft};
     if (hipSetDevice(o.device) != hipSuccess) return die(5, "HIP device %s is not available", std::to_string(o.device).c_str());
     const size_t nstack = (size_t)P * Y * X, nvol = (size_t)X * Y * T;
     DevBuf dstack, dvol;

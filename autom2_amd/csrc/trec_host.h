@@ -38,6 +38,11 @@ struct JobPlan {
     std::vector<TiltGeneral> tg;
     TiltGeneral *d_tg = nullptr;
     float *stack_copy = nullptr;
+    // ADMM: vectors and scalars, and the descriptors of its passes
+    // (0: htb = A^T b; 1: op(vol) -> ax; 2: op(p) -> h)
+    AdmmVecs av{};
+    JobDesc *d_adm[3] = {nullptr, nullptr, nullptr};
+    JobDesc h_adm[3]{};
     size_t bytes = 0;
 };
 

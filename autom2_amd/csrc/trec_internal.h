@@ -97,6 +97,9 @@ struct JobDesc {
     // projection, as (c, c+1) pairs: entry c + 1 of quad q of slice (yb, z)
     // holds rows 4q .. 4q+3 of cells c and c+1, c in [-1, X); else NULL
     void *hvol;              // [nyb][T][4][X + 1] x 16 bytes
+    // BACK_LIN (ADMM): out = lin_a * sum_theta s_theta(res) + lin_b * vol
+    float *out;              // [nyb][T][X][RB]
+    float lin_a, lin_b;
 };
 
 // A multi-job launch: jobs[k] owns work-groups [off[k], off[k+1]).
@@ -145,7 +148,8 @@ unsigned back_blocks(const JobDesc &h);
 enum BackMode {
     BACK_SIRT = 0,  // val += s over all tilts, then vol += gamma * val / C
     BACK_BPT = 1,   // vol += s / c per tilt (c above the threshold)
-    BACK_SART = 2   // one tilt: vol += gamma * s / c
+    BACK_SART = 2,  // one tilt: vol += gamma * s / c
+    BACK_LIN = 3    // out = lin_a * sum s + lin_b * vol, float only (ADMM's operator)
 };
 
 // General-geometry kernels (natural layouts, any pitch).
@@ -202,6 +206,21 @@ struct XfLaunch {
 hipError_t launch_xf_terms(const XfLaunch &a, hipStream_t st);
 hipError_t launch_xf_mean(const XfLaunch &a, hipStream_t st);
 hipError_t launch_xf_warp(const XfLaunch &a, hipStream_t st);
+
+// ADMM (admm.hip): the volume-sized vectors of one reconstruction, all in
+// the row-blocked layout, and its device scalars.
+constexpr int ADMM_DOT_BLOCKS = 2048;
+enum AdmmStep { ADMM_HTB = 0, ADMM_R0, ADMM_STEP, ADMM_RES, ADMM_DIR, ADMM_SOFT };
+struct AdmmVecs {
+    float *vol, *htb, *u, *d, *r, *p, *h, *ax;
+    size_t n;      // floats per vector
+    double *part;  // [2][ADMM_DOT_BLOCKS]
+    double *S;     // r.r, h.p, alpha, r.h, beta
+    float mu, soft;
+};
+hipError_t launch_admm_vec(const AdmmVecs &a, int step, hipStream_t st);
+// which 0: r.r and h.p, then alpha; which 1: r.h, then beta
+hipError_t launch_admm_dots(const AdmmVecs &a, int which, hipStream_t st);
 
 // Small helpers.
 hipError_t launch_set_scale(JobDesc *d, const double *sumsq, double n, hipStream_t st);

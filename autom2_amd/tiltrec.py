@@ -43,7 +43,8 @@ class Geom(ctypes.Structure):
 
 class Method(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("iterations", ctypes.c_int), ("gamma", ctypes.c_float),
-                ("storage", ctypes.c_int), ("layout", ctypes.c_int)]
+                ("storage", ctypes.c_int), ("layout", ctypes.c_int), ("cg_iterations", ctypes.c_int),
+                ("soft", ctypes.c_float)]
 
 
 class Job(ctypes.Structure):
@@ -121,9 +122,11 @@ def parse_method(text):
     """--method value -> (kind, iterations, gamma), as opts.cpp:287-335 reads it.
 
     "SIRT,20,1.0" and "SART,5,0.5" carry an iteration count and a relaxation;
-    "BPT", "FBP" and "WBP" carry nothing; "ADMM,n,cg,gamma,soft" is parsed and
-    refused later (not provided).  Unlike the reference, a bare "SIRT" is an
-    error instead of reading uninitialised values (opts.cpp:112-124).
+    "BPT", "FBP" and "WBP" carry nothing; "ADMM,n,cg,gamma,soft" carries the
+    outer iterations, the CG steps, the penalty and the soft threshold (the
+    last two are returned by parse_admm).  Unlike the reference, a bare
+    "SIRT" is an error instead of reading uninitialised values
+    (opts.cpp:112-124).
     """
     parts = text.split(",")
     name = parts[0]
@@ -135,12 +138,18 @@ def parse_method(text):
             raise ValueError(f"{name} needs NAME,iterations,gamma")
         return kind, int(parts[1]), float(parts[2])
     if kind == ADMM:
-        if len(parts) != 5:
-            raise ValueError("ADMM needs ADMM,iterations,cgiter,gamma,soft")
-        return kind, int(parts[1]), float(parts[3])
+        return kind, parse_admm(text)[0], parse_admm(text)[2]
     if len(parts) != 1:
         raise ValueError(f"{name} takes no parameters")
     return kind, 0, 1.0
+
+
+def parse_admm(text):
+    """"ADMM,iterations,cgiter,gamma,soft" -> (iterations, cgiter, gamma, soft)."""
+    parts = text.split(",")
+    if parts[0] != "ADMM" or len(parts) != 5:
+        raise ValueError("ADMM needs ADMM,iterations,cgiter,gamma,soft")
+    return int(parts[1]), int(parts[2]), float(parts[3]), float(parts[4])
 
 
 def parse_geometry(text):
@@ -155,12 +164,15 @@ def make_geom(nx, ny, nproj, thickness, offset=0.0, pitch=0.0, zshift=0.0, rules
     return Geom(nx, ny, nproj, thickness, offset, pitch, zshift, rules)
 
 
-def make_method(kind, iterations=0, gamma=1.0, storage=STORE_F32, layout=LAYOUT_Y):
+def make_method(kind, iterations=0, gamma=1.0, storage=STORE_F32, layout=LAYOUT_Y, cg_iterations=0, soft=0.0):
     if isinstance(kind, str):
-        kind, iterations, gamma = parse_method(kind)
+        text = kind
+        kind, iterations, gamma = parse_method(text)
+        if kind == ADMM:
+            iterations, cg_iterations, gamma, soft = parse_admm(text)
     if isinstance(storage, str):
         storage = STORAGE_NAMES[storage]
-    return Method(kind, iterations, gamma, storage, layout)
+    return Method(kind, iterations, gamma, storage, layout, cg_iterations, soft)
 
 
 def _angles(angles):

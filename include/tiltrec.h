@@ -74,7 +74,7 @@ typedef enum {
     TREC_WBP = 2,  /* R-weighted back-projection */
     TREC_SIRT = 3,
     TREC_SART = 4,
-    TREC_ADMM = 5  /* not provided */
+    TREC_ADMM = 5  /* TiltRecZ's ADMM (pitch 0, |tilt| <= ~78 degrees) */
 } trec_method_kind;
 
 typedef enum {
@@ -111,13 +111,16 @@ typedef struct {
     int rules;       /* trec_rules */
 } trec_geom;
 
-/* --method NAME,iterations,gamma (opts.cpp:287-335) plus the forms. */
+/* --method NAME,iterations,gamma or ADMM,iterations,cgiter,gamma,soft
+ * (opts.cpp:287-335) plus the forms. */
 typedef struct {
     int method;      /* trec_method_kind */
-    int iterations;  /* SIRT/SART iteration count */
-    float gamma;     /* relaxation */
-    int storage;     /* trec_storage (SIRT/SART); WBP/BPT use float */
+    int iterations;  /* SIRT/SART iterations; ADMM outer iterations */
+    float gamma;     /* relaxation; ADMM's penalty mu */
+    int storage;     /* trec_storage (SIRT/SART); BPT/FBP/WBP/ADMM use float */
     int layout;      /* trec_layout of `volume` and `initial` */
+    int cg_iterations; /* ADMM: conjugate-gradient steps per outer iteration */
+    float soft;        /* ADMM: soft-threshold */
 } trec_method;
 
 typedef struct trec_context trec_context;

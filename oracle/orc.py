@@ -52,6 +52,8 @@ def lib():
         L.orc_optimal_dft_size.restype = _I
         L.orc_spatial_taps.argtypes = [_I, _I, _F]
         L.orc_fbp_filter_f64.argtypes = [_D, _I, _I, _I]
+        L.orc_admm_f32.argtypes = [_D, _I, _I, _I, _I, _F, _F, _I, _I, ctypes.c_float, ctypes.c_float, _F]
+        L.orc_admm_f64.argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, _I, ctypes.c_double, ctypes.c_double, _D]
         L.orc_bytes_from_stats.argtypes = [_F, ctypes.c_long, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
         L.orc_float_to_byte.argtypes = [_F, ctypes.c_long, ctypes.c_void_p, _F]
         L.orc_xf_table.argtypes = [_I, _F]
@@ -201,6 +203,27 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64)
     nb = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))
+
+
+def admm_f32(coef, proj, T, org, outer, cgiter, mu, soft, vol=None):
+    """TiltRecZ-mpi's ADMM (main-mpiz.cpp:603-1013), FP32 form."""
+    proj = np.ascontiguousarray(proj, dtype=np.float32)
+    P, X, rows, _ = _dims(proj, T)
+    out = np.zeros((rows, T, X), np.float32) if vol is None else np.array(vol, np.float32)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    lib().orc_admm_f32(_dp(c), P, X, rows, T, _org(org), _fp(proj), int(outer), int(cgiter), float(mu), float(soft),
+                       _fp(out))
+    return out
+
+
+def admm_f64(coef, proj, T, org, outer, cgiter, mu, soft, vol=None):
+    proj = np.ascontiguousarray(proj, dtype=np.float64)
+    P, X, rows, _ = _dims(proj, T)
+    out = np.zeros((rows, T, X), np.float64) if vol is None else np.array(vol, np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    lib().orc_admm_f64(_dp(c), P, X, rows, T, _org(org), _dp(proj), int(outer), int(cgiter), float(mu), float(soft),
+                       _dp(out))
+    return out
 
 
 def optimal_dft_size(n):

@@ -707,3 +707,235 @@ void orc_float_to_byte(const float *v, long n, unsigned char *out, float *stats)
     stats[0] = mean;
     stats[1] = sd;
 }
+
+/* ------------------------------------------------------------------ ADMM
+ * TiltRecZ-mpi's ADMM (main-mpiz.cpp:603-1013), in the y-major layout of
+ * this file (the reference's z-major loops visit the same terms):
+ *   htb  = sum_theta A_theta^T b_theta          (Reproject_admm_htb: tap
+ *          weight times pixel, no normalisation)
+ *   op(x) = sum_theta A_theta^T [(A_theta x) / w_theta] + mu x
+ *          (Reproject_admm_atax + ATAmuLTL; w_theta = A_theta 1, and a tap
+ *          contributes only where |w| > 10e-6)
+ *   outer loop: htb += (u - d) mu; x0 = op(vol); applycg; u = soft(vol + d);
+ *               d += vol - u
+ *   applycg: r = htb - x0, p = r; per iteration h = op(p),
+ *            alpha = (r.r) / (h.p), vol += alpha p, r = htb - op(vol),
+ *            beta = (r.h) / (h.p), p = r + beta p; alpha, beta = 0 when
+ *            |h.p| <= 10e-6; dot products of float products summed in
+ *            double, the updates evaluated in double.
+ * htb keeps its updates across outer iterations, as the reference's does. */
+
+static void admm_op_f32(const float org[3], const double *coef, int P, int W, int len, int T, const float *x,
+                        float mu, float *out, float *ax, float *w)
+{
+    size_t px = (size_t)len * W, size = px * T;
+    memset(out, 0, sizeof(float) * size);
+    for (int idx = 0; idx < P; idx++) {
+        const double *c = coef + 8 * (size_t)idx;
+        memset(ax, 0, sizeof(float) * px);
+        memset(w, 0, sizeof(float) * px);
+        reproject_f32(org, x, c, ax, w, T, len, W);
+        for (int y = 0; y < len; y++)
+            for (int z = 0; z < T; z++)
+                for (int xx = 0; xx < W; xx++) {
+                    orc_w t;
+                    valcoef(org, xx, y, z, c, &t);
+                    float *o = out + ((size_t)y * T + z) * W + xx;
+                    const int xs[4] = {t.xm, t.xm + 1, t.xm, t.xm + 1};
+                    const int ys[4] = {t.ym, t.ym, t.ym + 1, t.ym + 1};
+                    const float wx[4] = {1 - t.xd, t.xd, 1 - t.xd, t.xd};
+                    const float wy[4] = {1 - t.yd, 1 - t.yd, t.yd, t.yd};
+                    for (int k = 0; k < 4; k++) {
+                        if (xs[k] < 0 || xs[k] >= W || ys[k] < 0 || ys[k] >= len) continue;
+                        const size_t n = (size_t)xs[k] + (size_t)ys[k] * W;
+                        if (fabsf(w[n]) > 10e-6) *o += wx[k] * wy[k] * ax[n] / w[n];
+                    }
+                }
+    }
+    for (size_t i = 0; i < size; i++) out[i] += mu * x[i];
+}
+
+void orc_admm_f32(const double *coef, int P, int W, int len, int T, const float org[3], const float *proj,
+                  int outer, int cgiter, float mu, float soft, float *vol)
+{
+    size_t px = (size_t)len * W, size = px * T;
+    float *htb = (float *)calloc(size, sizeof(float));
+    float *u = (float *)calloc(size, sizeof(float));
+    float *d = (float *)calloc(size, sizeof(float));
+    float *x0 = (float *)malloc(sizeof(float) * size);
+    float *r = (float *)malloc(sizeof(float) * size);
+    float *p = (float *)malloc(sizeof(float) * size);
+    float *h = (float *)malloc(sizeof(float) * size);
+    float *ax = (float *)malloc(sizeof(float) * px);
+    float *w = (float *)malloc(sizeof(float) * px);
+    for (int idx = 0; idx < P; idx++) {
+        const double *c = coef + 8 * (size_t)idx;
+        const float *one = proj + px * idx;
+        for (int y = 0; y < len; y++)
+            for (int z = 0; z < T; z++)
+                for (int xx = 0; xx < W; xx++) {
+                    orc_w t;
+                    valcoef(org, xx, y, z, c, &t);
+                    float *o = htb + ((size_t)y * T + z) * W + xx;
+                    const int xs[4] = {t.xm, t.xm + 1, t.xm, t.xm + 1};
+                    const int ys[4] = {t.ym, t.ym, t.ym + 1, t.ym + 1};
+                    const float wx[4] = {1 - t.xd, t.xd, 1 - t.xd, t.xd};
+                    const float wy[4] = {1 - t.yd, 1 - t.yd, t.yd, t.yd};
+                    for (int k = 0; k < 4; k++) {
+                        if (xs[k] < 0 || xs[k] >= W || ys[k] < 0 || ys[k] >= len) continue;
+                        *o += wx[k] * wy[k] * one[(size_t)xs[k] + (size_t)ys[k] * W];
+                    }
+                }
+    }
+    for (int it = 0; it < outer; it++) {
+        for (size_t i = 0; i < size; i++) htb[i] += (u[i] - d[i]) * mu;
+        admm_op_f32(org, coef, P, W, len, T, vol, mu, x0, ax, w);
+        for (size_t i = 0; i < size; i++) {
+            r[i] = htb[i] - x0[i];
+            p[i] = r[i];
+        }
+        for (int cg = 0; cg < cgiter; cg++) {
+            admm_op_f32(org, coef, P, W, len, T, p, mu, h, ax, w);
+            double d0 = 0, d2 = 0, d1 = 0, alpha = 0, beta = 0;
+            for (size_t i = 0; i < size; i++) {
+                d0 += r[i] * r[i];
+                d2 += h[i] * p[i];
+            }
+            if (d2 > 10e-6 || -d2 > 10e-6) alpha = d0 / d2;
+            for (size_t i = 0; i < size; i++) vol[i] = vol[i] + alpha * p[i];
+            admm_op_f32(org, coef, P, W, len, T, vol, mu, x0, ax, w);
+            for (size_t i = 0; i < size; i++) {
+                r[i] = htb[i] - x0[i];
+                d1 += r[i] * h[i];
+            }
+            if (d2 > 10e-6 || -d2 > 10e-6) beta = d1 / d2;
+            for (size_t i = 0; i < size; i++) p[i] = r[i] + beta * p[i];
+        }
+        for (size_t i = 0; i < size; i++) {
+            if (vol[i] + d[i] < -soft)
+                u[i] = vol[i] + d[i] + soft;
+            else if (vol[i] + d[i] > soft)
+                u[i] = vol[i] + d[i] - soft;
+            else
+                u[i] = 0;
+        }
+        for (size_t i = 0; i < size; i++) d[i] += vol[i] - u[i];
+    }
+    free(htb);
+    free(u);
+    free(d);
+    free(x0);
+    free(r);
+    free(p);
+    free(h);
+    free(ax);
+    free(w);
+}
+
+/* The same in double throughout (weights before the float rounding). */
+static void admm_op_f64(const float org[3], const double *coef, int P, int W, int len, int T, const double *x,
+                        double mu, double *out, double *ax, double *w)
+{
+    size_t px = (size_t)len * W, size = px * T;
+    memset(out, 0, sizeof(double) * size);
+    for (int idx = 0; idx < P; idx++) {
+        const double *c = coef + 8 * (size_t)idx;
+        memset(ax, 0, sizeof(double) * px);
+        memset(w, 0, sizeof(double) * px);
+        reproject_f64(org, x, c, ax, w, T, len, W);
+        for (int y = 0; y < len; y++)
+            for (int z = 0; z < T; z++)
+                for (int xx = 0; xx < W; xx++) {
+                    orc_w t;
+                    valcoef(org, xx, y, z, c, &t);
+                    double *o = out + ((size_t)y * T + z) * W + xx;
+                    const int xs[4] = {t.xm, t.xm + 1, t.xm, t.xm + 1};
+                    const int ys[4] = {t.ym, t.ym, t.ym + 1, t.ym + 1};
+                    const double wx[4] = {1 - t.xdd, t.xdd, 1 - t.xdd, t.xdd};
+                    const double wy[4] = {1 - t.ydd, 1 - t.ydd, t.ydd, t.ydd};
+                    for (int k = 0; k < 4; k++) {
+                        if (xs[k] < 0 || xs[k] >= W || ys[k] < 0 || ys[k] >= len) continue;
+                        const size_t n = (size_t)xs[k] + (size_t)ys[k] * W;
+                        if (fabs(w[n]) > 10e-6) *o += wx[k] * wy[k] * ax[n] / w[n];
+                    }
+                }
+    }
+    for (size_t i = 0; i < size; i++) out[i] += mu * x[i];
+}
+
+void orc_admm_f64(const double *coef, int P, int W, int len, int T, const float org[3], const double *proj,
+                  int outer, int cgiter, double mu, double soft, double *vol)
+{
+    size_t px = (size_t)len * W, size = px * T;
+    double *htb = (double *)calloc(size, sizeof(double));
+    double *u = (double *)calloc(size, sizeof(double));
+    double *d = (double *)calloc(size, sizeof(double));
+    double *x0 = (double *)malloc(sizeof(double) * size);
+    double *r = (double *)malloc(sizeof(double) * size);
+    double *p = (double *)malloc(sizeof(double) * size);
+    double *h = (double *)malloc(sizeof(double) * size);
+    double *ax = (double *)malloc(sizeof(double) * px);
+    double *w = (double *)malloc(sizeof(double) * px);
+    for (int idx = 0; idx < P; idx++) {
+        const double *c = coef + 8 * (size_t)idx;
+        const double *one = proj + px * idx;
+        for (int y = 0; y < len; y++)
+            for (int z = 0; z < T; z++)
+                for (int xx = 0; xx < W; xx++) {
+                    orc_w t;
+                    valcoef(org, xx, y, z, c, &t);
+                    double *o = htb + ((size_t)y * T + z) * W + xx;
+                    const int xs[4] = {t.xm, t.xm + 1, t.xm, t.xm + 1};
+                    const int ys[4] = {t.ym, t.ym, t.ym + 1, t.ym + 1};
+                    const double wx[4] = {1 - t.xdd, t.xdd, 1 - t.xdd, t.xdd};
+                    const double wy[4] = {1 - t.ydd, 1 - t.ydd, t.ydd, t.ydd};
+                    for (int k = 0; k < 4; k++) {
+                        if (xs[k] < 0 || xs[k] >= W || ys[k] < 0 || ys[k] >= len) continue;
+                        *o += wx[k] * wy[k] * one[(size_t)xs[k] + (size_t)ys[k] * W];
+                    }
+                }
+    }
+    for (int it = 0; it < outer; it++) {
+        for (size_t i = 0; i < size; i++) htb[i] += (u[i] - d[i]) * mu;
+        admm_op_f64(org, coef, P, W, len, T, vol, mu, x0, ax, w);
+        for (size_t i = 0; i < size; i++) {
+            r[i] = htb[i] - x0[i];
+            p[i] = r[i];
+        }
+        for (int cg = 0; cg < cgiter; cg++) {
+            admm_op_f64(org, coef, P, W, len, T, p, mu, h, ax, w);
+            double d0 = 0, d2 = 0, d1 = 0, alpha = 0, beta = 0;
+            for (size_t i = 0; i < size; i++) {
+                d0 += r[i] * r[i];
+                d2 += h[i] * p[i];
+            }
+            if (d2 > 10e-6 || -d2 > 10e-6) alpha = d0 / d2;
+            for (size_t i = 0; i < size; i++) vol[i] = vol[i] + alpha * p[i];
+            admm_op_f64(org, coef, P, W, len, T, vol, mu, x0, ax, w);
+            for (size_t i = 0; i < size; i++) {
+                r[i] = htb[i] - x0[i];
+                d1 += r[i] * h[i];
+            }
+            if (d2 > 10e-6 || -d2 > 10e-6) beta = d1 / d2;
+            for (size_t i = 0; i < size; i++) p[i] = r[i] + beta * p[i];
+        }
+        for (size_t i = 0; i < size; i++) {
+            if (vol[i] + d[i] < -soft)
+                u[i] = vol[i] + d[i] + soft;
+            else if (vol[i] + d[i] > soft)
+                u[i] = vol[i] + d[i] - soft;
+            else
+                u[i] = 0;
+        }
+        for (size_t i = 0; i < size; i++) d[i] += vol[i] - u[i];
+    }
+    free(htb);
+    free(u);
+    free(d);
+    free(x0);
+    free(r);
+    free(p);
+    free(h);
+    free(ax);
+    free(w);
+}

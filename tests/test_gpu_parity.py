@@ -230,8 +230,8 @@ def test_bad_arguments_fail_loudly(torch_cuda, trec, ctx):
     assert ei.value.status == trec.ERR_INVALID
     g = trec.make_geom(4, 4, 3, 4)
     with pytest.raises(trec.TrecError) as ei:
-        ctx.reconstruct(g, [0, 1, 2], s, trec.make_method("ADMM,5,3,1.0,0.1"), out)
-    assert ei.value.status == trec.ERR_UNSUPPORTED
+        ctx.reconstruct(g, [0, 1, 2], s, trec.make_method("ADMM,5,3,1.0,-0.1"), out)
+    assert ei.value.status == trec.ERR_INVALID
 
 
 @pytest.mark.parametrize("method", ["SIRT,3,1.0", "SART,1,0.5"])

@@ -128,6 +128,11 @@ def test_cli_missing_input_fails(tmp_path):
     assert r.returncode == 4 and "cannot open" in r.stderr
 
 
-def test_cli_unsupported_methods(tmp_path):
-    r = run_cli(BASE + ["--method", "ADMM,5,3,1.0,0.1"])
-    assert r.returncode == 3
+def test_cli_parses_admm_and_fbp():
+    import json
+    r = run_cli(BASE + ["--method", "ADMM,5,3,1.0,0.1", "--dry-run"])
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert (d["method"], d["iterations"], d["cgiter"], d["gamma"], d["soft"]) == ("ADMM", 5, 3, 1.0, 0.1)
+    r = run_cli(BASE + ["--method", "FBP", "--f2b", "--dry-run"])
+    assert r.returncode == 0 and json.loads(r.stdout)["f2b"] is True

@@ -267,3 +267,82 @@ def test_rweight_constant_row_known_answer(orc):
         out = orc.rweight_f64(np.full((2, X), c))
         np.testing.assert_allclose(out, 0.2 * c * N / X, rtol=1e-12)
     np.testing.assert_allclose(orc.rweight_f64(np.full((1, 256), 1.0)), 0.21953125, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- ADMM
+
+def _matrix_admm(Ws, ps, outer, cgiter, mu, soft):
+    """TiltRecZ-mpi's ADMM (main-mpiz.cpp:603-1013) on explicit FP64 system
+    matrices: op(x) = sum W^T D W x + mu x with D = 1/(W 1) where |W 1| >
+    1e-5, htb = sum W^T b, CG with the reference's alpha/beta and true
+    residual, soft thresholding, the persistent htb update."""
+    n = Ws[0].shape[1]
+    D = []
+    for W in Ws:
+        w = W.sum(axis=1)
+        D.append(np.where(np.abs(w) > 10e-6, 1.0 / np.where(w == 0, 1, w), 0.0))
+
+    def op(x):
+        out = mu * x
+        for W, d in zip(Ws, D):
+            out = out + W.T @ (d * (W @ x))
+        return out
+
+    htb = sum(W.T @ p for W, p in zip(Ws, ps))
+    vol = np.zeros(n)
+    u = np.zeros(n)
+    dd = np.zeros(n)
+    for _ in range(outer):
+        htb = htb + (u - dd) * mu
+        r = htb - op(vol)
+        p = r.copy()
+        for _ in range(cgiter):
+            h = op(p)
+            d0, d2 = r @ r, h @ p
+            alpha = d0 / d2 if abs(d2) > 10e-6 else 0.0
+            vol = vol + alpha * p
+            r = htb - op(vol)
+            beta = (r @ h) / d2 if abs(d2) > 10e-6 else 0.0
+            p = r + beta * p
+        t = vol + dd
+        u = np.where(t < -soft, t + soft, np.where(t > soft, t - soft, 0.0))
+        dd = dd + vol - u
+    return vol
+
+
+@pytest.mark.parametrize("pitch", [0.0, 6.0])
+def test_admm_matches_matrix_form(orc, pitch):
+    rng = np.random.default_rng(11)
+    P, rows, T, X = 5, 4, 6, 9
+    c = orc.coeffs_mpi(tilt_set(P, 55.0), 0.0, pitch, 0.0)
+    org = orc.origin(X, T, T / 2)
+    proj = rng.random((P, rows, X))
+    Ws = [sysmat(c[k], X, rows, T, org) for k in range(P)]
+    ps = [proj[k].ravel() for k in range(P)]
+    ref = _matrix_admm(Ws, ps, 3, 4, 0.5, 0.02)
+    got = orc.admm_f64(c, proj, T, org, 3, 4, 0.5, 0.02)
+    np.testing.assert_allclose(got.ravel(), ref, rtol=1e-9, atol=1e-11)
+    g32 = orc.admm_f32(c, proj.astype(np.float32), T, org, 3, 4, 0.5, 0.02)
+    assert orc.rel_l2(g32, ref.reshape(g32.shape)) < 1e-5
+
+
+def test_admm_soft_zero_keeps_d_zero_and_one_outer_is_cg(orc):
+    """With soft = 0, u = vol + d and d stays 0.  One outer iteration is
+    then cgiter steps of the reference's CG on (sum W^T D W + mu I) v = htb
+    from v = 0; with cgiter = 1 this is one steepest-descent step,
+    v = (r.r / r.Ar) r with r = htb."""
+    rng = np.random.default_rng(12)
+    P, rows, T, X = 4, 3, 5, 8
+    c = orc.coeffs_mpi(tilt_set(P, 40.0))
+    org = orc.origin(X, T, T / 2)
+    proj = rng.random((P, rows, X))
+    Ws = [sysmat(c[k], X, rows, T, org) for k in range(P)]
+    mu = 0.3
+    htb = sum(W.T @ proj[k].ravel() for k, W in enumerate(Ws))
+    A = mu * np.eye(htb.size)
+    for W in Ws:
+        w = W.sum(axis=1)
+        A = A + W.T @ np.diag(np.where(np.abs(w) > 1e-5, 1 / np.where(w == 0, 1, w), 0)) @ W
+    step = (htb @ htb) / (htb @ A @ htb) * htb
+    got = orc.admm_f64(c, proj, T, org, 1, 1, mu, 0.0)
+    np.testing.assert_allclose(got.ravel(), step, rtol=1e-10, atol=1e-12)
