@@ -58,6 +58,32 @@ def time_form(trec, ctx, torch, geom, ang, stack_d, storage, iters, steps, warmu
     return [ev[k].elapsed_time(ev[k + 1]) for k in range(steps)], vol
 
 
+def methods_form(trec, ctx, torch, geom, ang, stack_d, reps=3):
+    """Milliseconds per call of the one-pass methods and of ADMM on the same
+    series (stack resident, volume in layout Y)."""
+    vol = torch.empty((geom.ny, geom.thickness, geom.nx), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    res = {}
+    for text in ("BPT", "WBP", "FBP", "ADMM,2,5,0.5,0.001"):
+        m = trec.make_method(text)
+        ctx.reconstruct(geom, ang, stack_d, m, vol, stream=stream)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(stream)
+        for k in range(reps):
+            ctx.reconstruct(geom, ang, stack_d, m, vol, stream=stream)
+            ev[k + 1].record(stream)
+        torch.cuda.synchronize()
+        ms = sum(ev[k].elapsed_time(ev[k + 1]) for k in range(reps)) / reps
+        res[text] = {"ms_per_call": round(ms, 3)}
+        if m.method == trec.ADMM:
+            # per outer iteration the operator is applied to the start and
+            # twice per CG step (the direction, then the true residual)
+            res[text]["operator_applications"] = m.iterations * (2 * m.cg_iterations + 1)
+        log(f"[bench] {text}: {ms:.2f} ms/call")
+    return res
+
+
 def cpu_baseline(ang, stack, T, iters, rows):
     """The CPU restatement of TiltRec-mpi's SIRT (oracle, FP32, one thread)
     on `rows` detector rows of the same series: rows are independent at
@@ -348,6 +374,7 @@ def main():
             log(f"[bench] {name}: {fm:.2f} ms/step")
 
     if not args.no_forms:
+        forms["methods"] = methods_form(trec, ctx, torch, geom, ang, stack_d)
         del vol
         torch.cuda.empty_cache()
         forms["serving"] = serving_form(trec, torch, args.serve_requests, 2.0)
