@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -49,8 +50,21 @@ std::string tune_key(const trec_geom &g, int storage)
 namespace trec {
 
 // Event timing of launch groups (trec_profile_*).
-enum ProfKind { PROF_PREP = 0, PROF_FORWARD, PROF_COMBINE, PROF_BACK, PROF_FINISH, PROF_XF_MEAN, PROF_XF_WARP, PROF_N };
-const char *const prof_names[PROF_N] = {"prep", "forward", "combine", "back", "finish", "xf_mean", "xf_warp"};
+enum ProfKind {
+    PROF_PREP = 0,
+    PROF_FORWARD,
+    PROF_COMBINE,
+    PROF_BACK,
+    PROF_FINISH,
+    PROF_XF_MEAN,
+    PROF_XF_WARP,
+    PROF_GP_PROFILE,
+    PROF_GP_SEARCH,
+    PROF_GP_VIEWS,
+    PROF_N
+};
+const char *const prof_names[PROF_N] = {"prep",    "forward", "combine",    "back",      "finish",
+                                        "xf_mean", "xf_warp", "gp_profile", "gp_search", "gp_views"};
 
 struct Prof {
     bool on = false;
@@ -1272,6 +1286,91 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
     }
     tl_prof = nullptr;
     HIPCHK(ctx, e);
+    return TREC_OK;
+}
+
+trec_status trec_geopar(trec_context *ctx, const float *volume, int nx, int ny, int nz, int axis, int flags,
+                        float *views, float *chips, float *result_host, int *exhaustive_host, void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    if (!volume) return fail(ctx, TREC_ERR_INVALID, "null volume");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(ctx, TREC_ERR_INVALID, "bad volume size %d x %d x %d", nx, ny, nz);
+    if (axis != TREC_GP_VIEW_X && axis != TREC_GP_VIEW_Y) return fail(ctx, TREC_ERR_INVALID, "bad view axis %d", axis);
+    if (ny > GP_MAXN) return fail(ctx, TREC_ERR_INVALID, "%d rows per view exceed %d", ny, GP_MAXN);
+    if (axis == TREC_GP_VIEW_Y && nz > 65535)
+        return fail(ctx, TREC_ERR_INVALID, "%d sections exceed 65535 for y views", nz);
+    const int nviews = axis == TREC_GP_VIEW_X ? nx : nz, cols = axis == TREC_GP_VIEW_X ? nz : nx;
+    const int nchips = cols > GP_WIDTH ? (cols - GP_WIDTH - 1) / GP_STEP + 1 : 0;
+    if ((size_t)nviews * nchips > (size_t)INT_MAX)
+        return fail(ctx, TREC_ERR_INVALID, "%d views of %d chips are too many", nviews, nchips);
+    // partial min/max: about 4096 work-groups over the volume
+    int splits;
+    if (axis == TREC_GP_VIEW_X) {
+        const size_t tiles = (size_t)(nx + WG - 1) / WG;
+        splits = (int)std::max<size_t>(1, std::min<size_t>({4096 / tiles, 1024, (size_t)nz * ny}));
+    } else {
+        splits = (int)std::max<size_t>(1, std::min<size_t>({4096 / (size_t)nz, 1024, ((size_t)ny * nx + WG - 1) / WG}));
+    }
+    const size_t b_part = align_up(sizeof(float) * splits * nviews), b_ab = align_up(sizeof(float2) * nviews);
+    const size_t b_prof = align_up(sizeof(float) * (size_t)nviews * nchips * ny);
+    const size_t b_chips = chips ? 0 : align_up(sizeof(float4) * (size_t)nviews * std::max(nchips, 1));
+    const size_t b_views = views ? 0 : align_up(sizeof(float) * 3 * (size_t)nviews);
+    const size_t b_small = align_up(sizeof(float) * 4 + sizeof(int) * 4);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    trec_status s = ensure_arena(ctx, 2 * b_part + b_ab + b_prof + b_chips + b_views + b_small);
+    if (s != TREC_OK) return s;
+    char *base = ctx->arena;
+    GpLaunch a{};
+    a.vol = volume;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.axis = axis;
+    a.nviews = nviews;
+    a.cols = cols;
+    a.nchips = nchips;
+    a.exhaustive = (flags & TREC_GP_EXHAUSTIVE) ? 1 : 0;
+    a.splits = splits;
+    a.pmin = (float *)base;
+    a.pmax = (float *)(base + b_part);
+    a.ab = (float2 *)(base + 2 * b_part);
+    a.prof = (float *)(base + 2 * b_part + b_ab);
+    char *p = base + 2 * b_part + b_ab + b_prof;
+    a.chips = chips ? (float4 *)chips : (float4 *)p;
+    p += b_chips;
+    a.views = views ? views : (float *)p;
+    p += b_views;
+    a.avg = (float *)p;
+    a.flags = (int *)(p + sizeof(float) * 4);
+    hipStream_t st = (hipStream_t)stream;
+    tl_prof = &ctx->prof;
+    hipError_t e = launch_zero((float *)a.flags, 4, st);
+    if (e == hipSuccess) {
+        ProfScope ps(PROF_GP_PROFILE, st);
+        e = launch_gp_scale(a, st);
+        if (e == hipSuccess) e = launch_gp_profiles(a, st);
+    }
+    if (e == hipSuccess) {
+        ProfScope ps(PROF_GP_SEARCH, st);
+        e = launch_gp_search(a, st);
+    }
+    if (e == hipSuccess) {
+        ProfScope ps(PROF_GP_VIEWS, st);
+        e = launch_gp_views(a, st);
+    }
+    tl_prof = nullptr;
+    HIPCHK(ctx, e);
+    if (result_host || exhaustive_host) {
+        float hb[4];
+        int hf[4];
+        HIPCHK(ctx, hipMemcpyAsync(hb, a.avg, sizeof hb, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(hf, a.flags, sizeof hf, hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        if (hf[0]) return fail(ctx, TREC_ERR_INVALID, "the volume holds non-finite values");
+        if (result_host) memcpy(result_host, hb, 3 * sizeof(float));
+        if (exhaustive_host) *exhaustive_host = hf[1];
+    }
     return TREC_OK;
 }
 

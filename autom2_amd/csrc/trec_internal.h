@@ -222,6 +222,31 @@ hipError_t launch_admm_vec(const AdmmVecs &a, int step, hipStream_t st);
 // which 0: r.r and h.p, then alpha; which 1: r.h, then beta
 hipError_t launch_admm_dots(const AdmmVecs &a, int which, hipStream_t st);
 
+// GeoParGen (geopar.hip): views of a volume, chips of GP_WIDTH columns every
+// GP_STEP, a step-function search of at least GP_MINT + 1 rows per chip.
+constexpr int GP_STEP = 50;
+constexpr int GP_WIDTH = 50;
+constexpr int GP_MINT = 30;
+constexpr int GP_MAXN = 4096;  // profile rows the search keeps in LDS
+constexpr int GP_WCAP = 256;   // window members checked exactly per chip
+constexpr int GP_SPLITS = 64;  // partial min/max per view
+struct GpLaunch {
+    const float *vol;  // [nz][ny][nx]
+    int nx, ny, nz, axis, nviews, cols, nchips, exhaustive;
+    float *pmin, *pmax;  // [GP_SPLITS][nviews]
+    int splits;
+    float2 *ab;          // per view: x * a + b normalises to [0, 1]
+    float *prof;         // [nviews][nchips][ny]
+    float4 *chips;       // [nviews][nchips]: z, thickness, ncc, l
+    float *views;        // [nviews][3]: angle, thickness, z
+    float *avg;          // [3]
+    int *flags;          // [0]: non-finite voxels seen; [1]: chips searched exhaustively
+};
+hipError_t launch_gp_scale(const GpLaunch &a, hipStream_t st);
+hipError_t launch_gp_profiles(const GpLaunch &a, hipStream_t st);
+hipError_t launch_gp_search(const GpLaunch &a, hipStream_t st);
+hipError_t launch_gp_views(const GpLaunch &a, hipStream_t st);
+
 // Small helpers.
 hipError_t launch_set_scale(JobDesc *d, const double *sumsq, double n, hipStream_t st);
 hipError_t launch_unblock_proj(const float *src, int P, int X, int Y, int nyb, float *dst, hipStream_t st);

@@ -93,9 +93,12 @@ SIGNATURES = [
     ("trec_request_footprint", _I, [ctypes.POINTER(Geom), ctypes.POINTER(Method), ctypes.POINTER(ctypes.c_size_t)]),
     ("trec_server_destroy", None, [_P]),
     ("trec_apply_xf", _I, [_P, _I, _I, _I, _PF, _I, _P, _P, _P]),
+    ("trec_geopar", _I, [_P, _P, _I, _I, _I, _I, _I, _P, _P, _PF, ctypes.POINTER(_I), _P]),
 ]
 
 XF_CUBIC, XF_LANCZOS4 = 1, 2  # mrcstack -m: 2 selects Lanczos-4, anything else cubic
+GP_VIEW_X, GP_VIEW_Y = 0, 1    # GeoParGen's "x" (default) and "y" views
+GP_EXHAUSTIVE = 1              # trec_geopar flag: exact NCC of every pair
 
 _lib = None
 
@@ -332,6 +335,19 @@ class Context:
         x = np.ascontiguousarray(np.asarray(xf, np.float32).reshape(nsec, 6))
         _check(lib().trec_apply_xf(self._h, nx, ny, nsec, x.ctypes.data_as(_PF), int(mode), _ptr(stack),
                                    _ptr(out), _stream(stream)), self._h)
+
+    def geopar(self, volume, axis=GP_VIEW_X, views=None, chips=None, exhaustive=False, stream=None):
+        """GeoParGen (trec_geopar) on a [nz][ny][nx] float32 device volume.
+        views ([nviews][3]) and chips ([nviews][nchips][4]) are optional
+        float32 device tensors.  Waits for the stream; returns (averages
+        angle, thickness, z as floats, chips searched exhaustively)."""
+        nz, ny, nx = volume.shape
+        res = (ctypes.c_float * 3)()
+        nex = _I()
+        _check(lib().trec_geopar(self._h, _ptr(volume), nx, ny, nz, int(axis), GP_EXHAUSTIVE if exhaustive else 0,
+                                 _ptr(views), _ptr(chips), ctypes.cast(res, _PF), ctypes.byref(nex),
+                                 _stream(stream)), self._h)
+        return (res[0], res[1], res[2]), nex.value
 
 
 class Server:

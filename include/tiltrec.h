@@ -32,6 +32,9 @@
  *                                                        trec_server_*
  *   mrcstack's per-section cv::warpAffine loop          trec_apply_xf
  *     (src/mrctools/src/mrcstack/main.cpp:40-80)
+ *   GeoParGen's per-view ViewSlicer loop and averages   trec_geopar
+ *     (src/GeoParGen/src/main/main.cpp:71-215,
+ *      main/mrcview.h:43-147, mrcview.cpp:13-127)
  *
  * Conventions
  *   - Every array argument is a device pointer (hipMalloc'd memory on the
@@ -218,6 +221,38 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec,
                           const float *xf_host, int mode, const float *in,
                           float *out, void *stream);
 
+/* GeoParGen (src/GeoParGen/src/main/main.cpp:71-215; ViewSlicer,
+ * mrcview.h:43-147; mrcview.cpp:13-127): the tilt, thickness and centre of
+ * the specimen slab in a volume of nz sections of ny x nx floats (an MRC
+ * volume in file order; TiltRec-* output is (X, thickness, Y)).
+ *   axis TREC_GP_VIEW_X (GeoParGen's default "x"): nx views, view j the
+ *   ny x nz plane at column j; TREC_GP_VIEW_Y ("y"): the nz sections.
+ * Each view is normalised to [0, 1], cut into chips of 50 columns every 50,
+ * and each chip's row profile is matched against every rectangle that is
+ * zero on rows [l, r], r >= l + 30, by |NCC|, with the reference's float
+ * running maximum deciding near-ties exactly as its scan does.  The view's
+ * angle is the atan, in degrees, of the slope of z = (l + r) / 2 over the
+ * chip positions, weighted by (ncc / sum ncc)^2; the same weights average
+ * thickness r - l and z.
+ * Outputs (all optional):
+ *   views  device [nviews][3] floats: angle, thickness, z per view;
+ *   chips  device [nviews][nchips][4] floats: z, thickness, ncc, l per
+ *          chip (-1, -1, 0, -1 when nothing was found);
+ *   result_host  host [3]: the printed averages (float sums over views
+ *          divided by the view count);
+ *   exhaustive_host  host: chips whose search needed the full exact scan.
+ * With result_host or exhaustive_host the call waits for the stream and
+ * returns TREC_ERR_INVALID if the volume held non-finite values.
+ * flags: TREC_GP_EXHAUSTIVE evaluates every pair of every chip exactly (the
+ * reference's own cost; the default gives the same results).
+ * Limits: ny <= 4096; y views need nz <= 65535. */
+enum { TREC_GP_VIEW_X = 0, TREC_GP_VIEW_Y = 1 };
+enum { TREC_GP_EXHAUSTIVE = 1 };
+trec_status trec_geopar(trec_context *ctx, const float *volume, int nx,
+                        int ny, int nz, int axis, int flags, float *views,
+                        float *chips, float *result_host, int *exhaustive_host,
+                        void *stream);
+
 /* One job of a batch: a tilt series and where its volume goes. */
 typedef struct {
     trec_geom geom;
@@ -245,8 +280,10 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom,
  * reconstruction calls is bracketed by HIP events on the stream it runs on
  * (HIP graphs are not used meanwhile).  kernel is one of "prep" (layout
  * transposes, filter, weights), "forward", "combine", "back", "finish",
- * "xf_mean" (trec_apply_xf's coordinate terms and section means) and
- * "xf_warp" (its warp);
+ * "xf_mean" (trec_apply_xf's coordinate terms and section means),
+ * "xf_warp" (its warp), "gp_profile" (trec_geopar's normalisation and chip
+ * profiles), "gp_search" (its pair search) and "gp_views" (its
+ * regressions and averages);
  * trec_profile_read waits for the recorded events and returns the summed
  * milliseconds and the number of launch groups since enabling. */
 trec_status trec_profile_enable(trec_context *ctx, int on);

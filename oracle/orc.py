@@ -63,6 +63,20 @@ def lib():
         L.orc_xf_coord.argtypes = [_D, _I, _I, ctypes.POINTER(ctypes.c_int)]
         L.orc_warp_affine.argtypes = [_F, _I, _I, _D, _I, ctypes.c_float, _F]
         L.orc_apply_xf.argtypes = [_F, _I, _I, _I, _F, _I, _F, _F]
+        L.orc_gp_nchips.argtypes = [_I]
+        L.orc_gp_nchips.restype = _I
+        L.orc_gp_npairs.argtypes = [_I]
+        L.orc_gp_npairs.restype = ctypes.c_longlong
+        L.orc_gp_view.argtypes = [_F, _I, _I, _I, _I, _I, _F]
+        L.orc_gp_normalize.argtypes = [_F, ctypes.c_longlong, _F]
+        L.orc_gp_profiles.argtypes = [_F, _I, _I, _F]
+        L.orc_gp_ncc.argtypes = [_F, _I, _I, _I]
+        L.orc_gp_ncc.restype = ctypes.c_double
+        _PI = ctypes.POINTER(ctypes.c_int)
+        L.orc_gp_search.argtypes = [_F, _I, _PI, _PI, _PI, _PI]
+        L.orc_gp_search.restype = ctypes.c_float
+        L.orc_gp_view_result.argtypes = [_PI, _PI, _F, _I, _F]
+        L.orc_geopar.argtypes = [_F, _I, _I, _I, _I, _F, _F, _F]
         _lib = L
     return _lib
 
@@ -319,3 +333,77 @@ def apply_xf(stack, xf, mode=1):
     cv = np.empty(P, np.float32)
     lib().orc_apply_xf(_fp(s), W, H, P, _fp(x), int(mode), _fp(out), _fp(cv))
     return out, cv
+
+
+# ---------------------------------------------------------------- GeoParGen
+
+def gp_nchips(w):
+    return lib().orc_gp_nchips(int(w))
+
+
+def gp_npairs(n):
+    return lib().orc_gp_npairs(int(n))
+
+
+def gp_view(vol, axis, v):
+    """View v of a [nz][ny][nx] volume: axis 0 ("x") -> [ny][nz], axis 1 -> [ny][nx]."""
+    a = np.ascontiguousarray(vol, np.float32)
+    nz, ny, nx = a.shape
+    out = np.empty((ny, nz) if axis == 0 else (ny, nx), np.float32)
+    lib().orc_gp_view(_fp(a), nx, ny, nz, int(axis), int(v), _fp(out))
+    return out
+
+
+def gp_normalize(img):
+    a = np.ascontiguousarray(img, np.float32)
+    out = np.empty_like(a)
+    lib().orc_gp_normalize(_fp(a), a.size, _fp(out))
+    return out
+
+
+def gp_profiles(norm):
+    """[rows][cols] normalised view -> [nchips][rows] chip profiles."""
+    a = np.ascontiguousarray(norm, np.float32)
+    rows, cols = a.shape
+    out = np.empty((gp_nchips(cols), rows), np.float32)
+    lib().orc_gp_profiles(_fp(a), rows, cols, _fp(out))
+    return out
+
+
+def gp_ncc(prof, l, r):
+    a = np.ascontiguousarray(prof, np.float32)
+    return lib().orc_gp_ncc(_fp(a), a.size, int(l), int(r))
+
+
+def gp_search(prof):
+    """-> (z, thickness, ncc float, l, r) as getThicknessAndZ(prof, 30, 60)."""
+    a = np.ascontiguousarray(prof, np.float32)
+    z, t, lo, hi = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    nc = lib().orc_gp_search(_fp(a), a.size, ctypes.byref(z), ctypes.byref(t), ctypes.byref(lo),
+                             ctypes.byref(hi))
+    return z.value, t.value, np.float32(nc), lo.value, hi.value
+
+
+def gp_view_result(z, thick, nc):
+    zz = np.ascontiguousarray(z, np.int32)
+    tt = np.ascontiguousarray(thick, np.int32)
+    cc = np.ascontiguousarray(nc, np.float32)
+    out = np.empty(3, np.float32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib().orc_gp_view_result(zz.ctypes.data_as(ip), tt.ctypes.data_as(ip), _fp(cc), len(cc), _fp(out))
+    return out
+
+
+def geopar(vol, axis=0):
+    """GeoParGen on a [nz][ny][nx] float volume.  Returns (views [nviews][3]
+    = angle, thickness, z; chips [nviews][nchips][4] = z, thickness, ncc, l;
+    averages [3])."""
+    a = np.ascontiguousarray(vol, np.float32)
+    nz, ny, nx = a.shape
+    nv, cols = (nx, nz) if axis == 0 else (nz, nx)
+    nc = gp_nchips(cols)
+    views = np.empty((nv, 3), np.float32)
+    chips = np.empty((nv, nc, 4), np.float32)
+    avg = np.empty(3, np.float32)
+    lib().orc_geopar(_fp(a), nx, ny, nz, int(axis), _fp(views), _fp(chips), _fp(avg))
+    return views, chips, avg
