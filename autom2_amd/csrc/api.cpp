@@ -1304,13 +1304,13 @@ trec_status trec_geopar(trec_context *ctx, const float *volume, int nx, int ny, 
     const int nchips = cols > GP_WIDTH ? (cols - GP_WIDTH - 1) / GP_STEP + 1 : 0;
     if ((size_t)nviews * nchips > (size_t)INT_MAX)
         return fail(ctx, TREC_ERR_INVALID, "%d views of %d chips are too many", nviews, nchips);
-    // partial min/max: about 4096 work-groups over the volume
+    // partial min/max: about 2048 work-groups over the volume
     int splits;
     if (axis == TREC_GP_VIEW_X) {
         const size_t tiles = (size_t)(nx + WG - 1) / WG;
-        splits = (int)std::max<size_t>(1, std::min<size_t>({4096 / tiles, 1024, (size_t)nz * ny}));
+        splits = (int)std::max<size_t>(1, std::min<size_t>({2048 / tiles, 512, (size_t)nz * ny}));
     } else {
-        splits = (int)std::max<size_t>(1, std::min<size_t>({4096 / (size_t)nz, 1024, ((size_t)ny * nx + WG - 1) / WG}));
+        splits = (int)std::max<size_t>(1, std::min<size_t>({2048 / (size_t)nz, 512, ((size_t)ny * nx + WG - 1) / WG}));
     }
     const size_t b_part = align_up(sizeof(float) * splits * nviews), b_ab = align_up(sizeof(float2) * nviews);
     const size_t b_prof = align_up(sizeof(float) * (size_t)nviews * nchips * ny);

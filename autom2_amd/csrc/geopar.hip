@@ -95,12 +95,26 @@ __global__ __launch_bounds__(WG) void k_gp_minmax_y(GpLaunch a)
 __global__ __launch_bounds__(WG) void k_gp_scale(GpLaunch a)
 {
 #pragma clang fp contract(off)
-    const int v = blockIdx.x * WG + threadIdx.x;
-    if (v >= a.nviews) return;
+    // 64 views per work-group (lanes), the partials split over the 4 waves,
+    // so each load of a wave reads 64 neighbouring views of one partial
+    __shared__ float red[2][WG / 64][64];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int v = blockIdx.x * 64 + lane;
     float lo = INFINITY, hi = -INFINITY;
-    for (int s = 0; s < a.splits; ++s) {
-        lo = fminf(lo, a.pmin[(size_t)s * a.nviews + v]);
-        hi = fmaxf(hi, a.pmax[(size_t)s * a.nviews + v]);
+    if (v < a.nviews) {
+#pragma unroll 8
+        for (int s = g; s < a.splits; s += WG / 64) {
+            lo = fminf(lo, a.pmin[(size_t)s * a.nviews + v]);
+            hi = fmaxf(hi, a.pmax[(size_t)s * a.nviews + v]);
+        }
+    }
+    red[0][g][lane] = lo;
+    red[1][g][lane] = hi;
+    __syncthreads();
+    if (g != 0 || v >= a.nviews) return;
+    for (int w = 1; w < WG / 64; ++w) {
+        lo = fminf(lo, red[0][w][lane]);
+        hi = fmaxf(hi, red[1][w][lane]);
     }
     const double smin = lo, smax = hi;
     double scale = 1.0 * (smax - smin > DBL_EPSILON ? 1. / (smax - smin) : 0);
@@ -199,6 +213,23 @@ __device__ void gp_pair(long long t, int m, int &l, int &r)
     r = L + GP_MINT + (int)(t - start(L));
 }
 
+// The window threshold for the largest approximate value mx, rounded down
+// to a float: ((mx - 2E)(1 - 2^-20)(1 - 2 er) - 2E) with er = 2^-18 bounding
+// the float tail of the approximation (conversions, the hardware reciprocal
+// square root, the product).  Comparing floats against a value rounded down
+// can only keep more pairs.
+__device__ float gp_theta_down(float mx, double E)
+{
+    const double er = 0x1p-18;
+    const double t = ((double)mx - 2.0 * E) * (1.0 - 0x1p-20) * (1.0 - 2.0 * er) - 2.0 * E;
+    float f = (float)t;
+    if ((double)f > t) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+// Entries past P[n]: lanes beyond the pair triangle read them.
+constexpr int GP_PAD = 64;
+
 __device__ int wg_max_int(int v, int *red)
 {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
@@ -224,10 +255,11 @@ __device__ float wg_max_float(float v, float *red)
     return m;
 }
 
-// One work-group per (view, chip).  Dynamic LDS: p1[n], P[n + 1] (prefix
-// sums of p1), A1[n + 1] and rs[n + 1] (per count k of zero rows: 1 - mean
-// of the rectangle and 1 / sqrt(sum1 * sum2)), the window list and its
-// exact values, and a scan buffer.
+// One work-group per (view, chip).  Dynamic LDS: p1[n], P[n + 1 + GP_PAD]
+// (prefix sums of p1, padded with the total), CK[n + 1] and rs[n + 1] (per count k of zero rows: (1 -
+// mean of the rectangle) * sum p1 and 1 / sqrt(sum1 * sum2)), the window
+// list and its exact values, and a scan buffer.  The approximation of pair
+// (l, r), k = r - l + 1, is |CK[k] - (P[r + 1] - P[l])| * rs[k].
 __global__ __launch_bounds__(WG) void k_gp_search(GpLaunch a)
 {
 #pragma clang fp contract(off)
@@ -235,8 +267,9 @@ __global__ __launch_bounds__(WG) void k_gp_search(GpLaunch a)
     __shared__ double sh_amean, sh_sum1, sh_ptot, sh_best;
     __shared__ int sh_cnt, ired[WG / 64];
     __shared__ float fred[WG / 64];
+    __shared__ float bmax[WG / 64][GP_MAXN / 64];  // per wave and block of 64 l
     const int n = a.ny, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double *p1 = gsm, *P = gsm + n, *A1 = P + (n + 1), *resw = A1 + (n + 1), *scan = resw + GP_WCAP;
+    double *p1 = gsm, *P = gsm + n, *CK = P + (n + 1 + GP_PAD), *resw = CK + (n + 1), *scan = resw + GP_WCAP;
     float *rs = (float *)(scan + WG);
     int *list = (int *)(rs + (n + 1));
     const float *prof = a.prof + (size_t)blockIdx.x * n;
@@ -245,10 +278,22 @@ __global__ __launch_bounds__(WG) void k_gp_search(GpLaunch a)
     for (int i = tid; i < n; i += WG) p1[i] = (double)prof[i];
     __syncthreads();
     const double invn = 1. / n;
+    // The sums below are sequential in row order (the reference's loops).
+    // Thread 0 reads GP_CH values at a time into registers so that the LDS
+    // loads of a chunk overlap each other instead of each waiting behind
+    // the previous step's store.
+    constexpr int GP_CH = 16;
     if (tid == 0) {
         // cv::mean: the double sum times 1 / n
         double s = 0.0;
-        for (int i = 0; i < n; ++i) s = s + p1[i];
+        for (int i0 = 0; i0 < n; i0 += GP_CH) {
+            double x[GP_CH];
+#pragma unroll
+            for (int k = 0; k < GP_CH; ++k) x[k] = i0 + k < n ? p1[i0 + k] : 0.0;
+#pragma unroll
+            for (int k = 0; k < GP_CH; ++k)
+                if (i0 + k < n) s = s + x[k];
+        }
         sh_amean = s * invn;
     }
     __syncthreads();
@@ -258,11 +303,23 @@ __global__ __launch_bounds__(WG) void k_gp_search(GpLaunch a)
     if (tid == 0) {
         double s1 = 0.0, c = 0.0;
         P[0] = 0.0;
-        for (int i = 0; i < n; ++i) {
-            s1 = s1 + p1[i] * p1[i];
-            c = c + p1[i];
-            P[i + 1] = c;
+        for (int i0 = 0; i0 < n; i0 += GP_CH) {
+            double x[GP_CH], pc[GP_CH];
+#pragma unroll
+            for (int k = 0; k < GP_CH; ++k) x[k] = i0 + k < n ? p1[i0 + k] : 0.0;
+#pragma unroll
+            for (int k = 0; k < GP_CH; ++k) {
+                if (i0 + k < n) {
+                    s1 = s1 + x[k] * x[k];
+                    c = c + x[k];
+                }
+                pc[k] = c;
+            }
+#pragma unroll
+            for (int k = 0; k < GP_CH; ++k)
+                if (i0 + k < n) P[i0 + k + 1] = pc[k];
         }
+        for (int i = n + 1; i < n + 1 + GP_PAD; ++i) P[i] = c;
         sh_sum1 = s1;
         sh_ptot = c;
         sh_cnt = 0;
@@ -281,40 +338,57 @@ __global__ __launch_bounds__(WG) void k_gp_search(GpLaunch a)
     if (!full) {
         for (int k = tid; k <= n; k += WG) {
             const double bm = (double)(n - k) * invn, one = 1.0 - bm;
-            A1[k] = one;
+            CK[k] = one * ptot;
             const double s2 = (double)(n - k) * (one * one) + (double)k * (bm * bm);
             rs[k] = __builtin_amdgcn_rsqf((float)(sum1 * s2));
         }
         __syncthreads();
-        // pass 1: the largest approximate NCC (NaN, at k = n, never counts)
+        // |approximate - exact| <= er * exact + 2 E (E: rounding of the
+        // double sums, er: the float tail); the window keeps every pair whose
+        // exact NCC is within 2^-20 of the maximum
+        const double E = 0x1p-53 * (16.0 * n * __builtin_sqrt(2.0 * n) + 2.0 * n + 64.0);
+        // Lanes take 64 neighbouring l, the waves interleave the lengths
+        // k = r - l + 1, so P[l] stays in a register and the per-k terms are
+        // wave-uniform.  Pass 1 finds the largest value, pass 2 collects the
+        // pairs at or above the window threshold.  Lanes past the triangle
+        // read the padding of P and drop out through the select.  k = n
+        // (the all-zero rectangle) is 0 / 0 in the reference: skipped.
+        const int lmax = n - GP_MINT - 1;  // l + 30 < n
         float mx = -1.f;
-        for (int l = wave; l < m; l += WG / 64) {
+        for (int b = 0; 64 * b <= lmax; ++b) {
+            const int l = 64 * b + lane;
             const double Pl = P[l];
-            for (int r = l + GP_MINT + lane; r < n; r += 64) {
-                const int k = r - l + 1;
-                const double num = A1[k] * ptot - (P[r + 1] - Pl);
+            const int kend = min(n - 64 * b, n - 1);
+            const int lim = l <= lmax ? n - l : -1;  // the lane's pairs: k <= lim
+            float mb = -1.f;
+#pragma unroll 4
+            for (int k = GP_MINT + 1 + wave; k <= kend; k += WG / 64) {
+                const double num = CK[k] - (P[l + k] - Pl);
                 const float v = fabsf((float)num) * rs[k];
-                mx = v > mx ? v : mx;
+                mb = fmaxf(mb, k <= lim ? v : -1.f);
             }
+            // the wave's largest value in this block lets pass 2 skip it
+            for (int o = 32; o > 0; o >>= 1) mb = fmaxf(mb, __shfl_xor(mb, o, 64));
+            if (lane == 0) bmax[wave][b] = mb;
+            mx = fmaxf(mx, mb);
         }
         const float Mf = wg_max_float(mx, fred);
         if (Mf < 0.f) full = true;  // no finite NCC: let the exact scan say so
         else {
-            // |approximate - exact| <= er * exact + 2 E (E: rounding of the
-            // double sums, er: the float tail); the window keeps every pair
-            // whose exact NCC is within 2^-20 of the maximum
-            const double E = 0x1p-53 * (16.0 * n * __builtin_sqrt(2.0 * n) + 2.0 * n + 64.0);
-            const double er = 0x1p-18;
-            const double theta = ((double)Mf - 2.0 * E) * (1.0 - 0x1p-20) * (1.0 - 2.0 * er) - 2.0 * E;
-            for (int l = wave; l < m; l += WG / 64) {
+            const float thg = gp_theta_down(Mf, E);
+            for (int b = 0; 64 * b <= lmax; ++b) {
+                if (bmax[wave][b] < thg) continue;
+                const int l = 64 * b + lane;
                 const double Pl = P[l];
-                for (int r = l + GP_MINT + lane; r < n; r += 64) {
-                    const int k = r - l + 1;
-                    const double num = A1[k] * ptot - (P[r + 1] - Pl);
+                const int kend = min(n - 64 * b, n - 1);
+                const int lim = l <= lmax ? n - l : -1;
+#pragma unroll 4
+                for (int k = GP_MINT + 1 + wave; k <= kend; k += WG / 64) {
+                    const double num = CK[k] - (P[l + k] - Pl);
                     const float v = fabsf((float)num) * rs[k];
-                    if ((double)v >= theta) {
+                    if (k <= lim && v >= thg) {
                         const int at = atomicAdd(&sh_cnt, 1);
-                        if (at < GP_WCAP) list[at] = l * n + r;
+                        if (at < GP_WCAP) list[at] = l * n + l + k - 1;
                     }
                 }
             }
@@ -432,20 +506,44 @@ __global__ __launch_bounds__(WG) void k_gp_views(GpLaunch a)
 }
 
 // main.cpp:206-208: float sums over views in order, divided by the count.
-__global__ void k_gp_average(GpLaunch a)
+// The work-group stages 1024 views at a time in LDS; thread 0 sums them in
+// order, reading a chunk into registers at a time.
+__global__ __launch_bounds__(WG) void k_gp_average(GpLaunch a)
 {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0) return;
+    __shared__ float buf[3 * 1024];
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-    for (int v = 0; v < a.nviews; ++v) {
-        s0 = s0 + a.views[3 * (size_t)v];
-        s1 = s1 + a.views[3 * (size_t)v + 1];
-        s2 = s2 + a.views[3 * (size_t)v + 2];
+    for (int base = 0; base < a.nviews; base += 1024) {
+        const int cnt = min(1024, a.nviews - base);
+        __syncthreads();
+        for (int k = threadIdx.x; k < 3 * cnt; k += WG) buf[k] = a.views[3 * (size_t)base + k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int i = 0;
+            for (; i + 8 <= cnt; i += 8) {
+                float x[24];
+#pragma unroll
+                for (int k = 0; k < 24; ++k) x[k] = buf[3 * i + k];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    s0 = s0 + x[3 * k];
+                    s1 = s1 + x[3 * k + 1];
+                    s2 = s2 + x[3 * k + 2];
+                }
+            }
+            for (; i < cnt; ++i) {
+                s0 = s0 + buf[3 * i];
+                s1 = s1 + buf[3 * i + 1];
+                s2 = s2 + buf[3 * i + 2];
+            }
+        }
     }
-    const float nv = (float)a.nviews;
-    a.avg[0] = s0 / nv;
-    a.avg[1] = s1 / nv;
-    a.avg[2] = s2 / nv;
+    if (threadIdx.x == 0) {
+        const float nv = (float)a.nviews;
+        a.avg[0] = s0 / nv;
+        a.avg[1] = s1 / nv;
+        a.avg[2] = s2 / nv;
+    }
 }
 
 // ------------------------------------------------------------ launchers
@@ -458,7 +556,7 @@ hipError_t launch_gp_scale(const GpLaunch &a, hipStream_t st)
         hipLaunchKernelGGL(k_gp_minmax_y, dim3(a.splits, a.nz), dim3(WG), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gp_scale, dim3((a.nviews + WG - 1) / WG), dim3(WG), 0, st, a);
+    hipLaunchKernelGGL(k_gp_scale, dim3((a.nviews + 63) / 64), dim3(WG), 0, st, a);
     return hipGetLastError();
 }
 
@@ -474,8 +572,8 @@ hipError_t launch_gp_profiles(const GpLaunch &a, hipStream_t st)
 
 size_t gp_search_lds(int n)
 {
-    return sizeof(double) * ((size_t)n + 2 * ((size_t)n + 1) + GP_WCAP + WG) + sizeof(float) * ((size_t)n + 1) +
-           sizeof(int) * GP_WCAP;
+    return sizeof(double) * ((size_t)n + 2 * ((size_t)n + 1) + GP_PAD + GP_WCAP + WG) +
+           sizeof(float) * ((size_t)n + 1) + sizeof(int) * GP_WCAP;
 }
 
 hipError_t launch_gp_search(const GpLaunch &a, hipStream_t st)
@@ -496,7 +594,7 @@ hipError_t launch_gp_views(const GpLaunch &a, hipStream_t st)
     hipLaunchKernelGGL(k_gp_views, dim3((a.nviews + WG - 1) / WG), dim3(WG), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_gp_average, dim3(1), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(k_gp_average, dim3(1), dim3(WG), 0, st, a);
     return hipGetLastError();
 }
 

@@ -229,11 +229,10 @@ constexpr int GP_WIDTH = 50;
 constexpr int GP_MINT = 30;
 constexpr int GP_MAXN = 4096;  // profile rows the search keeps in LDS
 constexpr int GP_WCAP = 256;   // window members checked exactly per chip
-constexpr int GP_SPLITS = 64;  // partial min/max per view
 struct GpLaunch {
     const float *vol;  // [nz][ny][nx]
     int nx, ny, nz, axis, nviews, cols, nchips, exhaustive;
-    float *pmin, *pmax;  // [GP_SPLITS][nviews]
+    float *pmin, *pmax;  // [splits][nviews] partial minima and maxima
     int splits;
     float2 *ab;          // per view: x * a + b normalises to [0, 1]
     float *prof;         // [nviews][nchips][ny]

@@ -240,6 +240,83 @@ def xf_stage(trec, ctx, torch, stack_d, xf, reps=5, cpu=True):
     return res
 
 
+def geopar_stage(trec, ctx, torch, ang, stack_d, reps=5, cpu=True):
+    """AuTom2's geometry workflow (CommandChainManager.qml:210-252) with
+    everything resident in HBM: TiltRec-cuda's SIRT x10 at thickness 300 of
+    the bench series (MRC (X, 300, Y), which is layout Y), then GeoParGen's
+    x views of that volume.  Per-call and per-kernel times (HIP events on the
+    launch stream), the exhaustive search for comparison, and the CPU oracle
+    on a sample of views."""
+    P, Y, X = stack_d.shape
+    T = 300
+    g = trec.make_geom(X, Y, P, T, rules=trec.RULES_CUDA)
+    m = trec.make_method("SIRT,10,1.0")
+    vol = torch.empty((Y, T, X), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    ctx.reconstruct(g, ang, stack_d, m, vol, stream=stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    ctx.reconstruct(g, ang, stack_d, m, vol, stream=stream)
+    ev[1].record(stream)
+    avg, nex = ctx.geopar(vol, trec.GP_VIEW_X, stream=stream)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    rec_ms, first_ms = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.geopar(vol, trec.GP_VIEW_X, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    call_ms = sorted(times)[reps // 2]
+    ctx.profile(True)
+    for _ in range(reps):
+        ctx.geopar(vol, trec.GP_VIEW_X, stream=stream)
+    torch.cuda.synchronize()
+    kern = {}
+    for k in ("gp_profile", "gp_search", "gp_views"):
+        tot, n = ctx.profile_read(k)
+        kern[k] = tot / max(n, 1)
+    ctx.profile(False)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    avg_x, _ = ctx.geopar(vol, trec.GP_VIEW_X, exhaustive=True, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    exh_ms = e0.elapsed_time(e1)
+    nchips = len(range(0, Y - 50, 50))
+    pairs = float(X) * nchips * (T - 30) * (T - 29) / 2
+    # the profile pass reads the volume twice: min/max, then the chips' sections
+    prof_bytes = 4.0 * X * T * Y + 4.0 * X * T * 50 * nchips
+    achieved = prof_bytes / (kern["gp_profile"] / 1e3) / 1e9
+    res = {"workload": f"GeoParGen x views of TiltRec-cuda's SIRT x10 volume of the bench series "
+                       f"(MRC {X} x {T} x {Y}): {X} views, {nchips} chips of {T} rows each",
+           "value": X / (call_ms / 1e3), "unit": "views/s", "ms_per_call": round(call_ms, 4),
+           "first_call_ms": round(first_ms, 4), "reconstruction_ms": round(rec_ms, 3),
+           "averages": [round(v, 5) for v in avg], "exhaustive_chips": nex,
+           "exhaustive_search_ms_per_call": round(exh_ms, 3), "same_as_exhaustive": list(avg) == list(avg_x),
+           "pairs_per_s": pairs / (call_ms / 1e3),
+           "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
+           "roofline": {"bound": "hbm", "kernel": "gp_profile", "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "bytes_per_launch": prof_bytes}}
+    if cpu:
+        from oracle import orc
+        n = 2
+        sub = vol[:, :, 500:500 + n].contiguous().cpu().numpy()
+        t0 = time.perf_counter()
+        orc.geopar(sub, 0)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": n / dt, "unit": "views/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} of {X} views ({dt:.2f} s); the reference evaluates each "
+                                         "view's search twice, the oracle once"}
+    del vol
+    return res
+
+
 def counters_run(iters):
     """The run the counter profiles measure: `iters` SIRT iterations of the
     bench workload through the library, then a 1 GiB device copy whose
@@ -298,6 +375,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--counters", action="store_true", help="only the run the counter profiles measure")
     ap.add_argument("--no-xf", action="store_true", help="skip the .xf application stage")
+    ap.add_argument("--no-geopar", action="store_true", help="skip the GeoParGen stage")
+    ap.add_argument("--geopar-only", action="store_true",
+                    help="only the GeoParGen stage (its own JSON line), as its profiles measure it")
     ap.add_argument("--xf-only", action="store_true",
                     help="only the .xf application stage (its own JSON line), as its profiles measure it")
     args = ap.parse_args()
@@ -321,6 +401,9 @@ def main():
     stack_d = torch.from_numpy(stack).cuda()
     geom = trec.make_geom(X, Y, P, T)
     ctx = trec.Context(0)
+    if args.geopar_only:
+        print(json.dumps({"geopar": geopar_stage(trec, ctx, torch, ang, stack_d, cpu=not args.no_cpu)}), flush=True)
+        return
     if args.xf_only:
         print(json.dumps({"xf_apply": xf_stage(trec, ctx, torch, stack_d, _xf, cpu=not args.no_cpu)}), flush=True)
         return
@@ -385,6 +468,12 @@ def main():
         log(f"[bench] .xf application: {xf_res['ms_per_call']:.3f} ms/call, warp "
             f"{xf_res['kernels_ms']['xf_warp']:.3f} ms ({xf_res['roofline']['frac']:.3f} of HBM peak)")
 
+    gp_res = None
+    if not args.no_geopar:
+        gp_res = geopar_stage(trec, ctx, torch, ang, stack_d, cpu=not args.no_cpu)
+        log(f"[bench] GeoParGen: {gp_res['ms_per_call']:.3f} ms/call, search {gp_res['kernels_ms']['gp_search']:.3f} ms, "
+            f"exhaustive {gp_res['exhaustive_search_ms_per_call']:.1f} ms")
+
     cpu = None
     if not args.no_cpu:
         rows = args.cpu_rows
@@ -418,6 +507,7 @@ def main():
         "cpu_baseline": cpu,
         "forms": forms,
         "xf_apply": xf_res,
+        "geopar": gp_res,
     }
     print(json.dumps(out), flush=True)
 
