@@ -302,7 +302,11 @@ def geopar_stage(trec, ctx, torch, ang, stack_d, reps=5, cpu=True):
            "kernels_ms": {k: round(v, 4) for k, v in kern.items()},
            "roofline": {"bound": "hbm", "kernel": "gp_profile", "achieved": round(achieved, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                        "bytes_per_launch": prof_bytes}}
+                        "bytes_per_launch": prof_bytes, "traffic": None}}
+    if (P, Y, X) == (61, 1024, 1024):
+        t = [profiled_traffic(k, "geopar1024_kernel_trace.md") for k in ("gp_minmax_x", "gp_profiles_x")]
+        if None not in t:
+            res["roofline"]["traffic"] = sum(t)
     if cpu:
         from oracle import orc
         n = 2
