@@ -1374,6 +1374,25 @@ trec_status trec_geopar(trec_context *ctx, const float *volume, int nx, int ny, 
     return TREC_OK;
 }
 
+trec_status trec_convert(trec_context *ctx, const void *in, int in_type, void *out, int out_type, size_t n,
+                         void *stream)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    if (!in || !out) return fail(ctx, TREC_ERR_INVALID, "null buffer");
+    if (in_type < TREC_DT_I8 || in_type > TREC_DT_F32) return fail(ctx, TREC_ERR_INVALID, "bad input type %d", in_type);
+    if (out_type != TREC_DT_F32 && out_type != TREC_DT_F16)
+        return fail(ctx, TREC_ERR_INVALID, "output type %d is not float or half", out_type);
+    if (n == 0) return TREC_OK;
+    static const size_t esz[] = {1, 1, 1, 2, 2, 2, 4};
+    const char *a = (const char *)in, *b = (const char *)out;
+    if (a < b + n * esz[out_type] && b < a + n * esz[in_type])
+        return fail(ctx, TREC_ERR_INVALID, "input and output overlap");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, launch_convert(in, in_type, out, out_type, n, (hipStream_t)stream));
+    return TREC_OK;
+}
+
 trec_status trec_float_to_byte(trec_context *ctx, const float *in, size_t n, unsigned char *out,
                                float *stats_host, void *stream)
 {

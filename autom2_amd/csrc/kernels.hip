@@ -1618,6 +1618,60 @@ hipError_t launch_fbp_filter(float *data, const float *taps, int N, int X, long 
     return hipGetLastError();
 }
 
+// Element conversion for MRC data (trec_convert): every type to float
+// exactly (bytes over 255 as mrctools' reader divides them), float to half
+// rounded to nearest even.  Grid-stride over n elements.
+template <typename Tin, typename Tout, bool DIV255>
+__global__ __launch_bounds__(WG) void k_convert(const Tin *__restrict__ in, Tout *__restrict__ out, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n; i += (size_t)gridDim.x * WG) {
+        float v = (float)in[i];
+        if (DIV255) v = v / 255.0f;
+        out[i] = (Tout)v;
+    }
+}
+
+template <typename Tout>
+static hipError_t convert_to(const void *in, int in_type, Tout *out, size_t n, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)std::min<size_t>(std::max<size_t>(cdiv(n, WG), 1), 16384);
+    switch (in_type) {
+    case TREC_DT_I8:
+        hipLaunchKernelGGL((k_convert<signed char, Tout, false>), dim3(blocks), dim3(WG), 0, st,
+                           (const signed char *)in, out, n);
+        break;
+    case TREC_DT_U8:
+        hipLaunchKernelGGL((k_convert<unsigned char, Tout, false>), dim3(blocks), dim3(WG), 0, st,
+                           (const unsigned char *)in, out, n);
+        break;
+    case TREC_DT_U8_DIV255:
+        hipLaunchKernelGGL((k_convert<unsigned char, Tout, true>), dim3(blocks), dim3(WG), 0, st,
+                           (const unsigned char *)in, out, n);
+        break;
+    case TREC_DT_I16:
+        hipLaunchKernelGGL((k_convert<short, Tout, false>), dim3(blocks), dim3(WG), 0, st, (const short *)in, out, n);
+        break;
+    case TREC_DT_U16:
+        hipLaunchKernelGGL((k_convert<unsigned short, Tout, false>), dim3(blocks), dim3(WG), 0, st,
+                           (const unsigned short *)in, out, n);
+        break;
+    case TREC_DT_F16:
+        hipLaunchKernelGGL((k_convert<_Float16, Tout, false>), dim3(blocks), dim3(WG), 0, st, (const _Float16 *)in,
+                           out, n);
+        break;
+    default:
+        hipLaunchKernelGGL((k_convert<float, Tout, false>), dim3(blocks), dim3(WG), 0, st, (const float *)in, out, n);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const void *in, int in_type, void *out, int out_type, size_t n, hipStream_t st)
+{
+    if (out_type == TREC_DT_F16) return convert_to(in, in_type, (_Float16 *)out, n, st);
+    return convert_to(in, in_type, (float *)out, n, st);
+}
+
 hipError_t launch_to_byte(const float *a, size_t n, double *part, float *stats, unsigned char *out, hipStream_t st)
 {
     const int nb = (int)std::min<size_t>(std::max<size_t>(cdiv(n, WG), 1), 2048);

@@ -5,7 +5,11 @@
 #include <cerrno>
 #include <cfloat>
 #include <cstdio>
+#include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <thread>
+#include <utility>
 
 namespace trec_cli {
 
@@ -19,16 +23,60 @@ struct File {
     }
 };
 
+// Bytes per element as read_raw returns them (mode 101 unpacked to bytes).
 size_t mode_bytes(int mode)
 {
     switch (mode) {
-    case MRC_BYTE: return 1;
+    case MRC_BYTE:
+    case MRC_NIBBLE: return 1;
     case MRC_SHORT:
     case MRC_USHORT:
     case MRC_HALF: return 2;
     case MRC_FLOAT: return 4;
     default: return 0;
     }
+}
+
+uint16_t bswap16(uint16_t v) { return (uint16_t)((v << 8) | (v >> 8)); }
+uint32_t bswap32(uint32_t v) { return (v << 24) | ((v << 8) & 0xff0000u) | ((v >> 8) & 0xff00u) | (v >> 24); }
+
+// Byte ranges of the header by field width.  Bytes 88-91 are MRC2014's
+// ISPG (int32; IMOD's ispg and nsymbt halves), 104-107 EXTTYP (text),
+// 108-111 NVERSION; 128-135 and 160-171 hold IMOD's 16-bit fields.
+void swap_header(MrcHeader &h)
+{
+    unsigned char *b = reinterpret_cast<unsigned char *>(&h);
+    auto w32 = [b](int lo, int hi) {
+        for (int o = lo; o < hi; o += 4) {
+            uint32_t v;
+            memcpy(&v, b + o, 4);
+            v = bswap32(v);
+            memcpy(b + o, &v, 4);
+        }
+    };
+    auto w16 = [b](int lo, int hi) {
+        for (int o = lo; o < hi; o += 2) {
+            uint16_t v;
+            memcpy(&v, b + o, 2);
+            v = bswap16(v);
+            memcpy(b + o, &v, 2);
+        }
+    };
+    w32(0, 96);     // nx .. amean, ISPG, next
+    w16(96, 98);    // creatid
+    w32(108, 112);  // NVERSION
+    w16(128, 136);  // nint, nreal, sub, zfac
+    w32(136, 160);  // min2 .. max4 (IMOD's stamp and flags)
+    w16(160, 172);  // idtype .. vd2
+    w32(172, 208);  // tilt angles, origin
+    w32(216, 224);  // rms, nlabl
+}
+
+bool plausible(const MrcHeader &h)
+{
+    const int lim = 1 << 28;
+    return h.nx > 0 && h.ny > 0 && h.nz > 0 && h.nx < lim && h.ny < lim && h.nz < lim && mode_bytes(h.mode) &&
+           h.next >= 0;
 }
 
 float half_to_float(uint16_t h)
@@ -58,7 +106,7 @@ float half_to_float(uint16_t h)
 
 }  // namespace
 
-bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err)
+bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err, bool *swapped)
 {
     File fh;
     fh.f = fopen(path.c_str(), "rb");
@@ -70,8 +118,24 @@ bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err)
         err = "short MRC header in " + path;
         return false;
     }
+    // MRC2014's machine stamp: 0x44 0x44 (or 0x44 0x41) little-endian, 0x11
+    // 0x11 big-endian; files without one are taken in whichever order gives
+    // a plausible header
+    const unsigned char s0 = (unsigned char)h.stamp[0];
+    bool be = s0 == 0x11;
+    if (s0 != 0x11 && s0 != 0x44 && !plausible(h)) {
+        MrcHeader t = h;
+        swap_header(t);
+        be = plausible(t);
+    }
+    if (be) swap_header(h);
+    if (swapped) *swapped = be;
     if (h.nx <= 0 || h.ny <= 0 || h.nz <= 0) {
         err = "bad MRC size in " + path;
+        return false;
+    }
+    if (h.mode == 3 || h.mode == 4) {
+        err = "complex MRC data (mode " + std::to_string(h.mode) + ") cannot be reconstructed";
         return false;
     }
     if (!mode_bytes(h.mode)) {
@@ -85,9 +149,10 @@ bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err)
     return true;
 }
 
-bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &data, std::string &err, ByteRule bytes)
+bool mrc_read_raw(const std::string &path, MrcHeader &h, std::vector<unsigned char> &raw, std::string &err)
 {
-    if (!mrc_read_header(path, h, err)) return false;
+    bool be = false;
+    if (!mrc_read_header(path, h, err, &be)) return false;
     File fh;
     fh.f = fopen(path.c_str(), "rb");
     if (!fh.f) {
@@ -100,11 +165,43 @@ bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &d
         err = "seek failed in " + path;
         return false;
     }
-    std::vector<unsigned char> raw(n * es);
+    if (h.mode == MRC_NIBBLE) {
+        // two pixels per byte, low nibble first, rows padded to whole bytes
+        const size_t rb = ((size_t)h.nx + 1) / 2, rows = (size_t)h.ny * h.nz;
+        std::vector<unsigned char> packed(rb * rows);
+        if (fread(packed.data(), 1, packed.size(), fh.f) != packed.size()) {
+            err = "truncated MRC data in " + path;
+            return false;
+        }
+        raw.resize(n);
+        for (size_t y = 0; y < rows; ++y)
+            for (int x = 0; x < h.nx; ++x) {
+                const unsigned char v = packed[y * rb + x / 2];
+                raw[y * h.nx + x] = (x & 1) ? (v >> 4) : (v & 15);
+            }
+        return true;
+    }
+    raw.resize(n * es);
     if (fread(raw.data(), es, n, fh.f) != n) {
         err = "truncated MRC data in " + path;
         return false;
     }
+    if (be && es == 2) {
+        for (size_t i = 0; i < n; ++i) std::swap(raw[2 * i], raw[2 * i + 1]);
+    } else if (be && es == 4) {
+        for (size_t i = 0; i < n; ++i) {
+            std::swap(raw[4 * i], raw[4 * i + 3]);
+            std::swap(raw[4 * i + 1], raw[4 * i + 2]);
+        }
+    }
+    return true;
+}
+
+bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &data, std::string &err, ByteRule bytes)
+{
+    std::vector<unsigned char> raw;
+    if (!mrc_read_raw(path, h, raw, err)) return false;
+    const size_t n = (size_t)h.nx * h.ny * h.nz;
     data.resize(n);
     switch (h.mode) {
     case MRC_BYTE:
@@ -112,6 +209,9 @@ bool mrc_read_float(const std::string &path, MrcHeader &h, std::vector<float> &d
             for (size_t i = 0; i < n; ++i) data[i] = raw[i] / 255.0f;
         else
             for (size_t i = 0; i < n; ++i) data[i] = (float)(signed char)raw[i];
+        break;
+    case MRC_NIBBLE:
+        for (size_t i = 0; i < n; ++i) data[i] = raw[i];
         break;
     case MRC_SHORT:
         for (size_t i = 0; i < n; ++i) {
@@ -162,23 +262,111 @@ MrcHeader mrc_new_header(int nx, int ny, int nz)
     return h;
 }
 
-bool mrc_write_float(const std::string &path, MrcHeader h, const float *data, std::string &err)
+namespace {
+
+float elem(const void *data, int mode, size_t i)
 {
-    const size_t sec = (size_t)h.nx * h.ny;
-    const float *mid = data + sec * (size_t)(int)(h.nz * .5);
-    float amin = FLT_MAX, amax = FLT_MIN;
-    long double amean = 0;
-    for (size_t i = sec; i--;) {
-        if (mid[i] > amax) amax = mid[i];
-        if (mid[i] < amin) amin = mid[i];
-        amean += mid[i];
+    if (mode == MRC_HALF) {
+        uint16_t v;
+        memcpy(&v, (const unsigned char *)data + 2 * i, 2);
+        return half_to_float(v);
     }
-    amean /= sec;
-    h.amin = amin;
-    h.amax = amax;
-    h.amean = (float)amean;
-    h.nlabl = 0;
-    h.mode = MRC_FLOAT;
+    return ((const float *)data)[i];
+}
+
+// Whole-map statistics for MRC2014 headers: min, max, mean and the RMS
+// deviation from the mean, in double over fixed slices (deterministic).
+void map_stats(const void *data, int mode, size_t n, float &amin, float &amax, float &amean, float &rms)
+{
+    const int nt = (int)std::max<unsigned>(1, std::min<unsigned>(16, std::thread::hardware_concurrency()));
+    std::vector<double> s1(nt, 0), s2(nt, 0);
+    std::vector<float> lo(nt, INFINITY), hi(nt, -INFINITY);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            const size_t a = n * t / nt, b = n * (t + 1) / nt;
+            double x1 = 0, x2 = 0;
+            float l = INFINITY, u = -INFINITY;
+            for (size_t i = a; i < b; ++i) {
+                const float v = elem(data, mode, i);
+                x1 += v;
+                x2 += (double)v * v;
+                l = v < l ? v : l;
+                u = v > u ? v : u;
+            }
+            s1[t] = x1;
+            s2[t] = x2;
+            lo[t] = l;
+            hi[t] = u;
+        });
+    for (auto &x : th) x.join();
+    double x1 = 0, x2 = 0;
+    amin = INFINITY;
+    amax = -INFINITY;
+    for (int t = 0; t < nt; ++t) {
+        x1 += s1[t];
+        x2 += s2[t];
+        amin = std::min(amin, lo[t]);
+        amax = std::max(amax, hi[t]);
+    }
+    const double mean = x1 / n;
+    amean = (float)mean;
+    rms = (float)std::sqrt(std::max(0.0, x2 / n - mean * mean));
+}
+
+}  // namespace
+
+void mrc_set_2014(MrcHeader &h, float pixel)
+{
+    memcpy(h.cmap, "MAP ", 4);
+    const unsigned char st[4] = {0x44, 0x44, 0, 0};
+    memcpy(h.stamp, st, 4);
+    const int32_t version = 20140;
+    memcpy(h.blank + (108 - 98), &version, 4);  // NVERSION, word 28
+    h.mx = h.nx;
+    h.my = h.ny;
+    h.mz = h.nz;
+    const float px = pixel > 0 ? pixel : 1.f;
+    h.xlen = px * h.nx;
+    h.ylen = px * h.ny;
+    h.zlen = px * h.nz;
+    h.alpha = h.beta = h.gamma = 90;
+}
+
+bool mrc_write_volume(const std::string &path, MrcHeader h, const void *data, int mode, bool mrc2014,
+                      std::string &err)
+{
+    if (mode != MRC_FLOAT && mode != MRC_HALF) {
+        err = "MRC output mode must be 2 or 12";
+        return false;
+    }
+    const size_t sec = (size_t)h.nx * h.ny, n = sec * h.nz;
+    if (mrc2014) {
+        float rms;
+        map_stats(data, mode, n, h.amin, h.amax, h.amean, rms);
+        h.rms = rms;
+        h.nlabl = 1;
+        memset(h.labels, 0, sizeof h.labels);
+        snprintf(h.labels[0], sizeof h.labels[0], "%-79s", "autom2_amd: TiltRec reconstruction on MI355X");
+    } else {
+        // UpdateHeader (mrcstack.cpp:104-142): section nz / 2, amax starting
+        // at FLT_MIN, the mean accumulated in long double
+        const size_t mid = sec * (size_t)(int)(h.nz * .5);
+        float amin = FLT_MAX, amax = FLT_MIN;
+        long double amean = 0;
+        for (size_t i = sec; i--;) {
+            const float v = elem(data, mode, mid + i);
+            if (v > amax) amax = v;
+            if (v < amin) amin = v;
+            amean += v;
+        }
+        amean /= sec;
+        h.amin = amin;
+        h.amax = amax;
+        h.amean = (float)amean;
+        h.nlabl = 0;
+    }
+    h.mode = mode;
     h.next = 0;
     File fh;
     fh.f = fopen(path.c_str(), "wb");
@@ -186,8 +374,8 @@ bool mrc_write_float(const std::string &path, MrcHeader h, const float *data, st
         err = "cannot create " + path + ": " + strerror(errno);
         return false;
     }
-    const size_t n = sec * h.nz;
-    if (fwrite(&h, sizeof h, 1, fh.f) != 1 || fwrite(data, 4, n, fh.f) != n) {
+    const size_t es = mode == MRC_HALF ? 2 : 4;
+    if (fwrite(&h, sizeof h, 1, fh.f) != 1 || fwrite(data, es, n, fh.f) != n) {
         err = "write failed for " + path;
         return false;
     }
@@ -198,6 +386,11 @@ bool mrc_write_float(const std::string &path, MrcHeader h, const float *data, st
     }
     fh.f = nullptr;
     return true;
+}
+
+bool mrc_write_float(const std::string &path, MrcHeader h, const float *data, std::string &err)
+{
+    return mrc_write_volume(path, h, data, MRC_FLOAT, false, err);
 }
 
 bool mrc_write_bytes(const std::string &path, MrcHeader h, const unsigned char *data, std::string &err)

@@ -5,8 +5,12 @@
 // format's own layout.  Reading follows MrcStackM::ReadBlock
 // (src/TiltRec/src/mrcmx/mrcstack.cpp:354-459): data start at 1024 + next,
 // mode 0 is read as signed char (savedate, :255-287), 1 as short, 2 as
-// float; modes 6 (unsigned short) and 12 (half) are read too.  Writing
-// follows InitializeHeader / WriteToFile / UpdateHeader (:14-142).
+// float.  Beyond the reference, the readers take the rest of MRC2014: modes
+// 6 (unsigned short), 12 (half) and 101 (4-bit), and big-endian files (by
+// the machine stamp, or by whichever byte order gives a plausible header
+// when there is none).  Writing follows InitializeHeader / WriteToFile /
+// UpdateHeader (:14-142); on request it writes mode 12 and an MRC2014
+// header instead.
 #pragma once
 
 #include <cstddef>
@@ -43,10 +47,15 @@ struct MrcHeader {
 #pragma pack(pop)
 static_assert(sizeof(MrcHeader) == 1024, "MRC header is 1024 bytes");
 
-enum { MRC_BYTE = 0, MRC_SHORT = 1, MRC_FLOAT = 2, MRC_USHORT = 6, MRC_HALF = 12 };
+enum { MRC_BYTE = 0, MRC_SHORT = 1, MRC_FLOAT = 2, MRC_USHORT = 6, MRC_HALF = 12, MRC_NIBBLE = 101 };
 
-// Reads the header; returns false with a message on failure.
-bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err);
+// Reads the header in native byte order; returns false with a message on
+// failure.  *swapped (may be NULL) tells whether the file is big-endian.
+bool mrc_read_header(const std::string &path, MrcHeader &h, std::string &err, bool *swapped = nullptr);
+
+// Reads the data as stored, in native byte order: nx * ny * nz elements of
+// the mode's type (mode 101 unpacked to one unsigned byte per pixel).
+bool mrc_read_raw(const std::string &path, MrcHeader &h, std::vector<unsigned char> &raw, std::string &err);
 
 // How mode 0 (bytes) becomes float: TiltRec's reader takes signed char
 // (mrcstack.cpp:255-287); mrctools' MrcStack takes unsigned char / 255
@@ -74,5 +83,16 @@ MrcHeader mrc_new_header(int nx, int ny, int nz);
 // from section nz/2 as UpdateHeader does (mrcstack.cpp:104-142: amax
 // starts at FLT_MIN, the mean is accumulated in long double) and nlabl = 0.
 bool mrc_write_float(const std::string &path, MrcHeader h, const float *data, std::string &err);
+
+// MRC2014 header fields: the "MAP " stamp, a little-endian machine stamp,
+// NVERSION 20140, sampling mx/my/mz = nx/ny/nz and cell lengths of `pixel`
+// (Angstrom, 1 when unknown) per voxel.
+void mrc_set_2014(MrcHeader &h, float pixel);
+
+// Writes mode 2 (float) or 12 (half) data.  Statistics: with mrc2014, min,
+// max, mean and RMS deviation of the whole map and one label; otherwise
+// UpdateHeader's, as mrc_write_float.
+bool mrc_write_volume(const std::string &path, MrcHeader h, const void *data, int mode, bool mrc2014,
+                      std::string &err);
 
 }  // namespace trec_cli

@@ -88,6 +88,8 @@ void usage()
                  "[--rules mpi|cuda]  thresholds and y origin of TiltRec-mpi or TiltRec-cuda\n"
                  "[--storage f32|f16|bf16]  projection/residual storage of SIRT and SART\n"
                  "[--device N]  HIP device\n"
+                 "[--output-mode 2|12]  MRC mode of the volume: 2 float (default), 12 half\n"
+                 "[--mrc2014]  MRC2014 header: MAP and machine stamps, sampling, whole-map statistics\n"
                  "-help(-h)\n"
                  "EXAMPLE:\n"
                  "  TiltRec-cuda --input BBb_fin.mrc --output BBb_SIRT.mrc --tiltfile BBb.rawtlt "
@@ -96,7 +98,7 @@ void usage()
 
 int get_opts(int argc, char **argv, Options &o)
 {
-    enum { OPT_RULES = 1000, OPT_STORAGE, OPT_DEVICE, OPT_DRY };
+    enum { OPT_RULES = 1000, OPT_STORAGE, OPT_DEVICE, OPT_DRY, OPT_OUTMODE, OPT_MRC2014 };
     static struct option longopts[] = {{"help", no_argument, nullptr, 'h'},
                                        {"input", required_argument, nullptr, 'i'},
                                        {"output", required_argument, nullptr, 'o'},
@@ -111,6 +113,8 @@ int get_opts(int argc, char **argv, Options &o)
                                        {"storage", required_argument, nullptr, OPT_STORAGE},
                                        {"device", required_argument, nullptr, OPT_DEVICE},
                                        {"dry-run", no_argument, nullptr, OPT_DRY},
+                                       {"output-mode", required_argument, nullptr, OPT_OUTMODE},
+                                       {"mrc2014", no_argument, nullptr, OPT_MRC2014},
                                        {nullptr, 0, nullptr, 0}};
     optind = 1;
     opterr = 0;
@@ -193,6 +197,13 @@ int get_opts(int argc, char **argv, Options &o)
             }
             break;
         case OPT_DRY: o.dry_run = true; break;
+        case OPT_OUTMODE:
+            if (!to_int(optarg, o.output_mode) || (o.output_mode != 2 && o.output_mode != 12)) {
+                fprintf(stderr, "Invalid argument for output-mode '%s' (2 or 12).\n", optarg);
+                return -1;
+            }
+            break;
+        case OPT_MRC2014: o.mrc2014 = true; break;
         default:
             fprintf(stderr, "Invalid option '%s'.\n", argv[optind - 1]);
             return -1;
@@ -200,6 +211,10 @@ int get_opts(int argc, char **argv, Options &o)
     }
     if (optind < argc) {
         fprintf(stderr, "Unexpected argument '%s'.\n", argv[optind]);
+        return -1;
+    }
+    if (o.f2b && o.output_mode != 2) {
+        fprintf(stderr, "--f2b writes bytes; it cannot be combined with --output-mode %d.\n", o.output_mode);
         return -1;
     }
     if (o.input.empty() || o.output.empty() || o.angle.empty() || o.method.empty() || o.thickness <= 0) {
@@ -218,7 +233,8 @@ std::string opts_json(const Options &o)
       << ",\"method\":\"" << esc(o.method) << "\",\"iterations\":" << o.iteration << ",\"gamma\":" << o.gamma
       << ",\"cgiter\":" << o.cgiter << ",\"soft\":" << o.soft
       << ",\"axis\":\"" << esc(o.axis) << "\",\"f2b\":" << (o.f2b ? "true" : "false") << ",\"rules\":\""
-      << esc(o.rules) << "\",\"storage\":\"" << esc(o.storage) << "\",\"device\":" << o.device << "}";
+      << esc(o.rules) << "\",\"storage\":\"" << esc(o.storage) << "\",\"device\":" << o.device
+      << ",\"output_mode\":" << o.output_mode << ",\"mrc2014\":" << (o.mrc2014 ? "true" : "false") << "}";
     return s.str();
 }
 

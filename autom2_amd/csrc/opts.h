@@ -17,10 +17,12 @@ struct Options {
     float gamma = 1.0f, soft = 0;
     std::string axis = "y";
     bool f2b = false;
-    // additions of this build
+    // additions beyond the reference's options
     std::string rules;     // "mpi" | "cuda" ("" = follow the program name)
     std::string storage = "f32";
     int device = 0;
+    int output_mode = 2;   // MRC mode of the volume: 2 (float) or 12 (half)
+    bool mrc2014 = false;  // MRC2014 header: MAP stamp, machine stamp, sampling, whole-map statistics
     bool dry_run = false;  // print the parsed options as JSON and exit
 };
 

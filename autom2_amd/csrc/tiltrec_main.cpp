@@ -16,6 +16,7 @@
 // main-cuda.cpp:406-411), so the workflow sees it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -115,9 +116,9 @@ int main(int argc, char **argv)
 
     const double t0 = now_s();
     MrcHeader hin;
-    std::vector<float> stack;
+    std::vector<unsigned char> stack;
     std::string err;
-    if (!mrc_read_float(o.input, hin, stack, err)) return die(4, "%s", err.c_str());
+    if (!mrc_read_raw(o.input, hin, stack, err)) return die(4, "%s", err.c_str());
     std::vector<float> angles;
     if (!read_angles(o.angle, angles)) return die(4, "cannot read tilt file %s", o.angle.c_str());
     if ((int)angles.size() < hin.nz)
@@ -127,10 +128,10 @@ int main(int argc, char **argv)
     printf("TiltRec-hip: %d projections of %d x %d -> %d x %d x %d (%s), method %s\n", P, X, Y, X, Y, T,
            layout == TREC_LAYOUT_Z ? "TiltRecZ orientation" : "TiltRec orientation", o.method.c_str());
 
-    std::vector<float> init;
+    MrcHeader hi{};
+    std::vector<unsigned char> init;
     if (!o.initial.empty()) {
-        MrcHeader hi;
-        if (!mrc_read_float(o.initial, hi, init, err)) return die(4, "%s", err.c_str());
+        if (!mrc_read_raw(o.initial, hi, init, err)) return die(4, "%s", err.c_str());
         const bool ok = layout == TREC_LAYOUT_Y ? (hi.nx == X && hi.ny == T && hi.nz == Y)
                                                 : (hi.nx == X && hi.ny == Y && hi.nz == T);
         if (!ok) return die(4, "initial volume %s does not match the output size", o.initial.c_str());
@@ -142,16 +143,31 @@ int main(int argc, char **argv)
     trec_method m{kind, o.iteration, o.gamma, storage, layout, o.cgiter, o.soft};
     if (hipSetDevice(o.device) != hipSuccess) return die(5, "HIP device %s is not available", std::to_string(o.device).c_str());
     const size_t nstack = (size_t)P * Y * X, nvol = (size_t)X * Y * T;
-    DevBuf dstack, dvol;
+    DevBuf dstack, dvol, draw;
     if (hipMalloc(&dstack.p, 4 * nstack) != hipSuccess || hipMalloc(&dvol.p, 4 * nvol) != hipSuccess)
         return die(5, "device allocation failed");
-    if (hipMemcpy(dstack.p, stack.data(), 4 * nstack, hipMemcpyHostToDevice) != hipSuccess)
-        return die(5, "copy to device failed");
-    if (!init.empty() && hipMemcpy(dvol.p, init.data(), 4 * nvol, hipMemcpyHostToDevice) != hipSuccess)
-        return die(5, "copy to device failed");
     trec_context *ctx = nullptr;
     trec_status s = trec_context_create(o.device, &ctx);
     if (s != TREC_OK) return die(5, "context: %s", trec_status_string(s));
+    // the file's own elements go up; the GPU widens them to float
+    // (MRC mode 0 as signed bytes, mrcstack.cpp:255-287)
+    auto upload = [&](const std::vector<unsigned char> &raw, int mode, void *dst, size_t n) -> bool {
+        if (mode == MRC_FLOAT) return hipMemcpy(dst, raw.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess;
+        if (!draw.p && hipMalloc(&draw.p, 2 * std::max(nstack, nvol)) != hipSuccess) return false;
+        if (hipMemcpy(draw.p, raw.data(), raw.size(), hipMemcpyHostToDevice) != hipSuccess) return false;
+        const int dt = mode == MRC_BYTE ? TREC_DT_I8
+                     : mode == MRC_NIBBLE ? TREC_DT_U8
+                     : mode == MRC_SHORT ? TREC_DT_I16
+                     : mode == MRC_USHORT ? TREC_DT_U16 : TREC_DT_F16;
+        return trec_convert(ctx, draw.p, dt, dst, TREC_DT_F32, n, nullptr) == TREC_OK &&
+               hipDeviceSynchronize() == hipSuccess;
+    };
+    if (!upload(stack, hin.mode, dstack.p, nstack) || (!init.empty() && !upload(init, hi.mode, dvol.p, nvol))) {
+        trec_context_destroy(ctx);
+        return die(5, "copy to device failed");
+    }
+    stack.clear();
+    stack.shrink_to_fit();
     const double t_up = now_s();
     s = trec_reconstruct(ctx, &g, angles.data(), (const float *)dstack.p, init.empty() ? nullptr : (const float *)dvol.p,
                          &m, (float *)dvol.p, nullptr);
@@ -163,6 +179,7 @@ int main(int argc, char **argv)
     }
     const double t_rec = now_s();
     MrcHeader hout = layout == TREC_LAYOUT_Y ? mrc_new_header(X, T, Y) : mrc_new_header(X, Y, T);
+    if (o.mrc2014) mrc_set_2014(hout, hin.mx > 0 && hin.xlen > 0 ? hin.xlen / hin.mx : 1.f);
     if (o.f2b) {
         // byte output over the whole volume's mean and deviation, as
         // TiltRecZ-cuda writes it (recon_z.cu:253-262)
@@ -184,12 +201,26 @@ int main(int argc, char **argv)
             return die(5, "copy from device failed");
         printf("f2b: mean %g, standard deviation %g\n", stats[0], stats[1]);
         if (!mrc_write_bytes(o.output, hout, bytes.data(), err)) return die(4, "%s", err.c_str());
+    } else if (o.output_mode == MRC_HALF) {
+        // half output: the GPU rounds to nearest even, half the bytes come down
+        DevBuf dhalf;
+        if (hipMalloc(&dhalf.p, 2 * nvol) != hipSuccess) {
+            trec_context_destroy(ctx);
+            return die(5, "device allocation failed");
+        }
+        s = trec_convert(ctx, dvol.p, TREC_DT_F32, dhalf.p, TREC_DT_F16, nvol, nullptr);
+        trec_context_destroy(ctx);
+        if (s != TREC_OK) return die(6, "half conversion failed: %s", trec_status_string(s));
+        std::vector<uint16_t> vol(nvol);
+        if (hipMemcpy(vol.data(), dhalf.p, 2 * nvol, hipMemcpyDeviceToHost) != hipSuccess)
+            return die(5, "copy from device failed");
+        if (!mrc_write_volume(o.output, hout, vol.data(), MRC_HALF, o.mrc2014, err)) return die(4, "%s", err.c_str());
     } else {
         trec_context_destroy(ctx);
         std::vector<float> vol(nvol);
         if (hipMemcpy(vol.data(), dvol.p, 4 * nvol, hipMemcpyDeviceToHost) != hipSuccess)
             return die(5, "copy from device failed");
-        if (!mrc_write_float(o.output, hout, vol.data(), err)) return die(4, "%s", err.c_str());
+        if (!mrc_write_volume(o.output, hout, vol.data(), MRC_FLOAT, o.mrc2014, err)) return die(4, "%s", err.c_str());
     }
     const double t_end = now_s();
     printf("read %.3f s, upload %.3f s, reconstruct %.3f s, write %.3f s, total %.3f s\n", t_read - t0,

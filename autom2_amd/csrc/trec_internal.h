@@ -133,6 +133,8 @@ hipError_t launch_fbp_filter(float *data, const float *taps, int N, int X, long 
 // --f2b: mean and standard deviation of n floats into stats[0..1] (part
 // holds 2048 doubles), then the byte mapping into out.
 hipError_t launch_to_byte(const float *a, size_t n, double *part, float *stats, unsigned char *out, hipStream_t st);
+// trec_convert's element conversion (types are trec_dtype values).
+hipError_t launch_convert(const void *in, int in_type, void *out, int out_type, size_t n, hipStream_t st);
 // out += sum of squares (in double) of n values of the storage type.
 hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipStream_t st);
 

@@ -93,11 +93,14 @@ SIGNATURES = [
     ("trec_request_footprint", _I, [ctypes.POINTER(Geom), ctypes.POINTER(Method), ctypes.POINTER(ctypes.c_size_t)]),
     ("trec_server_destroy", None, [_P]),
     ("trec_apply_xf", _I, [_P, _I, _I, _I, _PF, _I, _P, _P, _P]),
+    ("trec_convert", _I, [_P, _P, _I, _P, _I, ctypes.c_size_t, _P]),
     ("trec_geopar", _I, [_P, _P, _I, _I, _I, _I, _I, _P, _P, _PF, ctypes.POINTER(_I), _P]),
 ]
 
 XF_CUBIC, XF_LANCZOS4 = 1, 2  # mrcstack -m: 2 selects Lanczos-4, anything else cubic
 GP_VIEW_X, GP_VIEW_Y = 0, 1    # GeoParGen's "x" (default) and "y" views
+# trec_dtype: element types of MRC data
+DT_I8, DT_U8, DT_U8_DIV255, DT_I16, DT_U16, DT_F16, DT_F32 = range(7)
 GP_EXHAUSTIVE = 1              # trec_geopar flag: exact NCC of every pair
 
 _lib = None
@@ -335,6 +338,12 @@ class Context:
         x = np.ascontiguousarray(np.asarray(xf, np.float32).reshape(nsec, 6))
         _check(lib().trec_apply_xf(self._h, nx, ny, nsec, x.ctypes.data_as(_PF), int(mode), _ptr(stack),
                                    _ptr(out), _stream(stream)), self._h)
+
+    def convert(self, src, src_type, dst, dst_type, stream=None):
+        """trec_convert: the elements of device tensor src (interpreted as
+        src_type) into dst (DT_F32 or DT_F16), same element count."""
+        _check(lib().trec_convert(self._h, _ptr(src), int(src_type), _ptr(dst), int(dst_type), dst.numel(),
+                                  _stream(stream)), self._h)
 
     def geopar(self, volume, axis=GP_VIEW_X, views=None, chips=None, exhaustive=False, stream=None):
         """GeoParGen (trec_geopar) on a [nz][ny][nx] float32 device volume.

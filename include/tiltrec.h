@@ -21,6 +21,9 @@
  *     (src/TiltRec/src/filter/filter_prj.h:24)
  *   ApplyFilterInplace(modes 0, 1)                      trec_filter_fbp
  *     (filter_prj.cpp:266-446, 738-771)
+ *   MRC element reading loops                           trec_convert
+ *     (src/TiltRec/src/mrcmx/mrcstack.cpp:354-459,
+ *      mrctools/src/mrcimg/mrc2img.cpp:46-126)
  *   --f2b: thrust moments + CufloatToByteKernel         trec_float_to_byte
  *     (TiltRec_cuda_z/recon_z.cu:253-262,
  *      TiltRec_cuda_y/basic.cu:442-474)
@@ -175,6 +178,27 @@ trec_status trec_filter_rweight(trec_context *ctx, float *rows, int nx,
  * mirror-padded row.  Rows up to about 9,900 pixels. */
 trec_status trec_filter_fbp(trec_context *ctx, float *rows, int nx,
                             long long nrows, int mode, void *stream);
+
+/* Element types of MRC data as trec_convert takes them. */
+typedef enum {
+    TREC_DT_I8 = 0,         /* MRC mode 0 as TiltRec reads it: signed bytes */
+    TREC_DT_U8 = 1,         /* unsigned bytes (MRC mode 101 unpacked) */
+    TREC_DT_U8_DIV255 = 2,  /* MRC mode 0 as mrctools / GeoParGen read it:
+                               unsigned bytes / 255.0f */
+    TREC_DT_I16 = 3,        /* MRC mode 1 */
+    TREC_DT_U16 = 4,        /* MRC mode 6 */
+    TREC_DT_F16 = 5,        /* MRC mode 12 */
+    TREC_DT_F32 = 6         /* MRC mode 2 */
+} trec_dtype;
+
+/* Converts n device elements of in_type into out_type (TREC_DT_F32 or
+ * TREC_DT_F16) on the stream: integers and halves to float exactly, bytes
+ * over 255 as the mrctools reader divides them (mrc2img.cpp:60-75), float to
+ * half rounded to nearest even.  Replaces the reference readers' per-element
+ * host loops (mrcstack.cpp:354-459, mrc2img.cpp:46-126), so a CLI uploads
+ * the file's own bytes.  in and out must not overlap. */
+trec_status trec_convert(trec_context *ctx, const void *in, int in_type,
+                         void *out, int out_type, size_t n, void *stream);
 
 /* --f2b byte output (TiltRecZ-cuda, recon_z.cu:253-262; CufloatToByteKernel,
  * basic.cu:442-474): the mean and standard deviation of the n floats of `in`,
