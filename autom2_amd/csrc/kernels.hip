@@ -1135,6 +1135,83 @@ __global__ __launch_bounds__(WG) void k_rweight(float *__restrict__ data, const 
     }
 }
 
+// The same convolution ordered by source cell: y[j] = sum_m h[(j - m) mod N]
+// e[m].  Every lane of the work-group reads the same e[m] (the 4 rows as a
+// float4, one LDS broadcast) and applies it to RW_J outputs j = j0 + 256 k
+// with taps from h2, the kernel written out twice so that (j - m) mod N is
+// j - m + N without a branch: 16 multiply-adds per broadcast read and 4
+// conflict-free tap reads.  LDS: 16 N bytes of rows + 8 N bytes of taps.
+constexpr int RW_J = 4;
+__global__ __launch_bounds__(WG) void k_rweight_m(float *__restrict__ data, const float *__restrict__ hk, int N,
+                                                  int X, long long nrows, long long rstride, int xstride, int rpg,
+                                                  long long gstride)
+{
+    extern __shared__ float smem[];
+    float4 *e = reinterpret_cast<float4 *>(smem);  // [N]
+    float *h2 = smem + 4 * (size_t)N;               // [2 N]
+    const long long row0 = (long long)blockIdx.x * 4;
+    long long base[4];
+    bool live[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const long long k = row0 + q;
+        live[q] = k < nrows;
+        const long long kk = live[q] ? k : row0;
+        base[q] = (kk / rpg) * gstride + (kk % rpg) * rstride;
+    }
+    for (int m = threadIdx.x; m < 2 * N; m += WG) h2[m] = hk[m < N ? m : m - N];
+    const int nxp = N - X + 1;
+    for (int jx = threadIdx.x; jx < N; jx += WG) {
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float o;
+            if (jx < X) {
+                o = data[base[q] + (long long)jx * xstride];
+            } else {
+                const float a = data[base[q]];
+                const float bb = data[base[q] + (long long)(X - 1) * xstride];
+                o = (float(jx - X + 1) / float(nxp)) * a + (float(N - jx) / float(nxp)) * bb;
+            }
+            v[q] = o;
+        }
+        e[jx] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+    __syncthreads();
+    // rows are read into LDS before the barrier above, so writing them in
+    // place below races with nothing (each work-group owns its 4 rows)
+    for (int jb = 0; jb < X; jb += WG * RW_J) {
+        const int j0 = jb + (int)threadIdx.x;
+        float4 acc[RW_J];
+        int hb[RW_J];  // h2 index of tap (j - 0): j + N, clamped for outputs past X
+#pragma unroll
+        for (int k = 0; k < RW_J; ++k) {
+            acc[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            hb[k] = min(j0 + k * WG, X - 1) + N;
+        }
+        for (int m = 0; m < N; ++m) {
+            const float4 em = e[m];
+#pragma unroll
+            for (int k = 0; k < RW_J; ++k) {
+                const float t = h2[hb[k] - m];
+                acc[k].x = fmaf(t, em.x, acc[k].x);
+                acc[k].y = fmaf(t, em.y, acc[k].y);
+                acc[k].z = fmaf(t, em.z, acc[k].z);
+                acc[k].w = fmaf(t, em.w, acc[k].w);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RW_J; ++k) {
+            const int j = j0 + k * WG;
+            if (j >= X) continue;
+            const float out[4] = {acc[k].x, acc[k].y, acc[k].z, acc[k].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (live[q]) data[base[q] + (long long)j * xstride] = out[q];
+        }
+    }
+}
+
 // FBP spatial filter (ApplyFilterInplace modes 0 / 1, filter_prj.cpp:266-446),
 // on four rows per work-group, in place, with the addressing of k_rweight.
 // The reference convolves through OpenCV's DFT; the same circular
@@ -1589,6 +1666,19 @@ hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode
 hipError_t launch_rweight(float *data, const float *hk, int N, int X, long long nrows, long long row_stride,
                           int x_stride, int rows_per_group, long long group_stride, hipStream_t st)
 {
+    const unsigned blocks = (unsigned)((nrows + 3) / 4);
+    const size_t shm = 16 * (size_t)N + 8 * (size_t)N;
+    if (shm <= 160 * 1024) {
+        if (shm > 64 * 1024) {
+            static hipError_t attr = hipFuncSetAttribute((const void *)k_rweight_m,
+                                                         hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (attr != hipSuccess) return attr;
+        }
+        hipLaunchKernelGGL(k_rweight_m, dim3(blocks), dim3(WG), shm, st, data, hk, N, X, nrows, row_stride, x_stride,
+                           rows_per_group, group_stride);
+        return hipGetLastError();
+    }
+    // rows of about 6200 pixels and more: the tap-ordered kernel needs less LDS
     const size_t shmem = sizeof(float) * (size_t)((N + 3) & ~3) + sizeof(float4) * (size_t)N;
     if (shmem > 160 * 1024) return hipErrorInvalidValue;  // rows beyond X ~ 7400 do not fit one CU's LDS
     if (shmem > 64 * 1024) {
@@ -1596,7 +1686,6 @@ hipError_t launch_rweight(float *data, const float *hk, int N, int X, long long 
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (attr != hipSuccess) return attr;
     }
-    const unsigned blocks = (unsigned)((nrows + 3) / 4);
     hipLaunchKernelGGL(k_rweight, dim3(blocks), dim3(WG), shmem, st, data, hk, N, X, nrows, row_stride, x_stride,
                        rows_per_group, group_stride);
     return hipGetLastError();

@@ -390,7 +390,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+        print(f"bench.py measures one GPU: the library runs each series on one device (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
 

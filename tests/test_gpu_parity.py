@@ -1,7 +1,7 @@
 """HIP library vs the CPU oracle on identical seeded inputs.
 
 Tolerances: the plain FP32 form must be within relative L2 1e-5 of the FP32
-restatement of TiltRec-mpi (the north-star bar); the achieved figures are
+restatement of TiltRec-mpi; the achieved figures are
 printed.  The 16-bit storage forms are checked against the FP64 oracle at
 1e-3.  Every call goes through the C ABI (autom2_amd.tiltrec).
 """
@@ -133,14 +133,20 @@ def test_wbp(torch_cuda, trec, ctx, orc):
 
 
 def test_filter_api(torch_cuda, trec, ctx, orc):
+    """Widths through both filter kernels: the source-ordered one (one and
+    two passes of 1024 outputs) and, past about 6200 pixels, the tap-ordered
+    one."""
     rng = np.random.default_rng(8)
-    for X in (64, 100, 256):
-        rows = rng.standard_normal((10, X)).astype(np.float32)
+    for X in (2, 7, 64, 100, 256, 1024, 1100, 2300, 6500):
+        rows = (rng.standard_normal((10, X)) + np.linspace(0, 2, X)).astype(np.float32)
+        print(f"R-weighting X={X}", flush=True)
         t = dev(torch_cuda, rows)
         ctx.filter_rweight(t)
         torch_cuda.cuda.synchronize()
         ref = orc.rweight_f64(rows.astype(np.float64))
-        assert orc.rel_l2(t.cpu().numpy(), ref) < 1e-5
+        e = orc.rel_l2(t.cpu().numpy(), ref)
+        print(f"  rel L2 vs FP64 oracle {e:.3e}")
+        assert e < 1e-5
 
 
 def test_layout_z(torch_cuda, trec, ctx, orc):
