@@ -1404,6 +1404,105 @@ __global__ __launch_bounds__(WG) void k_general_forward(const GeneralJob g, int 
     }
 }
 
+// Integer range [lo, hi] of k with (k - o) in [a, b) (rounding slack eps),
+// clamped to [0, n).
+__device__ __forceinline__ void gp_range(double a, double b, double o, int n, int &lo, int &hi)
+{
+    const double eps = 1e-6 * (1.0 + fabs(a) + fabs(b));
+    const double l = ceil(a + o - eps), h = ceil(b + o + eps) - 1.0;
+    lo = l < 0.0 ? 0 : (l > n ? n : (int)l);
+    hi = h >= n ? n - 1 : (h < -1.0 ? -1 : (int)h);
+}
+
+// The splat above as a gather: pixel (i, j) of tilt th receives from voxel
+// (x, y, z) the weight max(0, 1 - |u - i|) max(0, 1 - |v - j|), which is the
+// splat's (1 - xd) or xd times (1 - yd) or yd, up to float rounding.  Sums
+// run over z, then y, then x ascending with no atomics, so the result is
+// deterministic.  Per layer the rows come from the v condition |v - j| < 1
+// (exactly when v does not depend on x, else from the inverse of the map's
+// (x, y) part on the square's corners); per row, as in k_forward, the first
+// column with u > i - 1 and its distance d0 from the pixel follow from one
+// double evaluation, and candidate c sits at d0 + a1 c.
+__global__ __launch_bounds__(WG) void k_general_gather(const GeneralJob g, int theta0, int ones)
+{
+    const unsigned nib = (g.X + WG - 1) / WG;
+    const int i = (int)(blockIdx.x % nib) * WG + threadIdx.x;
+    const int j = blockIdx.x / nib;
+    if (i >= g.X) return;
+    const int th = theta0 + blockIdx.y;
+    const TiltGeneral &t = g.tg[th];
+    const double a1 = t.a[1], a2 = t.a[2], b1 = t.b[1], b2 = t.b[2];
+    const double det = a1 * b2 - a2 * b1;
+    const double rb2 = b2 != 0.0 ? 1.0 / b2 : 0.0, rdet = det != 0.0 ? 1.0 / det : 0.0;
+    const double ox = (float)t.ox, oy = (float)t.oy;
+    const bool v_of_row = b1 == 0.0;
+    const double aa1 = fabs(a1);
+    // columns a pixel can see in one row: |a1| (x' - x) < 2
+    const int ncand = aa1 > 1e-9 ? (int)ceil(2.0 / aa1 - 1e-6) : g.X;
+    const double ra1 = aa1 > 1e-9 ? 1.0 / a1 : 0.0;
+    const float a1f = (float)a1, b1f = (float)b1;
+    float acc = 0.f;
+    for (int z = 0; z < g.T; ++z) {
+        const double Zc = (double)((float)z - (float)t.oz);
+        const double cu = t.a[0] + t.a[3] * Zc + t.ox;  // u = cu + a1 Xc + a2 Yc
+        const double cv = t.b[0] + t.b[3] * Zc + t.oy;  // v = cv + b1 Xc + b2 Yc
+        const double du = (double)(i - 1) - cu, dv = (double)(j - 1) - cv;
+        int y0, y1;
+        if (v_of_row && b2 != 0.0) {
+            const double p = dv * rb2, q = (dv + 2.0) * rb2;
+            gp_range(fmin(p, q), fmax(p, q), oy, g.Y, y0, y1);
+        } else if (det != 0.0) {
+            const double c = -b1 * rdet, d = a1 * rdet;
+            const double e0 = c * du + d * dv, e1 = c * (du + 2.0) + d * dv, e2 = c * du + d * (dv + 2.0),
+                         e3 = c * (du + 2.0) + d * (dv + 2.0);
+            gp_range(fmin(fmin(e0, e1), fmin(e2, e3)), fmax(fmax(e0, e1), fmax(e2, e3)), oy, g.Y, y0, y1);
+        } else {
+            y0 = 0;
+            y1 = g.Y - 1;
+        }
+        for (int y = y0; y <= y1; ++y) {
+            const double Yc = (double)((float)y - (float)t.oy);
+            const double V0 = cv + b2 * Yc - (double)j;  // v - j at Xc = 0
+            float wyr = 1.f;
+            if (v_of_row) {
+                wyr = fmaxf(0.f, 1.f - fabsf((float)V0));
+                if (wyr == 0.f) continue;  // the row adds nothing to pixel row j
+            }
+            const double U0 = cu + a2 * Yc - (double)i;   // u - i at Xc = 0
+            const float *vrow = g.vol + (g.layout == TREC_LAYOUT_Y ? ((size_t)y * g.T + z) * g.X
+                                                                   : ((size_t)z * g.Y + y) * g.X);
+            // first column xf whose u - i exceeds -1 (a1 > 0) or lies below
+            // +1 (a1 < 0), walking towards growing |d|; d0 = u(xf) - i
+            int xf, step;
+            float d0;
+            if (aa1 > 1e-9) {
+                const double sx = (a1 > 0 ? (-1.0 - U0) : (1.0 - U0)) * ra1 + ox;
+                const double fl = floor(sx);
+                xf = (int)fl + 1;
+                d0 = (float)(U0 + a1 * ((double)xf - ox));
+                step = 1;
+            } else {
+                xf = 0;
+                d0 = (float)U0;
+                step = 1;
+            }
+            const float a1s = aa1 > 1e-9 ? a1f : 0.f;
+            for (int c = 0; c < ncand; c += step) {
+                const int x = xf + c;
+                if ((unsigned)x >= (unsigned)g.X) continue;
+                const float d = fmaf(a1s, (float)c, d0);
+                const float wx = fmaxf(0.f, 1.f - fabsf(d));
+                float wy = wyr;
+                if (!v_of_row) wy = fmaxf(0.f, 1.f - fabsf((float)V0 + b1f * ((float)x - (float)ox)));
+                const float val = ones ? 1.f : vrow[x];
+                acc = fmaf(wx * wy, val, acc);
+            }
+        }
+    }
+    float *out = (ones ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+    out[(size_t)j * g.X + i] = acc;
+}
+
 // rval <- p - rval / rwt (rwt above the threshold), in place.
 __global__ __launch_bounds__(WG) void k_general_residual(const GeneralJob g, int theta0, size_t n)
 {
@@ -1794,8 +1893,13 @@ hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipSt
 
 hipError_t launch_general_forward(const GeneralJob &g, int theta0, int ntheta, bool ones, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_general_forward, dim3(cdiv((size_t)g.X * g.T, WG), ntheta), dim3(WG), 0, st, g, theta0,
-                       ones ? 1 : 0);
+    const size_t nblk = cdiv((size_t)g.X, WG) * (size_t)g.Y;
+    if (nblk <= 0x7fffffffu) {
+        hipLaunchKernelGGL(k_general_gather, dim3((unsigned)nblk, ntheta), dim3(WG), 0, st, g, theta0, ones ? 1 : 0);
+    } else {
+        hipLaunchKernelGGL(k_general_forward, dim3(cdiv((size_t)g.X * g.T, WG), ntheta), dim3(WG), 0, st, g, theta0,
+                           ones ? 1 : 0);
+    }
     return hipGetLastError();
 }
 

@@ -59,28 +59,33 @@ def time_form(trec, ctx, torch, geom, ang, stack_d, storage, iters, steps, warmu
 
 
 def methods_form(trec, ctx, torch, geom, ang, stack_d, reps=3):
-    """Milliseconds per call of the one-pass methods and of ADMM on the same
-    series (stack resident, volume in layout Y)."""
+    """Milliseconds per call of the one-pass methods, of ADMM, and of SIRT x20
+    with a 0.5 degree pitch (the geometry a GeoParGen estimate feeds back,
+    which runs the general path, bilinear in both detector axes), all on the
+    same series (stack resident, volume in layout Y)."""
     vol = torch.empty((geom.ny, geom.thickness, geom.nx), dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
+    pitched = trec.make_geom(geom.nx, geom.ny, geom.nproj, geom.thickness, pitch=0.5)
+    cases = [("BPT", "BPT", geom), ("WBP", "WBP", geom), ("FBP", "FBP", geom),
+             ("ADMM,2,5,0.5,0.001", "ADMM,2,5,0.5,0.001", geom), ("SIRT,20,1.0 pitch 0.5", "SIRT,20,1.0", pitched)]
     res = {}
-    for text in ("BPT", "WBP", "FBP", "ADMM,2,5,0.5,0.001"):
+    for label, text, g in cases:
         m = trec.make_method(text)
-        ctx.reconstruct(geom, ang, stack_d, m, vol, stream=stream)
+        ctx.reconstruct(g, ang, stack_d, m, vol, stream=stream)
         torch.cuda.synchronize()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
         ev[0].record(stream)
         for k in range(reps):
-            ctx.reconstruct(geom, ang, stack_d, m, vol, stream=stream)
+            ctx.reconstruct(g, ang, stack_d, m, vol, stream=stream)
             ev[k + 1].record(stream)
         torch.cuda.synchronize()
         ms = sum(ev[k].elapsed_time(ev[k + 1]) for k in range(reps)) / reps
-        res[text] = {"ms_per_call": round(ms, 3)}
+        res[label] = {"ms_per_call": round(ms, 3)}
         if m.method == trec.ADMM:
             # per outer iteration the operator is applied to the start and
             # twice per CG step (the direction, then the true residual)
-            res[text]["operator_applications"] = m.iterations * (2 * m.cg_iterations + 1)
-        log(f"[bench] {text}: {ms:.2f} ms/call")
+            res[label]["operator_applications"] = m.iterations * (2 * m.cg_iterations + 1)
+        log(f"[bench] {label}: {ms:.2f} ms/call")
     return res
 
 
