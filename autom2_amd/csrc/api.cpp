@@ -659,7 +659,7 @@ hipError_t issue_prep(JobPlan &p, hipStream_t st)
         }
         if (is_iterative(k)) {
             LCHK(launch_zero(q.rwt, P * Y * X, st));
-            LCHK(launch_general_forward(q, 0, (int)P, true, st));
+            LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, true, st));
         }
     }
     return hipSuccess;
@@ -680,13 +680,13 @@ hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
     switch (p.m.method) {
     case TREC_SIRT:
         LCHK(launch_zero(q.rval, P * Y * X, st));
-        LCHK(launch_general_forward(q, 0, (int)P, false, st));
+        LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, false, st));
         LCHK(launch_general_residual(q, 0, (int)P, st));
         return launch_general_back(q, BACK_SIRT, 0, (int)P, q.rval, st);
     case TREC_SART: {
         const int th = s % (int)P;
         LCHK(launch_zero(q.rval + (size_t)th * Y * X, Y * X, st));
-        LCHK(launch_general_forward(q, th, 1, false, st));
+        LCHK(launch_general_forward(q, p.tg.data(), th, 1, false, st));
         LCHK(launch_general_residual(q, th, 1, st));
         return launch_general_back(q, BACK_SART, th, 1, q.rval, st);
     }
@@ -1500,11 +1500,11 @@ trec_status trec_forward_project(trec_context *ctx, const trec_geom *geom, const
     q.layout = TREC_LAYOUT_Y;
     q.rval = rval;
     HIPCHK(ctx, launch_zero(rval, P * Y * X, st));
-    HIPCHK(ctx, launch_general_forward(q, 0, (int)P, false, st));
+    HIPCHK(ctx, launch_general_forward(q, p.tg.data(), 0, (int)P, false, st));
     if (rwt) {
         q.rwt = rwt;
         HIPCHK(ctx, launch_zero(rwt, P * Y * X, st));
-        HIPCHK(ctx, launch_general_forward(q, 0, (int)P, true, st));
+        HIPCHK(ctx, launch_general_forward(q, p.tg.data(), 0, (int)P, true, st));
     }
     return TREC_OK;
 }

@@ -1503,6 +1503,99 @@ __global__ __launch_bounds__(WG) void k_general_gather(const GeneralJob g, int t
     out[(size_t)j * g.X + i] = acc;
 }
 
+// GG_RJ detector rows j0 .. j0 + GG_RJ - 1 of pixel i per thread, for maps whose v
+// does not depend on x (b1 = 0) and grows with y (0 < b2 <= 1): the rows of
+// layer z with |v(y) - j| < 1 lie in ya - 1 .. ya + 2, ya = floor(y*) with
+// v(y*) = j (b2 < 1 can give the outer two a small weight), neighbouring
+// detector rows share three of the four, and a row's column sum
+// (k_general_gather's candidates) is formed once for all of them.  The
+// weights are k_general_gather's; each row's column sum is formed before it
+// is weighted, so the sums round differently.
+constexpr int GG_RJ = 8;
+__device__ __forceinline__ float gg_row(const GeneralJob &g, int y, int z, int i, double cu, double a2, double ra1,
+                                        double a1, float a1f, int ncand, double ox, const TiltGeneral &t, int ones)
+{
+    if ((unsigned)y >= (unsigned)g.Y) return 0.f;
+    const double Yc = (double)((float)y - (float)t.oy);
+    const double U0 = cu + a2 * Yc - (double)i;
+    const double sx = (a1 > 0 ? (-1.0 - U0) : (1.0 - U0)) * ra1 + ox;
+    const int xf = (int)floor(sx) + 1;
+    const float d0 = (float)(U0 + a1 * ((double)xf - ox));
+    const float *vrow = g.vol + (g.layout == TREC_LAYOUT_Y ? ((size_t)y * g.T + z) * g.X : ((size_t)z * g.Y + y) * g.X);
+    float s = 0.f;
+    for (int c = 0; c < ncand; ++c) {
+        const int x = xf + c;
+        if ((unsigned)x >= (unsigned)g.X) continue;
+        const float wx = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
+        s = fmaf(wx, ones ? 1.f : vrow[x], s);
+    }
+    return s;
+}
+
+__global__ __launch_bounds__(WG) void k_general_gather16(const GeneralJob g, int theta0, int ones)
+{
+    const unsigned nib = (g.X + WG - 1) / WG;
+    const int i = (int)(blockIdx.x % nib) * WG + threadIdx.x;
+    const int j0 = (int)(blockIdx.x / nib) * GG_RJ;
+    if (i >= g.X) return;
+    const int th = theta0 + blockIdx.y;
+    const TiltGeneral &t = g.tg[th];
+    const double a1 = t.a[1], a2 = t.a[2], b2 = t.b[2];
+    const double ra1 = 1.0 / a1, rb2 = 1.0 / b2;
+    const int ncand = (int)ceil(2.0 / fabs(a1) - 1e-6);
+    const float a1f = (float)a1, b2f = (float)b2;
+    const double ox = (float)t.ox, oy = (float)t.oy;
+    float acc[GG_RJ];
+#pragma unroll
+    for (int r = 0; r < GG_RJ; ++r) acc[r] = 0.f;
+    for (int z = 0; z < g.T; ++z) {
+        const double Zc = (double)((float)z - (float)t.oz);
+        const double cu = t.a[0] + t.a[3] * Zc + t.ox;
+        const double cv = t.b[0] + t.b[3] * Zc + t.oy;
+        // row sums of ybase .. ybase + 3 (wave-uniform rows)
+        int ybase = -0x3fffffff;
+        float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+#pragma unroll
+        for (int r = 0; r < GG_RJ; ++r) {
+            const int jr = j0 + r;
+            const int ya = (int)floor(((double)jr - cv) * rb2 + oy);  // v(ya) <= jr < v(ya + 1)
+            const int sh = ya - 1 - ybase;
+            if (sh == 1) {
+                c0 = c1;
+                c1 = c2;
+                c2 = c3;
+                c3 = gg_row(g, ya + 2, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
+            } else if (sh == 2) {
+                c0 = c2;
+                c1 = c3;
+                c2 = gg_row(g, ya + 1, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
+                c3 = gg_row(g, ya + 2, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
+            } else if (sh != 0) {
+                c0 = gg_row(g, ya - 1, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
+                c1 = gg_row(g, ya, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
+                c2 = gg_row(g, ya + 1, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
+                c3 = gg_row(g, ya + 2, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
+            }
+            ybase = ya - 1;
+            // v - j of row ya - 1, then the next three rows b2 apart
+            const float d0 = (float)(cv + b2 * (double)((float)(ya - 1) - (float)t.oy) - jr);
+            float wk[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) wk[k] = fmaxf(0.f, 1.f - fabsf(fmaf(b2f, (float)k, d0)));
+            float a = acc[r];
+            a = fmaf(wk[0], c0, a);
+            a = fmaf(wk[1], c1, a);
+            a = fmaf(wk[2], c2, a);
+            a = fmaf(wk[3], c3, a);
+            acc[r] = a;
+        }
+    }
+    float *out = (ones ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+#pragma unroll
+    for (int r = 0; r < GG_RJ; ++r)
+        if (j0 + r < g.Y) out[(size_t)(j0 + r) * g.X + i] = acc[r];
+}
+
 // rval <- p - rval / rwt (rwt above the threshold), in place.
 __global__ __launch_bounds__(WG) void k_general_residual(const GeneralJob g, int theta0, size_t n)
 {
@@ -1891,10 +1984,22 @@ hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_general_forward(const GeneralJob &g, int theta0, int ntheta, bool ones, hipStream_t st)
+hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
+                                  hipStream_t st)
 {
+    // the sixteen-row form needs v independent of x and growing with y, with
+    // columns that move with x (|a1| well above 0), in every tilt launched
+    bool rows16 = true;
+    for (int th = theta0; th < theta0 + ntheta; ++th) {
+        const TiltGeneral &t = host_tg[th];
+        rows16 = rows16 && t.b[1] == 0.0 && t.b[2] > 0.5 && t.b[2] <= 1.0 && fabs(t.a[1]) > 1e-3;
+    }
+    const size_t nblk16 = cdiv((size_t)g.X, WG) * cdiv((size_t)g.Y, GG_RJ);
     const size_t nblk = cdiv((size_t)g.X, WG) * (size_t)g.Y;
-    if (nblk <= 0x7fffffffu) {
+    if (rows16 && nblk16 <= 0x7fffffffu) {
+        hipLaunchKernelGGL(k_general_gather16, dim3((unsigned)nblk16, ntheta), dim3(WG), 0, st, g, theta0,
+                           ones ? 1 : 0);
+    } else if (nblk <= 0x7fffffffu) {
         hipLaunchKernelGGL(k_general_gather, dim3((unsigned)nblk, ntheta), dim3(WG), 0, st, g, theta0, ones ? 1 : 0);
     } else {
         hipLaunchKernelGGL(k_general_forward, dim3(cdiv((size_t)g.X * g.T, WG), ntheta), dim3(WG), 0, st, g, theta0,

@@ -165,7 +165,9 @@ struct GeneralJob {
     float *rval;            // [P][Y][X]
     float *rwt;             // [P][Y][X]
 };
-hipError_t launch_general_forward(const GeneralJob &g, int theta0, int ntheta, bool ones, hipStream_t st);
+// host_tg: the same tilt table on the host, for the choice of kernel
+hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
+                                  hipStream_t st);
 hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, hipStream_t st);
 hipError_t launch_general_back(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img,
                                hipStream_t st);
