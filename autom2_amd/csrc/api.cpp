@@ -682,16 +682,16 @@ hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
         LCHK(launch_zero(q.rval, P * Y * X, st));
         LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, false, st));
         LCHK(launch_general_residual(q, 0, (int)P, st));
-        return launch_general_back(q, BACK_SIRT, 0, (int)P, q.rval, st);
+        return launch_general_back(q, p.tg.data(), BACK_SIRT, 0, (int)P, q.rval, st);
     case TREC_SART: {
         const int th = s % (int)P;
         LCHK(launch_zero(q.rval + (size_t)th * Y * X, Y * X, st));
         LCHK(launch_general_forward(q, p.tg.data(), th, 1, false, st));
         LCHK(launch_general_residual(q, th, 1, st));
-        return launch_general_back(q, BACK_SART, th, 1, q.rval, st);
+        return launch_general_back(q, p.tg.data(), BACK_SART, th, 1, q.rval, st);
     }
     default:
-        return launch_general_back(q, BACK_BPT, 0, (int)P, is_filtered(p.m.method) ? p.stack_copy : p.stack, st);
+        return launch_general_back(q, p.tg.data(), BACK_BPT, 0, (int)P, is_filtered(p.m.method) ? p.stack_copy : p.stack, st);
     }
 }
 

@@ -1,0 +1,537 @@
+// general.hip -- the general path of libtiltrec_hip.so (any pitch, offset or
+// steep tilt), on the natural layouts: stack [P][Y][X], volume [Y][T][X] or
+// [T][Y][X].
+//
+// TiltRec maps voxel (x, y, z) of tilt th to u = a0 + a1 X + a2 Y + a3 Z + ox,
+// v = b0 + b1 X + b2 Y + b3 Z + oy, X = x - ox etc. (ValCoef,
+// main-mpi.cpp:204-226), splats it bilinearly in Reproject (main-mpi.cpp:
+// 286-344) and gathers it bilinearly in the back-projection (main-mpi.cpp:
+// 378-450).  Its b is (0, 0, 1, 0) turned by the pitch and offset only
+// (DecorateCoefficients, main-mpi.cpp:85-183), so every tilt maps rows the
+// same way; the fast kernels below rest on that:
+//
+//   k_general_rows        forward projection when v does not depend on x
+//                         (offset 0 or pitch 0): the splat factors into an
+//                         x pass per volume row and a y pass over rows.
+//   k_general_back_rows   back-projection: v and its two detector rows are
+//                         evaluated once per voxel, u once per voxel and tilt.
+//
+// Geometries outside their conditions use the reference-order kernels
+// (k_general_gather, k_general_back).  Every kernel is deterministic.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "trec_internal.h"
+
+namespace trec {
+
+static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+__device__ __forceinline__ float hat(float d) { return fmaxf(0.f, 1.f - fabsf(d)); }
+
+// ------------------------------------------------------------------ reference order
+
+// Exact ValCoef for any pitch (main-mpi.cpp:204-226).
+__device__ __forceinline__ void general_map(const TiltGeneral &t, int x, int y, int z, int &xm, int &ym, float &xd,
+                                            float &yd)
+{
+    const double Xc = (double)((float)x - (float)t.ox);
+    const double Yc = (double)((float)y - (float)t.oy);
+    const double Zc = (double)((float)z - (float)t.oz);
+    const double n0 = __dadd_rn(__dadd_rn(__dadd_rn(t.a[0], __dmul_rn(t.a[1], Xc)), __dmul_rn(t.a[2], Yc)),
+                                __dmul_rn(t.a[3], Zc));
+    const double n1 = __dadd_rn(__dadd_rn(__dadd_rn(t.b[0], __dmul_rn(t.b[1], Xc)), __dmul_rn(t.b[2], Yc)),
+                                __dmul_rn(t.b[3], Zc));
+    const double u = __dadd_rn(n0, t.ox), v = __dadd_rn(n1, t.oy);
+    const double fu = floor(u), fv = floor(v);
+    xm = (int)fu;
+    ym = (int)fv;
+    xd = (float)(u - fu);
+    yd = (float)(v - fv);
+}
+
+// Splat of one voxel column (x, z) over all rows into rval (or rwt with ones),
+// CuReprojectKernel / CuReWeightKernel semantics with float atomics.
+__global__ __launch_bounds__(WG) void k_general_forward(const GeneralJob g, int theta0, int ones)
+{
+    const int idx = blockIdx.x * WG + threadIdx.x;
+    if (idx >= g.X * g.T) return;
+    const int th = theta0 + blockIdx.y;
+    const int z = idx / g.X, x = idx % g.X;
+    const TiltGeneral &t = g.tg[th];
+    float *out = (ones ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+    for (int y = 0; y < g.Y; ++y) {
+        const size_t vi = g.layout == TREC_LAYOUT_Y ? ((size_t)y * g.T + z) * g.X + x : ((size_t)z * g.Y + y) * g.X + x;
+        const float v = ones ? 1.f : g.vol[vi];
+        int xm, ym;
+        float xd, yd;
+        general_map(t, x, y, z, xm, ym, xd, yd);
+        if (xm >= 0 && xm < g.X && ym >= 0 && ym < g.Y) atomicAdd(out + (size_t)ym * g.X + xm, (1 - xd) * (1 - yd) * v);
+        if (xm + 1 >= 0 && xm + 1 < g.X && ym >= 0 && ym < g.Y) atomicAdd(out + (size_t)ym * g.X + xm + 1, xd * (1 - yd) * v);
+        if (xm >= 0 && xm < g.X && ym + 1 >= 0 && ym + 1 < g.Y) atomicAdd(out + (size_t)(ym + 1) * g.X + xm, (1 - xd) * yd * v);
+        if (xm + 1 >= 0 && xm + 1 < g.X && ym + 1 >= 0 && ym + 1 < g.Y)
+            atomicAdd(out + (size_t)(ym + 1) * g.X + xm + 1, xd * yd * v);
+    }
+}
+
+// Integer range [lo, hi] of k with (k - o) in [a, b) (rounding slack eps),
+// clamped to [0, n).
+__device__ __forceinline__ void gp_range(double a, double b, double o, int n, int &lo, int &hi)
+{
+    const double eps = 1e-6 * (1.0 + fabs(a) + fabs(b));
+    const double l = ceil(a + o - eps), h = ceil(b + o + eps) - 1.0;
+    lo = l < 0.0 ? 0 : (l > n ? n : (int)l);
+    hi = h >= n ? n - 1 : (h < -1.0 ? -1 : (int)h);
+}
+
+// The splat above as a gather: pixel (i, j) of tilt th receives from voxel
+// (x, y, z) the weight max(0, 1 - |u - i|) max(0, 1 - |v - j|), which is the
+// splat's (1 - xd) or xd times (1 - yd) or yd, up to float rounding.  Sums
+// run over z, then y, then x ascending with no atomics, so the result is
+// deterministic.  Per layer the rows come from the v condition |v - j| < 1
+// (exactly when v does not depend on x, else from the inverse of the map's
+// (x, y) part on the square's corners); per row, as in k_forward, the first
+// column with u > i - 1 and its distance d0 from the pixel follow from one
+// double evaluation, and candidate c sits at d0 + a1 c.
+__global__ __launch_bounds__(WG) void k_general_gather(const GeneralJob g, int theta0, int ones)
+{
+    const unsigned nib = (g.X + WG - 1) / WG;
+    const int i = (int)(blockIdx.x % nib) * WG + threadIdx.x;
+    const int j = blockIdx.x / nib;
+    if (i >= g.X) return;
+    const int th = theta0 + blockIdx.y;
+    const TiltGeneral &t = g.tg[th];
+    const double a1 = t.a[1], a2 = t.a[2], b1 = t.b[1], b2 = t.b[2];
+    const double det = a1 * b2 - a2 * b1;
+    const double rb2 = b2 != 0.0 ? 1.0 / b2 : 0.0, rdet = det != 0.0 ? 1.0 / det : 0.0;
+    const double ox = (float)t.ox, oy = (float)t.oy;
+    const bool v_of_row = b1 == 0.0;
+    const double aa1 = fabs(a1);
+    // columns a pixel can see in one row: |a1| (x' - x) < 2
+    const int ncand = aa1 > 1e-9 ? (int)ceil(2.0 / aa1 - 1e-6) : g.X;
+    const double ra1 = aa1 > 1e-9 ? 1.0 / a1 : 0.0;
+    const float a1f = (float)a1, b1f = (float)b1;
+    float acc = 0.f;
+    for (int z = 0; z < g.T; ++z) {
+        const double Zc = (double)((float)z - (float)t.oz);
+        const double cu = t.a[0] + t.a[3] * Zc + t.ox;  // u = cu + a1 Xc + a2 Yc
+        const double cv = t.b[0] + t.b[3] * Zc + t.oy;  // v = cv + b1 Xc + b2 Yc
+        const double du = (double)(i - 1) - cu, dv = (double)(j - 1) - cv;
+        int y0, y1;
+        if (v_of_row && b2 != 0.0) {
+            const double p = dv * rb2, q = (dv + 2.0) * rb2;
+            gp_range(fmin(p, q), fmax(p, q), oy, g.Y, y0, y1);
+        } else if (det != 0.0) {
+            const double c = -b1 * rdet, d = a1 * rdet;
+            const double e0 = c * du + d * dv, e1 = c * (du + 2.0) + d * dv, e2 = c * du + d * (dv + 2.0),
+                         e3 = c * (du + 2.0) + d * (dv + 2.0);
+            gp_range(fmin(fmin(e0, e1), fmin(e2, e3)), fmax(fmax(e0, e1), fmax(e2, e3)), oy, g.Y, y0, y1);
+        } else {
+            y0 = 0;
+            y1 = g.Y - 1;
+        }
+        for (int y = y0; y <= y1; ++y) {
+            const double Yc = (double)((float)y - (float)t.oy);
+            const double V0 = cv + b2 * Yc - (double)j;  // v - j at Xc = 0
+            float wyr = 1.f;
+            if (v_of_row) {
+                wyr = fmaxf(0.f, 1.f - fabsf((float)V0));
+                if (wyr == 0.f) continue;  // the row adds nothing to pixel row j
+            }
+            const double U0 = cu + a2 * Yc - (double)i;   // u - i at Xc = 0
+            const float *vrow = g.vol + (g.layout == TREC_LAYOUT_Y ? ((size_t)y * g.T + z) * g.X
+                                                                   : ((size_t)z * g.Y + y) * g.X);
+            // first column xf whose u - i exceeds -1 (a1 > 0) or lies below
+            // +1 (a1 < 0), walking towards growing |d|; d0 = u(xf) - i
+            int xf, step;
+            float d0;
+            if (aa1 > 1e-9) {
+                const double sx = (a1 > 0 ? (-1.0 - U0) : (1.0 - U0)) * ra1 + ox;
+                const double fl = floor(sx);
+                xf = (int)fl + 1;
+                d0 = (float)(U0 + a1 * ((double)xf - ox));
+                step = 1;
+            } else {
+                xf = 0;
+                d0 = (float)U0;
+                step = 1;
+            }
+            const float a1s = aa1 > 1e-9 ? a1f : 0.f;
+            for (int c = 0; c < ncand; c += step) {
+                const int x = xf + c;
+                if ((unsigned)x >= (unsigned)g.X) continue;
+                const float d = fmaf(a1s, (float)c, d0);
+                const float wx = fmaxf(0.f, 1.f - fabsf(d));
+                float wy = wyr;
+                if (!v_of_row) wy = fmaxf(0.f, 1.f - fabsf((float)V0 + b1f * ((float)x - (float)ox)));
+                const float val = ones ? 1.f : vrow[x];
+                acc = fmaf(wx * wy, val, acc);
+            }
+        }
+    }
+    float *out = (ones ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+    out[(size_t)j * g.X + i] = acc;
+}
+
+// rval <- p - rval / rwt (rwt above the threshold), in place.
+__global__ __launch_bounds__(WG) void k_general_residual(const GeneralJob g, int theta0, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+    if (i >= n) return;
+    const size_t k = (size_t)theta0 * g.Y * g.X + i;
+    float r = g.rval[k];
+    const float w = g.rwt[k];
+    if (w > g.thr_res) r /= w;  // weights are >= 0: thr_res = 0 is the reference's w != 0
+    g.rval[k] = g.stack[k] - r;
+}
+
+__device__ __forceinline__ void general_bilinear(const float *img, int X, int Y, int xm, int ym, float xd, float yd,
+                                                 float &s, float &c)
+{
+    if (xm >= 0 && xm < X && ym >= 0 && ym < Y) {
+        s += (1 - xd) * (1 - yd) * img[(size_t)ym * X + xm];
+        c += (1 - xd) * (1 - yd);
+    }
+    if (xm + 1 >= 0 && xm + 1 < X && ym >= 0 && ym < Y) {
+        s += xd * (1 - yd) * img[(size_t)ym * X + xm + 1];
+        c += xd * (1 - yd);
+    }
+    if (xm >= 0 && xm < X && ym + 1 >= 0 && ym + 1 < Y) {
+        s += (1 - xd) * yd * img[(size_t)(ym + 1) * X + xm];
+        c += (1 - xd) * yd;
+    }
+    if (xm + 1 >= 0 && xm + 1 < X && ym + 1 >= 0 && ym + 1 < Y) {
+        s += xd * yd * img[(size_t)(ym + 1) * X + xm + 1];
+        c += xd * yd;
+    }
+}
+
+// Voxel-driven gather, BackMode semantics, natural layouts.
+__global__ __launch_bounds__(WG) void k_general_back(const GeneralJob g, int mode, int theta0, int ntheta,
+                                                     const float *__restrict__ img)
+{
+    const size_t idx0 = (size_t)blockIdx.x * WG + threadIdx.x;
+    if (idx0 >= (size_t)g.X * g.T * g.Y) return;
+    const int x = idx0 % g.X;
+    const int z = (idx0 / g.X) % g.T;
+    const int y = idx0 / ((size_t)g.X * g.T);
+    const size_t idx = g.layout == TREC_LAYOUT_Y ? idx0 : ((size_t)z * g.Y + y) * g.X + x;
+    float val = 0.f, wt = 0.f;
+    float out = g.vol[idx];
+    for (int th = theta0; th < theta0 + ntheta; ++th) {
+        int xm, ym;
+        float xd, yd;
+        general_map(g.tg[th], x, y, z, xm, ym, xd, yd);
+        float s = 0.f, c = 0.f;
+        general_bilinear(img + (size_t)th * g.Y * g.X, g.X, g.Y, xm, ym, xd, yd, s, c);
+        if (mode == BACK_SIRT) {
+            val += s;
+            wt += c;
+        } else if (c > g.thr_upd) {
+            out += (mode == BACK_SART) ? (s / c) * g.gamma : s / c;
+        }
+    }
+    if (mode == BACK_SIRT) {
+        if (wt > g.thr_upd) out += val / wt * g.gamma;
+    }
+    g.vol[idx] = out;
+}
+
+
+// ------------------------------------------------------------------ row windows
+
+// Forward projection for tilts whose v = cv(z) + b2 Y does not depend on x
+// (b1 = 0).  The splat's weight hat(u - i) hat(v - j), hat(d) = max(0,
+// 1 - |d|), then factors:
+//
+//   R[j][i] = sum_z sum_y hat(v(y, z) - j) S(y, z, i),
+//   S(y, z, i) = sum_x hat(u(x, y, z) - i) V[x, y, z],
+//
+// an x pass per volume row (u moves with y through a2) and a y pass whose
+// weights do not depend on i.  A thread owns detector rows j0 .. j0 + RJ - 1
+// of pixel i of one tilt.  With ya(j) = floor(y*), v(y*) = j, the rows of a
+// layer that reach row j are ya(j) - 1 .. ya(j) + 2 (b2 >= 1/2), and for
+// (RJ - 1) / RJ <= b2 <= 1, ya(j0 + r) - ya(j0) is r or r + 1: the rows
+// ya(j0) - 1 + k, k < RJ + 4 ("slots"), cover every detector row, and slot
+// k reaches only rows r in [k - 4, k].
+//
+// Each slot's S is formed from NC candidate columns starting at the first
+// with |u - i| < 1 (clamped into the row, so every load is in bounds; a
+// column that is no candidate gets hat = 0).  The y weights hat(v - j0 - r)
+// are the same in every lane: per layer the wave evaluates the 5 (RJ + 4)
+// slot/row pairs once, one per lane, and each slot reads its five from the
+// lanes that hold them (zero for rows it does not reach and for slots
+// outside the volume).  Sums run over z, then slots in y order, then columns
+// in x order.  Work-groups are ordered tilt first, so those in flight
+// together read the same rows of the volume.
+template <int RJ, int NC, bool ONES>
+__global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int theta0, int ntheta)
+{
+    constexpr int NS = RJ + 4;
+    constexpr int NE = 5 * NS;             // slot/row pairs: slot q / 5, row q / 5 - 4 + q % 5
+    constexpr int NH = (NE + 63) / 64;     // lane registers holding them
+    constexpr int SB = 5;                  // slots whose loads may be in flight together
+    const unsigned nib = (g.X + WG - 1) / WG;
+    const unsigned lin = blockIdx.x;
+    const int th = theta0 + (int)(lin % (unsigned)ntheta);
+    const unsigned rest = lin / (unsigned)ntheta;
+    // every lane stays to the end: the wave's lanes hold the y weights of all
+    // of them; pixels beyond the row repeat its last one and are not stored
+    const int i = (int)(rest % nib) * WG + threadIdx.x;
+    const int ic = min(i, g.X - 1);
+    const int j0 = (int)(rest / nib) * RJ;
+    const TiltGeneral &t = g.tg[th];
+    const double a1 = t.a[1], a2 = t.a[2], b2 = t.b[2];
+    const double ra1 = 1.0 / a1, rb2 = 1.0 / b2;
+    const double ox = (float)t.ox, oy = (float)t.oy;
+    // u - i = W + a1 x with W = cu - i + a2 Y - a1 ox; the first candidate is
+    // the least x with (W + a1 x - s1) / a1 > 0
+    const double s1ra1 = (a1 > 0.0 ? -1.0 : 1.0) * ra1;
+    const float a1f = (float)a1;
+    const int xmax = g.X - NC;
+    // volume strides of a row and a layer
+    const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
+    const size_t sz = g.layout == TREC_LAYOUT_Y ? (size_t)g.X : (size_t)g.Y * g.X;
+    // this lane's slot/row pairs
+    const int lane = (int)(threadIdx.x & 63);
+    int pk[NH];
+    double pkd[NH], pj[NH];
+    float pm[NH];
+    bool prow[NH];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+        const int q = 64 * h + lane, k = q / 5, r = k - 4 + q % 5;
+        pk[h] = k;
+        pkd[h] = (double)k;
+        pj[h] = (double)(j0 + k - 2);
+        pm[h] = (float)(r - k + 2);
+        prow[h] = q < NE && r >= 0 && r < RJ;
+    }
+    float acc[RJ];
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) acc[r] = 0.f;
+    for (int z = 0; z < g.T; ++z) {
+        const double Zc = (double)((float)z - (float)t.oz);
+        const double cu = t.a[0] + t.a[3] * Zc + t.ox;  // u = cu + a1 X + a2 Y
+        const double cv = t.b[0] + t.b[3] * Zc + t.oy;  // v = cv + b2 Y
+        const int ylo = (int)floor(((double)j0 - cv) * rb2 + oy) - 1;
+        const double Yc0 = (double)((float)ylo - (float)t.oy);
+        // y weights: v - (j0 + k - 2) of slot k, then hat of its distance to row r
+        float pw[NH];
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+            const float e = (float)((b2 * (Yc0 + pkd[h]) + cv) - pj[h]);
+            pw[h] = prow[h] && (unsigned)(ylo + pk[h]) < (unsigned)g.Y ? hat(e - pm[h]) : 0.f;
+        }
+        const double W0 = cu - (double)ic + a2 * Yc0 - a1 * ox;
+        const float *vlayer = g.vol + (size_t)z * sz;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const double W = a2 * (double)k + W0;
+            const int xf = (int)floor(s1ra1 - W * ra1) + 1;
+            const int xs = min(max(xf, 0), xmax);
+            const float d0 = (float)(a1 * (double)xs + W);  // u - i at column xs
+            float s = 0.f;
+            if (ONES) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s += hat(fmaf(a1f, (float)c, d0));
+            } else {
+                const int yc = min(max(ylo + k, 0), g.Y - 1);
+                const float *p = vlayer + (size_t)yc * sy + xs;
+                float v[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) v[c] = p[c];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s = fmaf(hat(fmaf(a1f, (float)c, d0)), v[c], s);
+            }
+#pragma unroll
+            for (int r = (k > 4 ? k - 4 : 0); r <= (k < RJ - 1 ? k : RJ - 1); ++r) {
+                const int q = 5 * k + r - k + 4;
+                const float w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pw[q / 64]), q % 64));
+                acc[r] = fmaf(w, s, acc[r]);
+            }
+            if ((k + 1) % SB == 0) __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (i >= g.X) return;
+    float *out = (ONES ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+#pragma unroll
+    for (int r = 0; r < RJ; ++r)
+        if (j0 + r < g.Y) out[(size_t)(j0 + r) * g.X + i] = acc[r];
+}
+
+// Back-projection for tilts that share b and the origins.  Each thread takes
+// GB_Q voxels 64 apart along x in one (y, z) row of the volume (one row per
+// wave); v, its detector rows ym, ym + 1 and their weights are evaluated
+// once per voxel with ValCoef's operation order, and per tilt only u, its
+// columns xm, xm + 1 and their weights.  Taps outside the projection get
+// weight 0 (their loads are clamped into it), as the reference skips them.
+constexpr int GB_Q = 2;
+__global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, int mode, int theta0, int ntheta,
+                                                          const float *__restrict__ img)
+{
+    const unsigned nxt = (g.X + GB_Q * 64 - 1) / (GB_Q * 64);
+    const unsigned row = __builtin_amdgcn_readfirstlane((blockIdx.x / nxt) * (WG / 64) + threadIdx.x / 64);
+    if (row >= (unsigned)g.Y * (unsigned)g.T) return;
+    const int xb = (int)(blockIdx.x % nxt) * GB_Q * 64 + (int)(threadIdx.x & 63);
+    const int y = g.layout == TREC_LAYOUT_Y ? (int)(row / g.T) : (int)(row % g.Y);
+    const int z = g.layout == TREC_LAYOUT_Y ? (int)(row % g.T) : (int)(row / g.Y);
+    const TiltGeneral &t0 = g.tg[theta0];
+    const double Yc = (double)((float)y - (float)t0.oy);
+    const double Zc = (double)((float)z - (float)t0.oz);
+    const unsigned X = g.X, Y = g.Y;
+    double Xc[GB_Q];
+    float wy0[GB_Q], wy1[GB_Q], val[GB_Q], wt[GB_Q], out[GB_Q];
+    unsigned r0[GB_Q], r1[GB_Q];
+    float *vout = g.vol + (size_t)row * g.X;
+#pragma unroll
+    for (int q = 0; q < GB_Q; ++q) {
+        const int x = xb + 64 * q;
+        Xc[q] = (double)((float)x - (float)t0.ox);
+        const double n1 = __dadd_rn(__dadd_rn(__dadd_rn(t0.b[0], __dmul_rn(t0.b[1], Xc[q])), __dmul_rn(t0.b[2], Yc)),
+                                    __dmul_rn(t0.b[3], Zc));
+        const double v = __dadd_rn(n1, t0.oy);
+        const double fv = floor(v);
+        const int ym = (int)fv;
+        const float yd = (float)(v - fv);
+        wy0[q] = (unsigned)ym < Y ? 1.f - yd : 0.f;
+        wy1[q] = (unsigned)(ym + 1) < Y ? yd : 0.f;
+        r0[q] = (unsigned)min(max(ym, 0), (int)Y - 1) * X;
+        r1[q] = (unsigned)min(max(ym + 1, 0), (int)Y - 1) * X;
+        val[q] = 0.f;
+        wt[q] = 0.f;
+        out[q] = x < g.X ? vout[x] : 0.f;
+    }
+    for (int th = theta0; th < theta0 + ntheta; ++th) {
+        const TiltGeneral &t = g.tg[th];
+        const double ub = t.a[0] + t.a[2] * Yc + t.a[3] * Zc + t.ox;  // u = ub + a1 X
+        const double a1 = t.a[1];
+        const float *im = img + (size_t)th * Y * X;
+#pragma unroll
+        for (int q = 0; q < GB_Q; ++q) {
+            const double u = a1 * Xc[q] + ub;
+            const double fu = floor(u);
+            const int xm = (int)fu;
+            const float xd = (float)(u - fu);
+            const float wx0 = (unsigned)xm < X ? 1.f - xd : 0.f;
+            const float wx1 = (unsigned)(xm + 1) < X ? xd : 0.f;
+            const unsigned c0 = (unsigned)min(max(xm, 0), (int)X - 1);
+            const unsigned c1 = (unsigned)min(max(xm + 1, 0), (int)X - 1);
+            const float i00 = im[r0[q] + c0], i01 = im[r0[q] + c1];
+            const float i10 = im[r1[q] + c0], i11 = im[r1[q] + c1];
+            const float s = wy0[q] * (wx0 * i00 + wx1 * i01) + wy1[q] * (wx0 * i10 + wx1 * i11);
+            const float c = (wx0 + wx1) * (wy0[q] + wy1[q]);
+            if (mode == BACK_SIRT) {
+                val[q] += s;
+                wt[q] += c;
+            } else if (c > g.thr_upd) {
+                out[q] += (mode == BACK_SART) ? (s / c) * g.gamma : s / c;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < GB_Q; ++q) {
+        const int x = xb + 64 * q;
+        if (mode == BACK_SIRT && wt[q] > g.thr_upd) out[q] += val[q] / wt[q] * g.gamma;
+        if (x < g.X) vout[x] = out[q];
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+namespace {
+
+bool same_rows(const TiltGeneral &a, const TiltGeneral &b)
+{
+    return std::memcmp(a.b, b.b, sizeof a.b) == 0 && a.ox == b.ox && a.oy == b.oy && a.oz == b.oz;
+}
+
+template <int RJ, int NC>
+void launch_rows(const GeneralJob &g, int theta0, int ntheta, bool ones, unsigned nblk, hipStream_t st)
+{
+    if (ones)
+        hipLaunchKernelGGL((k_general_rows<RJ, NC, true>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta);
+    else
+        hipLaunchKernelGGL((k_general_rows<RJ, NC, false>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta);
+}
+
+template <int RJ>
+void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, int nc, unsigned nblk, hipStream_t st)
+{
+    switch (nc) {
+    case 2: launch_rows<RJ, 2>(g, theta0, ntheta, ones, nblk, st); break;
+    case 3: launch_rows<RJ, 3>(g, theta0, ntheta, ones, nblk, st); break;
+    case 4: launch_rows<RJ, 4>(g, theta0, ntheta, ones, nblk, st); break;
+    default: launch_rows<RJ, 6>(g, theta0, ntheta, ones, nblk, st); break;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
+                                  hipStream_t st)
+{
+    // row windows: v independent of x and the same in every tilt launched,
+    // 7/8 <= b2 <= 1, at most 6 candidate columns (|a1| >= 1/3)
+    bool rows = ntheta > 0;
+    int nc = 2;
+    double b2min = 1.0;
+    for (int th = theta0; th < theta0 + ntheta && rows; ++th) {
+        const TiltGeneral &t = host_tg[th];
+        const double aa1 = fabs(t.a[1]);
+        rows = t.b[1] == 0.0 && t.b[2] <= 1.0 && same_rows(t, host_tg[theta0]) && aa1 >= 1.0 / 3.0;
+        if (!rows) break;
+        nc = std::max(nc, (int)std::ceil(2.0 / aa1 - 1e-6));
+        b2min = std::min(b2min, t.b[2]);
+    }
+    nc = nc <= 4 ? nc : 6;
+    const int rj = b2min >= 15.0 / 16.0 ? 16 : (b2min >= 7.0 / 8.0 ? 8 : 0);
+    rows = rows && rj > 0 && g.X >= nc;
+    const size_t nib = cdiv((size_t)g.X, WG);
+    if (rows) {
+        const size_t nblk = nib * cdiv((size_t)g.Y, rj) * (size_t)ntheta;
+        if (nblk <= 0x7fffffffu) {
+            if (rj == 16) launch_rows_nc<16>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
+            else launch_rows_nc<8>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
+            return hipGetLastError();
+        }
+    }
+    const size_t nblk = nib * (size_t)g.Y;
+    if (nblk <= 0x7fffffffu) {
+        hipLaunchKernelGGL(k_general_gather, dim3((unsigned)nblk, ntheta), dim3(WG), 0, st, g, theta0, ones ? 1 : 0);
+    } else {
+        hipLaunchKernelGGL(k_general_forward, dim3(cdiv((size_t)g.X * g.T, WG), ntheta), dim3(WG), 0, st, g, theta0,
+                           ones ? 1 : 0);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, hipStream_t st)
+{
+    const size_t n = (size_t)ntheta * g.Y * g.X;
+    hipLaunchKernelGGL(k_general_residual, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, int mode, int theta0, int ntheta,
+                               const float *img, hipStream_t st)
+{
+    bool rows = ntheta > 0 && (size_t)g.X * g.Y <= 0x80000000u;
+    for (int th = theta0; th < theta0 + ntheta && rows; ++th) rows = same_rows(host_tg[th], host_tg[theta0]);
+    const size_t nrows_wg = cdiv((size_t)g.Y * g.T, WG / 64);
+    const size_t nblk = cdiv((size_t)g.X, GB_Q * 64) * nrows_wg;
+    if (rows && nblk <= 0x7fffffffu) {
+        hipLaunchKernelGGL(k_general_back_rows, dim3((unsigned)nblk), dim3(WG), 0, st, g, mode, theta0, ntheta, img);
+        return hipGetLastError();
+    }
+    const size_t n = (size_t)g.X * g.T * g.Y;
+    hipLaunchKernelGGL(k_general_back, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, mode, theta0, ntheta, img);
+    return hipGetLastError();
+}
+
+}  // namespace trec

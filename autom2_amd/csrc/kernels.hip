@@ -20,8 +20,7 @@
 //             the SIRT update (basic.cu:177-190) or the BPT / SART updates
 //             (basic.cu:383-415, 417-438).
 //
-// General path (any pitch): natural layouts, 4-tap bilinear splat and
-// gather as the reference does it, float atomics in the splat.
+// The general path (any pitch) is in general.hip.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -1359,307 +1358,6 @@ __global__ __launch_bounds__(WG) void k_sumsq(const T *__restrict__ a, size_t n,
     if (threadIdx.x == 0) atomicAdd(out, red[0]);
 }
 
-// ------------------------------------------------------------------ general path
-
-// Exact ValCoef for any pitch (main-mpi.cpp:204-226).
-__device__ __forceinline__ void general_map(const TiltGeneral &t, int x, int y, int z, int &xm, int &ym, float &xd,
-                                            float &yd)
-{
-    const double Xc = (double)((float)x - (float)t.ox);
-    const double Yc = (double)((float)y - (float)t.oy);
-    const double Zc = (double)((float)z - (float)t.oz);
-    const double n0 = __dadd_rn(__dadd_rn(__dadd_rn(t.a[0], __dmul_rn(t.a[1], Xc)), __dmul_rn(t.a[2], Yc)),
-                                __dmul_rn(t.a[3], Zc));
-    const double n1 = __dadd_rn(__dadd_rn(__dadd_rn(t.b[0], __dmul_rn(t.b[1], Xc)), __dmul_rn(t.b[2], Yc)),
-                                __dmul_rn(t.b[3], Zc));
-    const double u = __dadd_rn(n0, t.ox), v = __dadd_rn(n1, t.oy);
-    const double fu = floor(u), fv = floor(v);
-    xm = (int)fu;
-    ym = (int)fv;
-    xd = (float)(u - fu);
-    yd = (float)(v - fv);
-}
-
-// Splat of one voxel column (x, z) over all rows into rval (or rwt with ones),
-// CuReprojectKernel / CuReWeightKernel semantics with float atomics.
-__global__ __launch_bounds__(WG) void k_general_forward(const GeneralJob g, int theta0, int ones)
-{
-    const int idx = blockIdx.x * WG + threadIdx.x;
-    if (idx >= g.X * g.T) return;
-    const int th = theta0 + blockIdx.y;
-    const int z = idx / g.X, x = idx % g.X;
-    const TiltGeneral &t = g.tg[th];
-    float *out = (ones ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
-    for (int y = 0; y < g.Y; ++y) {
-        const size_t vi = g.layout == TREC_LAYOUT_Y ? ((size_t)y * g.T + z) * g.X + x : ((size_t)z * g.Y + y) * g.X + x;
-        const float v = ones ? 1.f : g.vol[vi];
-        int xm, ym;
-        float xd, yd;
-        general_map(t, x, y, z, xm, ym, xd, yd);
-        if (xm >= 0 && xm < g.X && ym >= 0 && ym < g.Y) atomicAdd(out + (size_t)ym * g.X + xm, (1 - xd) * (1 - yd) * v);
-        if (xm + 1 >= 0 && xm + 1 < g.X && ym >= 0 && ym < g.Y) atomicAdd(out + (size_t)ym * g.X + xm + 1, xd * (1 - yd) * v);
-        if (xm >= 0 && xm < g.X && ym + 1 >= 0 && ym + 1 < g.Y) atomicAdd(out + (size_t)(ym + 1) * g.X + xm, (1 - xd) * yd * v);
-        if (xm + 1 >= 0 && xm + 1 < g.X && ym + 1 >= 0 && ym + 1 < g.Y)
-            atomicAdd(out + (size_t)(ym + 1) * g.X + xm + 1, xd * yd * v);
-    }
-}
-
-// Integer range [lo, hi] of k with (k - o) in [a, b) (rounding slack eps),
-// clamped to [0, n).
-__device__ __forceinline__ void gp_range(double a, double b, double o, int n, int &lo, int &hi)
-{
-    const double eps = 1e-6 * (1.0 + fabs(a) + fabs(b));
-    const double l = ceil(a + o - eps), h = ceil(b + o + eps) - 1.0;
-    lo = l < 0.0 ? 0 : (l > n ? n : (int)l);
-    hi = h >= n ? n - 1 : (h < -1.0 ? -1 : (int)h);
-}
-
-// The splat above as a gather: pixel (i, j) of tilt th receives from voxel
-// (x, y, z) the weight max(0, 1 - |u - i|) max(0, 1 - |v - j|), which is the
-// splat's (1 - xd) or xd times (1 - yd) or yd, up to float rounding.  Sums
-// run over z, then y, then x ascending with no atomics, so the result is
-// deterministic.  Per layer the rows come from the v condition |v - j| < 1
-// (exactly when v does not depend on x, else from the inverse of the map's
-// (x, y) part on the square's corners); per row, as in k_forward, the first
-// column with u > i - 1 and its distance d0 from the pixel follow from one
-// double evaluation, and candidate c sits at d0 + a1 c.
-__global__ __launch_bounds__(WG) void k_general_gather(const GeneralJob g, int theta0, int ones)
-{
-    const unsigned nib = (g.X + WG - 1) / WG;
-    const int i = (int)(blockIdx.x % nib) * WG + threadIdx.x;
-    const int j = blockIdx.x / nib;
-    if (i >= g.X) return;
-    const int th = theta0 + blockIdx.y;
-    const TiltGeneral &t = g.tg[th];
-    const double a1 = t.a[1], a2 = t.a[2], b1 = t.b[1], b2 = t.b[2];
-    const double det = a1 * b2 - a2 * b1;
-    const double rb2 = b2 != 0.0 ? 1.0 / b2 : 0.0, rdet = det != 0.0 ? 1.0 / det : 0.0;
-    const double ox = (float)t.ox, oy = (float)t.oy;
-    const bool v_of_row = b1 == 0.0;
-    const double aa1 = fabs(a1);
-    // columns a pixel can see in one row: |a1| (x' - x) < 2
-    const int ncand = aa1 > 1e-9 ? (int)ceil(2.0 / aa1 - 1e-6) : g.X;
-    const double ra1 = aa1 > 1e-9 ? 1.0 / a1 : 0.0;
-    const float a1f = (float)a1, b1f = (float)b1;
-    float acc = 0.f;
-    for (int z = 0; z < g.T; ++z) {
-        const double Zc = (double)((float)z - (float)t.oz);
-        const double cu = t.a[0] + t.a[3] * Zc + t.ox;  // u = cu + a1 Xc + a2 Yc
-        const double cv = t.b[0] + t.b[3] * Zc + t.oy;  // v = cv + b1 Xc + b2 Yc
-        const double du = (double)(i - 1) - cu, dv = (double)(j - 1) - cv;
-        int y0, y1;
-        if (v_of_row && b2 != 0.0) {
-            const double p = dv * rb2, q = (dv + 2.0) * rb2;
-            gp_range(fmin(p, q), fmax(p, q), oy, g.Y, y0, y1);
-        } else if (det != 0.0) {
-            const double c = -b1 * rdet, d = a1 * rdet;
-            const double e0 = c * du + d * dv, e1 = c * (du + 2.0) + d * dv, e2 = c * du + d * (dv + 2.0),
-                         e3 = c * (du + 2.0) + d * (dv + 2.0);
-            gp_range(fmin(fmin(e0, e1), fmin(e2, e3)), fmax(fmax(e0, e1), fmax(e2, e3)), oy, g.Y, y0, y1);
-        } else {
-            y0 = 0;
-            y1 = g.Y - 1;
-        }
-        for (int y = y0; y <= y1; ++y) {
-            const double Yc = (double)((float)y - (float)t.oy);
-            const double V0 = cv + b2 * Yc - (double)j;  // v - j at Xc = 0
-            float wyr = 1.f;
-            if (v_of_row) {
-                wyr = fmaxf(0.f, 1.f - fabsf((float)V0));
-                if (wyr == 0.f) continue;  // the row adds nothing to pixel row j
-            }
-            const double U0 = cu + a2 * Yc - (double)i;   // u - i at Xc = 0
-            const float *vrow = g.vol + (g.layout == TREC_LAYOUT_Y ? ((size_t)y * g.T + z) * g.X
-                                                                   : ((size_t)z * g.Y + y) * g.X);
-            // first column xf whose u - i exceeds -1 (a1 > 0) or lies below
-            // +1 (a1 < 0), walking towards growing |d|; d0 = u(xf) - i
-            int xf, step;
-            float d0;
-            if (aa1 > 1e-9) {
-                const double sx = (a1 > 0 ? (-1.0 - U0) : (1.0 - U0)) * ra1 + ox;
-                const double fl = floor(sx);
-                xf = (int)fl + 1;
-                d0 = (float)(U0 + a1 * ((double)xf - ox));
-                step = 1;
-            } else {
-                xf = 0;
-                d0 = (float)U0;
-                step = 1;
-            }
-            const float a1s = aa1 > 1e-9 ? a1f : 0.f;
-            for (int c = 0; c < ncand; c += step) {
-                const int x = xf + c;
-                if ((unsigned)x >= (unsigned)g.X) continue;
-                const float d = fmaf(a1s, (float)c, d0);
-                const float wx = fmaxf(0.f, 1.f - fabsf(d));
-                float wy = wyr;
-                if (!v_of_row) wy = fmaxf(0.f, 1.f - fabsf((float)V0 + b1f * ((float)x - (float)ox)));
-                const float val = ones ? 1.f : vrow[x];
-                acc = fmaf(wx * wy, val, acc);
-            }
-        }
-    }
-    float *out = (ones ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
-    out[(size_t)j * g.X + i] = acc;
-}
-
-// GG_RJ detector rows j0 .. j0 + GG_RJ - 1 of pixel i per thread, for maps whose v
-// does not depend on x (b1 = 0) and grows with y (0 < b2 <= 1): the rows of
-// layer z with |v(y) - j| < 1 lie in ya - 1 .. ya + 2, ya = floor(y*) with
-// v(y*) = j (b2 < 1 can give the outer two a small weight), neighbouring
-// detector rows share three of the four, and a row's column sum
-// (k_general_gather's candidates) is formed once for all of them.  The
-// weights are k_general_gather's; each row's column sum is formed before it
-// is weighted, so the sums round differently.
-constexpr int GG_RJ = 8;
-__device__ __forceinline__ float gg_row(const GeneralJob &g, int y, int z, int i, double cu, double a2, double ra1,
-                                        double a1, float a1f, int ncand, double ox, const TiltGeneral &t, int ones)
-{
-    if ((unsigned)y >= (unsigned)g.Y) return 0.f;
-    const double Yc = (double)((float)y - (float)t.oy);
-    const double U0 = cu + a2 * Yc - (double)i;
-    const double sx = (a1 > 0 ? (-1.0 - U0) : (1.0 - U0)) * ra1 + ox;
-    const int xf = (int)floor(sx) + 1;
-    const float d0 = (float)(U0 + a1 * ((double)xf - ox));
-    const float *vrow = g.vol + (g.layout == TREC_LAYOUT_Y ? ((size_t)y * g.T + z) * g.X : ((size_t)z * g.Y + y) * g.X);
-    float s = 0.f;
-    for (int c = 0; c < ncand; ++c) {
-        const int x = xf + c;
-        if ((unsigned)x >= (unsigned)g.X) continue;
-        const float wx = fmaxf(0.f, 1.f - fabsf(fmaf(a1f, (float)c, d0)));
-        s = fmaf(wx, ones ? 1.f : vrow[x], s);
-    }
-    return s;
-}
-
-__global__ __launch_bounds__(WG) void k_general_gather16(const GeneralJob g, int theta0, int ones)
-{
-    const unsigned nib = (g.X + WG - 1) / WG;
-    const int i = (int)(blockIdx.x % nib) * WG + threadIdx.x;
-    const int j0 = (int)(blockIdx.x / nib) * GG_RJ;
-    if (i >= g.X) return;
-    const int th = theta0 + blockIdx.y;
-    const TiltGeneral &t = g.tg[th];
-    const double a1 = t.a[1], a2 = t.a[2], b2 = t.b[2];
-    const double ra1 = 1.0 / a1, rb2 = 1.0 / b2;
-    const int ncand = (int)ceil(2.0 / fabs(a1) - 1e-6);
-    const float a1f = (float)a1, b2f = (float)b2;
-    const double ox = (float)t.ox, oy = (float)t.oy;
-    float acc[GG_RJ];
-#pragma unroll
-    for (int r = 0; r < GG_RJ; ++r) acc[r] = 0.f;
-    for (int z = 0; z < g.T; ++z) {
-        const double Zc = (double)((float)z - (float)t.oz);
-        const double cu = t.a[0] + t.a[3] * Zc + t.ox;
-        const double cv = t.b[0] + t.b[3] * Zc + t.oy;
-        // row sums of ybase .. ybase + 3 (wave-uniform rows)
-        int ybase = -0x3fffffff;
-        float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
-#pragma unroll
-        for (int r = 0; r < GG_RJ; ++r) {
-            const int jr = j0 + r;
-            const int ya = (int)floor(((double)jr - cv) * rb2 + oy);  // v(ya) <= jr < v(ya + 1)
-            const int sh = ya - 1 - ybase;
-            if (sh == 1) {
-                c0 = c1;
-                c1 = c2;
-                c2 = c3;
-                c3 = gg_row(g, ya + 2, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
-            } else if (sh == 2) {
-                c0 = c2;
-                c1 = c3;
-                c2 = gg_row(g, ya + 1, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
-                c3 = gg_row(g, ya + 2, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
-            } else if (sh != 0) {
-                c0 = gg_row(g, ya - 1, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
-                c1 = gg_row(g, ya, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
-                c2 = gg_row(g, ya + 1, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
-                c3 = gg_row(g, ya + 2, z, i, cu, a2, ra1, a1, a1f, ncand, ox, t, ones);
-            }
-            ybase = ya - 1;
-            // v - j of row ya - 1, then the next three rows b2 apart
-            const float d0 = (float)(cv + b2 * (double)((float)(ya - 1) - (float)t.oy) - jr);
-            float wk[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) wk[k] = fmaxf(0.f, 1.f - fabsf(fmaf(b2f, (float)k, d0)));
-            float a = acc[r];
-            a = fmaf(wk[0], c0, a);
-            a = fmaf(wk[1], c1, a);
-            a = fmaf(wk[2], c2, a);
-            a = fmaf(wk[3], c3, a);
-            acc[r] = a;
-        }
-    }
-    float *out = (ones ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
-#pragma unroll
-    for (int r = 0; r < GG_RJ; ++r)
-        if (j0 + r < g.Y) out[(size_t)(j0 + r) * g.X + i] = acc[r];
-}
-
-// rval <- p - rval / rwt (rwt above the threshold), in place.
-__global__ __launch_bounds__(WG) void k_general_residual(const GeneralJob g, int theta0, size_t n)
-{
-    const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
-    if (i >= n) return;
-    const size_t k = (size_t)theta0 * g.Y * g.X + i;
-    float r = g.rval[k];
-    const float w = g.rwt[k];
-    if (w > g.thr_res) r /= w;  // weights are >= 0: thr_res = 0 is the reference's w != 0
-    g.rval[k] = g.stack[k] - r;
-}
-
-__device__ __forceinline__ void general_bilinear(const float *img, int X, int Y, int xm, int ym, float xd, float yd,
-                                                 float &s, float &c)
-{
-    if (xm >= 0 && xm < X && ym >= 0 && ym < Y) {
-        s += (1 - xd) * (1 - yd) * img[(size_t)ym * X + xm];
-        c += (1 - xd) * (1 - yd);
-    }
-    if (xm + 1 >= 0 && xm + 1 < X && ym >= 0 && ym < Y) {
-        s += xd * (1 - yd) * img[(size_t)ym * X + xm + 1];
-        c += xd * (1 - yd);
-    }
-    if (xm >= 0 && xm < X && ym + 1 >= 0 && ym + 1 < Y) {
-        s += (1 - xd) * yd * img[(size_t)(ym + 1) * X + xm];
-        c += (1 - xd) * yd;
-    }
-    if (xm + 1 >= 0 && xm + 1 < X && ym + 1 >= 0 && ym + 1 < Y) {
-        s += xd * yd * img[(size_t)(ym + 1) * X + xm + 1];
-        c += xd * yd;
-    }
-}
-
-// Voxel-driven gather, BackMode semantics, natural layouts.
-__global__ __launch_bounds__(WG) void k_general_back(const GeneralJob g, int mode, int theta0, int ntheta,
-                                                     const float *__restrict__ img)
-{
-    const size_t idx0 = (size_t)blockIdx.x * WG + threadIdx.x;
-    if (idx0 >= (size_t)g.X * g.T * g.Y) return;
-    const int x = idx0 % g.X;
-    const int z = (idx0 / g.X) % g.T;
-    const int y = idx0 / ((size_t)g.X * g.T);
-    const size_t idx = g.layout == TREC_LAYOUT_Y ? idx0 : ((size_t)z * g.Y + y) * g.X + x;
-    float val = 0.f, wt = 0.f;
-    float out = g.vol[idx];
-    for (int th = theta0; th < theta0 + ntheta; ++th) {
-        int xm, ym;
-        float xd, yd;
-        general_map(g.tg[th], x, y, z, xm, ym, xd, yd);
-        float s = 0.f, c = 0.f;
-        general_bilinear(img + (size_t)th * g.Y * g.X, g.X, g.Y, xm, ym, xd, yd, s, c);
-        if (mode == BACK_SIRT) {
-            val += s;
-            wt += c;
-        } else if (c > g.thr_upd) {
-            out += (mode == BACK_SART) ? (s / c) * g.gamma : s / c;
-        }
-    }
-    if (mode == BACK_SIRT) {
-        if (wt > g.thr_upd) out += val / wt * g.gamma;
-    }
-    g.vol[idx] = out;
-}
-
 // ------------------------------------------------------------------ small helpers
 
 __global__ void k_set_scale(JobDesc *d, const double *sumsq, double n)
@@ -1981,45 +1679,6 @@ hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipSt
         break;
     default: hipLaunchKernelGGL(k_sumsq<float>, dim3(blocks), dim3(WG), 0, st, (const float *)a, n, out); break;
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
-                                  hipStream_t st)
-{
-    // the sixteen-row form needs v independent of x and growing with y, with
-    // columns that move with x (|a1| well above 0), in every tilt launched
-    bool rows16 = true;
-    for (int th = theta0; th < theta0 + ntheta; ++th) {
-        const TiltGeneral &t = host_tg[th];
-        rows16 = rows16 && t.b[1] == 0.0 && t.b[2] > 0.5 && t.b[2] <= 1.0 && fabs(t.a[1]) > 1e-3;
-    }
-    const size_t nblk16 = cdiv((size_t)g.X, WG) * cdiv((size_t)g.Y, GG_RJ);
-    const size_t nblk = cdiv((size_t)g.X, WG) * (size_t)g.Y;
-    if (rows16 && nblk16 <= 0x7fffffffu) {
-        hipLaunchKernelGGL(k_general_gather16, dim3((unsigned)nblk16, ntheta), dim3(WG), 0, st, g, theta0,
-                           ones ? 1 : 0);
-    } else if (nblk <= 0x7fffffffu) {
-        hipLaunchKernelGGL(k_general_gather, dim3((unsigned)nblk, ntheta), dim3(WG), 0, st, g, theta0, ones ? 1 : 0);
-    } else {
-        hipLaunchKernelGGL(k_general_forward, dim3(cdiv((size_t)g.X * g.T, WG), ntheta), dim3(WG), 0, st, g, theta0,
-                           ones ? 1 : 0);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, hipStream_t st)
-{
-    const size_t n = (size_t)ntheta * g.Y * g.X;
-    hipLaunchKernelGGL(k_general_residual, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_general_back(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img,
-                               hipStream_t st)
-{
-    const size_t n = (size_t)g.X * g.T * g.Y;
-    hipLaunchKernelGGL(k_general_back, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, mode, theta0, ntheta, img);
     return hipGetLastError();
 }
 

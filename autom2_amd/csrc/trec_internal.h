@@ -154,7 +154,7 @@ enum BackMode {
     BACK_LIN = 3    // out = lin_a * sum s + lin_b * vol, float only (ADMM's operator)
 };
 
-// General-geometry kernels (natural layouts, any pitch).
+// General-geometry kernels (general.hip: natural layouts, any pitch).
 struct GeneralJob {
     int X, Y, T, P;
     float gamma, thr_upd, thr_res;
@@ -169,8 +169,8 @@ struct GeneralJob {
 hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
                                   hipStream_t st);
 hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, hipStream_t st);
-hipError_t launch_general_back(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img,
-                               hipStream_t st);
+hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, int mode, int theta0, int ntheta,
+                               const float *img, hipStream_t st);
 
 // .xf application (xform.hip).  OpenCV warpAffine's fixed point: 10-bit row
 // and column terms, 1/32-pixel steps, a bias of half a step.

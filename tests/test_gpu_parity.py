@@ -183,10 +183,41 @@ def test_pitch_general_path(torch_cuda, trec, ctx, orc, method):
     assert report(f"pitch {method}", got, ref) < TOL_F32
 
 
-@pytest.mark.parametrize("pitch", [0.0, 5.0])
-def test_forward_project_api(torch_cuda, trec, ctx, orc, pitch):
-    P, Y, T, X = 7, 18, 10, 36
-    ang = tilts(P)
+# Offset 0 with a pitch: v does not depend on x, so the forward takes the
+# row-window kernel (16 detector rows per thread up to ~20 degrees of pitch,
+# 8 up to ~29, the per-pixel gather beyond) and the back-projection the
+# per-voxel row kernel.  X > 256 and Y not a multiple of the row block
+# exercise the partial work-groups; a 70 degree span needs 6 candidate
+# columns per row.
+@pytest.mark.parametrize("method,pitch,span,layout", [
+    ("SIRT,3,1.0", 0.5, 60.0, 0), ("SIRT,3,1.0", 20.0, 70.0, 0), ("SIRT,3,1.0", 25.0, 60.0, 1),
+    ("SIRT,3,1.0", 35.0, 60.0, 0), ("SART,1,0.8", 4.0, 60.0, 0), ("BPT", 12.0, 66.0, 1)])
+def test_pitch_rows_path(torch_cuda, trec, ctx, orc, method, pitch, span, layout):
+    P, Y, T, X = 9, 37, 14, 300
+    ang, coef, org, stack = case(orc, P, Y, T, X, span=span, seed=21, pitch=pitch)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, layout=layout)
+    if layout == 1:
+        got = got.transpose(1, 0, 2)
+    name = method.split(",")[0]
+    if name == "BPT":
+        ref = orc.bpt_f32(coef, stack, T, org)
+    elif name == "SIRT":
+        ref = orc.sirt_f32(coef, stack, T, org, 3, 1.0)
+    else:
+        ref = orc.sart_f32(coef, stack, T, org, 1, 0.8)
+    assert report(f"pitch {pitch} {method} span {span} layout {layout}", got, ref) < TOL_F32
+    again = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, layout=layout)
+    if layout == 1:
+        again = again.transpose(1, 0, 2)
+    np.testing.assert_array_equal(again, got)  # deterministic
+
+
+@pytest.mark.parametrize("pitch,span,shape", [(0.0, 60.0, (7, 18, 10, 36)), (5.0, 60.0, (7, 18, 10, 36)),
+                                              (20.0, 70.0, (7, 37, 10, 300)), (25.0, 60.0, (7, 37, 10, 300)),
+                                              (35.0, 60.0, (5, 21, 9, 70))])
+def test_forward_project_api(torch_cuda, trec, ctx, orc, pitch, span, shape):
+    P, Y, T, X = shape
+    ang = tilts(P, span)
     coef = orc.coeffs_mpi(ang, 0.0, pitch, 0.0)
     org = orc.origin(X, T, T / 2)
     vol = smooth_volume(Y, T, X, 12)
