@@ -367,12 +367,16 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
 
 // Back-projection for tilts that share b and the origins.  Each thread takes
 // GB_Q voxels 64 apart along x in one (y, z) row of the volume (one row per
-// wave); v, its detector rows ym, ym + 1 and their weights are evaluated
-// once per voxel with ValCoef's operation order, and per tilt only u, its
-// columns xm, xm + 1 and their weights.  Taps outside the projection get
-// weight 0 (their loads are clamped into it), as the reference skips them.
-constexpr int GB_Q = 2;
-__global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, int mode, int theta0, int ntheta,
+// wave).  v is evaluated once per voxel with ValCoef's operation order, u
+// once per voxel and tilt.  The bilinear taps are read as two column pairs
+// (c0, c0 + 1) of rows (r0, r0 + 1), with c0 = min(max(floor u, 0), X - 2)
+// and r0 likewise, and each tap is weighted hat(u - c) hat(v - r): inside
+// the projection that is (1 - xd, xd) x (1 - yd, yd), and a tap the
+// reference skips because it lies outside gets 0 (its neighbour inside
+// keeps its weight), so no tap is ever read out of bounds.
+constexpr int GB_Q = 4;
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, int theta0, int ntheta,
                                                           const float *__restrict__ img)
 {
     const unsigned nxt = (g.X + GB_Q * 64 - 1) / (GB_Q * 64);
@@ -387,7 +391,7 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
     const unsigned X = g.X, Y = g.Y;
     double Xc[GB_Q];
     float wy0[GB_Q], wy1[GB_Q], val[GB_Q], wt[GB_Q], out[GB_Q];
-    unsigned r0[GB_Q], r1[GB_Q];
+    unsigned o0[GB_Q], o1[GB_Q];  // byte offsets of rows r0, r0 + 1 in a projection
     float *vout = g.vol + (size_t)row * g.X;
 #pragma unroll
     for (int q = 0; q < GB_Q; ++q) {
@@ -396,13 +400,12 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
         const double n1 = __dadd_rn(__dadd_rn(__dadd_rn(t0.b[0], __dmul_rn(t0.b[1], Xc[q])), __dmul_rn(t0.b[2], Yc)),
                                     __dmul_rn(t0.b[3], Zc));
         const double v = __dadd_rn(n1, t0.oy);
-        const double fv = floor(v);
-        const int ym = (int)fv;
-        const float yd = (float)(v - fv);
-        wy0[q] = (unsigned)ym < Y ? 1.f - yd : 0.f;
-        wy1[q] = (unsigned)(ym + 1) < Y ? yd : 0.f;
-        r0[q] = (unsigned)min(max(ym, 0), (int)Y - 1) * X;
-        r1[q] = (unsigned)min(max(ym + 1, 0), (int)Y - 1) * X;
+        const int r0 = min(max((int)v, 0), (int)Y - 2);
+        const float dy = (float)(v - (double)r0);
+        wy0[q] = hat(dy);
+        wy1[q] = hat(dy - 1.f);
+        o0[q] = (unsigned)r0 * X * 4u;
+        o1[q] = o0[q] + X * 4u;
         val[q] = 0.f;
         wt[q] = 0.f;
         out[q] = x < g.X ? vout[x] : 0.f;
@@ -411,33 +414,30 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
         const TiltGeneral &t = g.tg[th];
         const double ub = t.a[0] + t.a[2] * Yc + t.a[3] * Zc + t.ox;  // u = ub + a1 X
         const double a1 = t.a[1];
-        const float *im = img + (size_t)th * Y * X;
+        const char *im = (const char *)(img + (size_t)th * Y * X);
 #pragma unroll
         for (int q = 0; q < GB_Q; ++q) {
             const double u = a1 * Xc[q] + ub;
-            const double fu = floor(u);
-            const int xm = (int)fu;
-            const float xd = (float)(u - fu);
-            const float wx0 = (unsigned)xm < X ? 1.f - xd : 0.f;
-            const float wx1 = (unsigned)(xm + 1) < X ? xd : 0.f;
-            const unsigned c0 = (unsigned)min(max(xm, 0), (int)X - 1);
-            const unsigned c1 = (unsigned)min(max(xm + 1, 0), (int)X - 1);
-            const float i00 = im[r0[q] + c0], i01 = im[r0[q] + c1];
-            const float i10 = im[r1[q] + c0], i11 = im[r1[q] + c1];
-            const float s = wy0[q] * (wx0 * i00 + wx1 * i01) + wy1[q] * (wx0 * i10 + wx1 * i11);
-            const float c = (wx0 + wx1) * (wy0[q] + wy1[q]);
-            if (mode == BACK_SIRT) {
+            const int c0 = min(max((int)u, 0), (int)X - 2);
+            const float dx = (float)(u - (double)c0);
+            const float w0 = hat(dx), w1 = hat(dx - 1.f);
+            const unsigned oc = (unsigned)c0 * 4u;
+            const float2 p0 = *(const float2 *)(im + (o0[q] + oc));
+            const float2 p1 = *(const float2 *)(im + (o1[q] + oc));
+            const float s = wy0[q] * fmaf(w1, p0.y, w0 * p0.x) + wy1[q] * fmaf(w1, p1.y, w0 * p1.x);
+            const float c = (w0 + w1) * (wy0[q] + wy1[q]);
+            if (MODE == BACK_SIRT) {
                 val[q] += s;
                 wt[q] += c;
             } else if (c > g.thr_upd) {
-                out[q] += (mode == BACK_SART) ? (s / c) * g.gamma : s / c;
+                out[q] += (MODE == BACK_SART) ? (s / c) * g.gamma : s / c;
             }
         }
     }
 #pragma unroll
     for (int q = 0; q < GB_Q; ++q) {
         const int x = xb + 64 * q;
-        if (mode == BACK_SIRT && wt[q] > g.thr_upd) out[q] += val[q] / wt[q] * g.gamma;
+        if (MODE == BACK_SIRT && wt[q] > g.thr_upd) out[q] += val[q] / wt[q] * g.gamma;
         if (x < g.X) vout[x] = out[q];
     }
 }
@@ -521,12 +521,19 @@ hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, 
 hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, int mode, int theta0, int ntheta,
                                const float *img, hipStream_t st)
 {
-    bool rows = ntheta > 0 && (size_t)g.X * g.Y <= 0x80000000u;
+    // byte offsets within a projection fit 32 bits; two columns and rows
+    bool rows = ntheta > 0 && (size_t)g.X * g.Y * 4 <= 0xffffffffu && g.X >= 2 && g.Y >= 2;
     for (int th = theta0; th < theta0 + ntheta && rows; ++th) rows = same_rows(host_tg[th], host_tg[theta0]);
     const size_t nrows_wg = cdiv((size_t)g.Y * g.T, WG / 64);
     const size_t nblk = cdiv((size_t)g.X, GB_Q * 64) * nrows_wg;
     if (rows && nblk <= 0x7fffffffu) {
-        hipLaunchKernelGGL(k_general_back_rows, dim3((unsigned)nblk), dim3(WG), 0, st, g, mode, theta0, ntheta, img);
+        const dim3 grid((unsigned)nblk);
+        if (mode == BACK_SIRT)
+            hipLaunchKernelGGL(k_general_back_rows<BACK_SIRT>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        else if (mode == BACK_SART)
+            hipLaunchKernelGGL(k_general_back_rows<BACK_SART>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        else
+            hipLaunchKernelGGL(k_general_back_rows<BACK_BPT>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
         return hipGetLastError();
     }
     const size_t n = (size_t)g.X * g.T * g.Y;

@@ -189,11 +189,15 @@ def test_pitch_general_path(torch_cuda, trec, ctx, orc, method):
 # per-voxel row kernel.  X > 256 and Y not a multiple of the row block
 # exercise the partial work-groups; a 70 degree span needs 6 candidate
 # columns per row.
-@pytest.mark.parametrize("method,pitch,span,layout", [
-    ("SIRT,3,1.0", 0.5, 60.0, 0), ("SIRT,3,1.0", 20.0, 70.0, 0), ("SIRT,3,1.0", 25.0, 60.0, 1),
-    ("SIRT,3,1.0", 35.0, 60.0, 0), ("SART,1,0.8", 4.0, 60.0, 0), ("BPT", 12.0, 66.0, 1)])
-def test_pitch_rows_path(torch_cuda, trec, ctx, orc, method, pitch, span, layout):
-    P, Y, T, X = 9, 37, 14, 300
+# A volume thicker than it is wide (T = 40, X = 48) projects its corners
+# past the detector's columns and, with the pitch, its end rows past its rows.
+@pytest.mark.parametrize("method,pitch,span,layout,shape", [
+    ("SIRT,3,1.0", 0.5, 60.0, 0, (9, 37, 14, 300)), ("SIRT,3,1.0", 20.0, 70.0, 0, (9, 37, 14, 300)),
+    ("SIRT,3,1.0", 25.0, 60.0, 1, (9, 37, 14, 300)), ("SIRT,3,1.0", 35.0, 60.0, 0, (9, 37, 14, 300)),
+    ("SART,1,0.8", 4.0, 60.0, 0, (9, 37, 14, 300)), ("BPT", 12.0, 66.0, 1, (9, 37, 14, 300)),
+    ("SIRT,3,1.0", 20.0, 60.0, 0, (9, 21, 40, 48)), ("BPT", 8.0, 60.0, 1, (9, 21, 40, 48))])
+def test_pitch_rows_path(torch_cuda, trec, ctx, orc, method, pitch, span, layout, shape):
+    P, Y, T, X = shape
     ang, coef, org, stack = case(orc, P, Y, T, X, span=span, seed=21, pitch=pitch)
     got = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, layout=layout)
     if layout == 1:
