@@ -389,6 +389,9 @@ def main():
                     help="only the GeoParGen stage (its own JSON line), as its profiles measure it")
     ap.add_argument("--xf-only", action="store_true",
                     help="only the .xf application stage (its own JSON line), as its profiles measure it")
+    ap.add_argument("--methods-only", action="store_true",
+                    help="only the other methods and the pitched SIRT (their own JSON line), as their profiles "
+                         "measure them")
     args = ap.parse_args()
     if args.counters:
         counters_run(2)
@@ -415,6 +418,9 @@ def main():
         return
     if args.xf_only:
         print(json.dumps({"xf_apply": xf_stage(trec, ctx, torch, stack_d, _xf, cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.methods_only:
+        print(json.dumps({"methods": methods_form(trec, ctx, torch, geom, ang, stack_d)}), flush=True)
         return
     t0 = time.perf_counter()
     ctx.autotune(geom, trec.make_method(trec.SIRT, args.iters, 1.0))
