@@ -15,6 +15,9 @@
 //                         x pass per volume row and a y pass over rows.
 //   k_general_back_rows   back-projection: v and its two detector rows are
 //                         evaluated once per voxel, u once per voxel and tilt.
+//   k_general_back_lds    the same when v does not depend on x: a wave's row
+//                         of voxels shares its two detector rows, which the
+//                         wave stages in LDS per tilt.
 //
 // Geometries outside their conditions use the reference-order kernels
 // (k_general_gather, k_general_back).  Every kernel is deterministic.
@@ -442,6 +445,130 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
     }
 }
 
+// Back-projection for tilts that also have b1 = 0.  Then v, and with it the
+// two detector rows r0, r0 + 1 and their weights, is the same for every
+// voxel of a wave's (y, z) row.  Per tilt the wave stages the columns of
+// those two rows that its GL_Q * 64 voxels reach into its own LDS, with
+// coalesced loads: from c = min(max(trunc(u), 0), X - 2) at its least u, at
+// most (64 GL_Q - 1) |a1| + 3 <= GL_W columns.  It takes its taps from there.  That is
+// half the bytes of reading the taps from memory, where neighbouring lanes
+// read the same columns again.  The next tilt's window is loaded while the
+// current one is used.  Weights and sums are k_general_back_rows'.
+constexpr int GL_Q = 8;        // voxels per lane, 64 apart
+constexpr int GL_W = 516;      // staged columns per row
+constexpr int GL_PITCH = 520;  // floats between the two staged rows
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int theta0, int ntheta,
+                                                         const float *__restrict__ img)
+{
+    __shared__ float stage[WG / 64][2 * GL_PITCH];
+    const unsigned nxt = (g.X + GL_Q * 64 - 1) / (GL_Q * 64);
+    const unsigned wave = threadIdx.x / 64;
+    const unsigned row = __builtin_amdgcn_readfirstlane((blockIdx.x / nxt) * (WG / 64) + wave);
+    if (row >= (unsigned)g.Y * (unsigned)g.T) return;  // the whole wave: it shares nothing with the others
+    const int lane = (int)(threadIdx.x & 63);
+    const int xb0 = (int)(blockIdx.x % nxt) * GL_Q * 64;
+    const int y = g.layout == TREC_LAYOUT_Y ? (int)(row / g.T) : (int)(row % g.Y);
+    const int z = g.layout == TREC_LAYOUT_Y ? (int)(row % g.T) : (int)(row / g.Y);
+    const TiltGeneral &t0 = g.tg[theta0];
+    const double Yc = (double)((float)y - (float)t0.oy);
+    const double Zc = (double)((float)z - (float)t0.oz);
+    const int X = g.X, Y = g.Y;
+    // ValCoef's v with b1 X = 0
+    const double v = __dadd_rn(__dadd_rn(__dadd_rn(t0.b[0], __dmul_rn(t0.b[2], Yc)), __dmul_rn(t0.b[3], Zc)), t0.oy);
+    const int r0 = min(max((int)v, 0), Y - 2);
+    const float dy = (float)(v - (double)r0);
+    const float wy0 = hat(dy), wy1 = hat(dy - 1.f), wys = wy0 + wy1;
+    const unsigned o0 = (unsigned)r0 * (unsigned)X, o1 = o0 + (unsigned)X;
+    const double X0 = (double)((float)xb0 - (float)t0.ox);                  // X of the wave's first voxel
+    const double X1 = (double)((float)(xb0 + GL_Q * 64 - 1) - (float)t0.ox);  // and of its last
+    double Xc[GL_Q];
+    float val[GL_Q], wt[GL_Q], out[GL_Q];
+    float *vout = g.vol + (size_t)row * g.X;
+#pragma unroll
+    for (int q = 0; q < GL_Q; ++q) {
+        const int x = xb0 + lane + 64 * q;
+        Xc[q] = (double)((float)x - (float)t0.ox);
+        val[q] = 0.f;
+        wt[q] = 0.f;
+        out[q] = x < X ? vout[x] : 0.f;
+    }
+    float *lds = stage[wave];
+    // window of tilt th: u = ub + a1 X, its first column, and the staged
+    // values of columns cmin + lane + 64 m.  The voxels' u below come from
+    // the same ub and the same fused multiply-add as the window's ends, so
+    // they lie between them.
+    constexpr int GM = (GL_W + 63) / 64;
+    static_assert(64 * (GM - 1) <= GL_W && GL_W <= GL_PITCH, "staged columns fit a row, full chunks inside");
+    float s0[GM], s1[GM];
+    int cnext = 0;
+    double ubnext = 0.0, a1next = 0.0;
+    auto load = [&](int th) {
+        const TiltGeneral &t = g.tg[th];
+        ubnext = t.a[0] + t.a[2] * Yc + t.a[3] * Zc + t.ox;
+        a1next = t.a[1];
+        const double u0 = __fma_rn(a1next, X0, ubnext), u1 = __fma_rn(a1next, X1, ubnext);
+        cnext = min(max((int)fmin(u0, u1), 0), X - 2);
+        const float *im = img + (size_t)th * Y * X;
+        if (cnext + 64 * GM <= X) {
+            // the window lies inside the row: one lane offset from the row bases
+            const float *p0 = im + o0 + cnext + lane, *p1 = im + o1 + cnext + lane;
+#pragma unroll
+            for (int m = 0; m < GM; ++m) {
+                s0[m] = p0[64 * m];
+                s1[m] = p1[64 * m];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < GM; ++m) {
+                const unsigned col = (unsigned)min(cnext + lane + 64 * m, X - 1);
+                s0[m] = im[o0 + col];
+                s1[m] = im[o1 + col];
+            }
+        }
+    };
+    load(theta0);
+    for (int th = theta0; th < theta0 + ntheta; ++th) {
+        const int cmin = cnext;
+        const double ub = ubnext, a1 = a1next;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int m = 0; m < GM; ++m) {
+            if (m < GM - 1 || lane + 64 * m < GL_W) {
+                lds[lane + 64 * m] = s0[m];
+                lds[GL_PITCH + lane + 64 * m] = s1[m];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (th + 1 < theta0 + ntheta) load(th + 1);
+#pragma unroll
+        for (int q = 0; q < GL_Q; ++q) {
+            const double u = __fma_rn(a1, Xc[q], ub);
+            const int c0 = min(max((int)u, 0), X - 2);
+            const float dx = (float)(u - (double)c0);
+            const float w0 = hat(dx), w1 = hat(dx - 1.f);
+            // c0 - cmin <= (64 GL_Q - 1) |a1| + 1; the clamp only guards the bound
+            const int li = min(max(c0 - cmin, 0), GL_W - 2);
+            const float s = wy0 * fmaf(w1, lds[li + 1], w0 * lds[li]) +
+                            wy1 * fmaf(w1, lds[GL_PITCH + li + 1], w0 * lds[GL_PITCH + li]);
+            const float c = (w0 + w1) * wys;
+            if (MODE == BACK_SIRT) {
+                val[q] += s;
+                wt[q] += c;
+            } else if (c > g.thr_upd) {
+                out[q] += (MODE == BACK_SART) ? (s / c) * g.gamma : s / c;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int q = 0; q < GL_Q; ++q) {
+        const int x = xb0 + lane + 64 * q;
+        if (MODE == BACK_SIRT && wt[q] > g.thr_upd) out[q] += val[q] / wt[q] * g.gamma;
+        if (x < X) vout[x] = out[q];
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 
 namespace {
@@ -526,6 +653,19 @@ hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, 
     for (int th = theta0; th < theta0 + ntheta && rows; ++th) rows = same_rows(host_tg[th], host_tg[theta0]);
     const size_t nrows_wg = cdiv((size_t)g.Y * g.T, WG / 64);
     const size_t nblk = cdiv((size_t)g.X, GB_Q * 64) * nrows_wg;
+    bool flat = rows;  // v independent of x: the staged kernel
+    for (int th = theta0; th < theta0 + ntheta && flat; ++th) flat = host_tg[th].b[1] == 0.0;
+    const size_t nblk_lds = cdiv((size_t)g.X, GL_Q * 64) * nrows_wg;
+    if (flat && nblk_lds <= 0x7fffffffu) {
+        const dim3 grid((unsigned)nblk_lds);
+        if (mode == BACK_SIRT)
+            hipLaunchKernelGGL(k_general_back_lds<BACK_SIRT>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        else if (mode == BACK_SART)
+            hipLaunchKernelGGL(k_general_back_lds<BACK_SART>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        else
+            hipLaunchKernelGGL(k_general_back_lds<BACK_BPT>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        return hipGetLastError();
+    }
     if (rows && nblk <= 0x7fffffffu) {
         const dim3 grid((unsigned)nblk);
         if (mode == BACK_SIRT)
