@@ -482,15 +482,19 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
     const unsigned o0 = (unsigned)r0 * (unsigned)X, o1 = o0 + (unsigned)X;
     const double X0 = (double)((float)xb0 - (float)t0.ox);                  // X of the wave's first voxel
     const double X1 = (double)((float)(xb0 + GL_Q * 64 - 1) - (float)t0.ox);  // and of its last
+    // SIRT sums, per voxel, the column-weighted taps of each detector row and
+    // the column weights over the tilts; the row weights, the same in every
+    // tilt, are applied once at the end
     double Xc[GL_Q];
-    float val[GL_Q], wt[GL_Q], out[GL_Q];
+    float h0[GL_Q], h1[GL_Q], hw[GL_Q], out[GL_Q];
     float *vout = g.vol + (size_t)row * g.X;
 #pragma unroll
     for (int q = 0; q < GL_Q; ++q) {
         const int x = xb0 + lane + 64 * q;
         Xc[q] = (double)((float)x - (float)t0.ox);
-        val[q] = 0.f;
-        wt[q] = 0.f;
+        h0[q] = 0.f;
+        h1[q] = 0.f;
+        hw[q] = 0.f;
         out[q] = x < X ? vout[x] : 0.f;
     }
     float *lds = stage[wave];
@@ -541,22 +545,26 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
         }
         __builtin_amdgcn_wave_barrier();
         if (th + 1 < theta0 + ntheta) load(th + 1);
+        // u - cmin = ubr + a1 X; the staged column li = c0 - cmin of tap c0 =
+        // min(max(trunc u, 0), X - 2), which is at most (64 GL_Q - 1) |a1| + 1
+        // (the bound GL_W - 2 only guards it)
+        const double ubr = ub - (double)cmin;
+        const int lhi = min(X - 2 - cmin, GL_W - 2);
 #pragma unroll
         for (int q = 0; q < GL_Q; ++q) {
-            const double u = __fma_rn(a1, Xc[q], ub);
-            const int c0 = min(max((int)u, 0), X - 2);
-            const float dx = (float)(u - (double)c0);
+            const double ur = __fma_rn(a1, Xc[q], ubr);
+            const int li = min(max((int)ur, 0), lhi);
+            const float dx = (float)(ur - (double)li);
             const float w0 = hat(dx), w1 = hat(dx - 1.f);
-            // c0 - cmin <= (64 GL_Q - 1) |a1| + 1; the clamp only guards the bound
-            const int li = min(max(c0 - cmin, 0), GL_W - 2);
-            const float s = wy0 * fmaf(w1, lds[li + 1], w0 * lds[li]) +
-                            wy1 * fmaf(w1, lds[GL_PITCH + li + 1], w0 * lds[GL_PITCH + li]);
-            const float c = (w0 + w1) * wys;
             if (MODE == BACK_SIRT) {
-                val[q] += s;
-                wt[q] += c;
-            } else if (c > g.thr_upd) {
-                out[q] += (MODE == BACK_SART) ? (s / c) * g.gamma : s / c;
+                h0[q] = fmaf(w1, lds[li + 1], fmaf(w0, lds[li], h0[q]));
+                h1[q] = fmaf(w1, lds[GL_PITCH + li + 1], fmaf(w0, lds[GL_PITCH + li], h1[q]));
+                hw[q] += w0 + w1;
+            } else {
+                const float s = wy0 * fmaf(w1, lds[li + 1], w0 * lds[li]) +
+                                wy1 * fmaf(w1, lds[GL_PITCH + li + 1], w0 * lds[GL_PITCH + li]);
+                const float c = (w0 + w1) * wys;
+                if (c > g.thr_upd) out[q] += (MODE == BACK_SART) ? (s / c) * g.gamma : s / c;
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -564,7 +572,10 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
 #pragma unroll
     for (int q = 0; q < GL_Q; ++q) {
         const int x = xb0 + lane + 64 * q;
-        if (MODE == BACK_SIRT && wt[q] > g.thr_upd) out[q] += val[q] / wt[q] * g.gamma;
+        if (MODE == BACK_SIRT) {
+            const float val = wy0 * h0[q] + wy1 * h1[q], wt = wys * hw[q];
+            if (wt > g.thr_upd) out[q] += val / wt * g.gamma;
+        }
         if (x < X) vout[x] = out[q];
     }
 }
