@@ -356,10 +356,6 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
         if (!(c[2] == 0.0 && c[4] == 0.0 && c[5] == 0.0 && c[6] == 1.0 && c[7] == 0.0 && c[1] >= 0.2)) rows = false;
     }
     p.path = rows ? PATH_ROWS : PATH_GENERAL;
-    if (m.method == TREC_ADMM && !rows)
-        return fail(ctx, TREC_ERR_UNSUPPORTED,
-                    "ADMM runs on the row-blocked path only: pitch 0 and every |tilt + offset| up to about 78 "
-                    "degrees");
 
     // numeric rules (see trec_rules)
     float thr_upd = 0.f, thr_res = 0.f;
@@ -495,9 +491,18 @@ size_t carve(JobPlan &p, char *base)
         GeneralJob &q = p.gj;
         p.d_tg = (TiltGeneral *)take(sizeof(TiltGeneral) * P);
         q.tg = p.d_tg;
-        q.rval = is_iterative(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
-        q.rwt = is_iterative(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
+        q.rval = uses_forward(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
+        q.rwt = uses_forward(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
         p.stack_copy = is_filtered(k) ? (float *)take(4 * P * Y * X) : nullptr;
+        if (k == TREC_ADMM) {
+            // the CG and ADMM vectors, each a volume in the job's layout
+            AdmmVecs &a = p.av;
+            a.n = X * Y * T;
+            float **vecs[7] = {&a.htb, &a.u, &a.d, &a.r, &a.p, &a.h, &a.ax};
+            for (float **v : vecs) *v = (float *)take(4 * a.n);
+            a.part = (double *)take(sizeof(double) * 2 * ADMM_DOT_BLOCKS);
+            a.S = (double *)take(sizeof(double) * 8);
+        }
         p.hk = is_filtered(k) ? (float *)take(4 * p.hk_host.size()) : nullptr;
     }
     p.bytes = off;
@@ -657,7 +662,7 @@ hipError_t issue_prep(JobPlan &p, hipStream_t st)
                 LCHK(launch_fbp_filter(p.stack_copy, p.hk, p.N, (int)X, (long long)P * Y, 0, 1, 1, (long long)X,
                                        st));
         }
-        if (is_iterative(k)) {
+        if (uses_forward(k)) {
             LCHK(launch_zero(q.rwt, P * Y * X, st));
             LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, true, st));
         }
@@ -681,13 +686,13 @@ hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
     case TREC_SIRT:
         LCHK(launch_zero(q.rval, P * Y * X, st));
         LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, false, st));
-        LCHK(launch_general_residual(q, 0, (int)P, st));
+        LCHK(launch_general_residual(q, 0, (int)P, false, st));
         return launch_general_back(q, p.tg.data(), BACK_SIRT, 0, (int)P, q.rval, st);
     case TREC_SART: {
         const int th = s % (int)P;
         LCHK(launch_zero(q.rval + (size_t)th * Y * X, Y * X, st));
         LCHK(launch_general_forward(q, p.tg.data(), th, 1, false, st));
-        LCHK(launch_general_residual(q, th, 1, st));
+        LCHK(launch_general_residual(q, th, 1, false, st));
         return launch_general_back(q, p.tg.data(), BACK_SART, th, 1, q.rval, st);
     }
     default:
@@ -848,11 +853,68 @@ hipError_t issue_admm(JobPlan &p, hipStream_t st)
     return hipSuccess;
 }
 
+// The same recursion on a general-path job: the operator is the general
+// forward projection of x, the quotient -R / w (0 where w is at most the
+// threshold) and the back-projection out = -sum s + mu x.
+hipError_t issue_admm_general(JobPlan &p, hipStream_t st)
+{
+    const GeneralJob &q0 = p.gj;
+    const int P = p.g.nproj;
+    AdmmVecs a = p.av;
+    a.vol = q0.vol;
+    a.mu = p.m.gamma;
+    a.soft = p.m.soft;
+    // htb = A^T b: the stack's back-projection sum, no x term
+    {
+        ProfScope ps(PROF_BACK, st);
+        GeneralJob q = q0;
+        LCHK(launch_zero(a.htb, a.n, st));
+        q.vol = a.htb;
+        q.out = a.htb;
+        q.lin_a = 1.f;
+        q.lin_b = 0.f;
+        LCHK(launch_general_back(q, p.tg.data(), BACK_LIN, 0, P, p.stack, st));
+    }
+    auto op = [&](float *x, float *out) {
+        GeneralJob q = q0;
+        q.vol = x;
+        q.out = out;
+        q.lin_a = -1.f;
+        q.lin_b = p.m.gamma;
+        {
+            ProfScope ps(PROF_FORWARD, st);
+            LCHK(launch_zero(q.rval, (size_t)P * q.Y * q.X, st));
+            LCHK(launch_general_forward(q, p.tg.data(), 0, P, false, st));
+            LCHK(launch_general_residual(q, 0, P, true, st));
+        }
+        ProfScope ps(PROF_BACK, st);
+        return launch_general_back(q, p.tg.data(), BACK_LIN, 0, P, q.rval, st);
+    };
+    LCHK(launch_zero(a.u, a.n, st));
+    LCHK(launch_zero(a.d, a.n, st));
+    for (int it = 0; it < p.m.iterations; ++it) {
+        LCHK(launch_admm_vec(a, ADMM_HTB, st));
+        LCHK(op(a.vol, a.ax));
+        LCHK(launch_admm_vec(a, ADMM_R0, st));
+        for (int cg = 0; cg < p.m.cg_iterations; ++cg) {
+            LCHK(op(a.p, a.h));
+            LCHK(launch_admm_dots(a, 0, st));
+            LCHK(launch_admm_vec(a, ADMM_STEP, st));
+            LCHK(op(a.vol, a.ax));
+            LCHK(launch_admm_vec(a, ADMM_RES, st));
+            LCHK(launch_admm_dots(a, 1, st));
+            LCHK(launch_admm_vec(a, ADMM_DIR, st));
+        }
+        LCHK(launch_admm_vec(a, ADMM_SOFT, st));
+    }
+    return hipSuccess;
+}
+
 hipError_t issue_all(const std::vector<JobPlan *> &plans, hipStream_t st)
 {
     for (auto *p : plans) LCHK(issue_prep(*p, st));
     for (auto *p : plans)
-        if (p->m.method == TREC_ADMM) LCHK(issue_admm(*p, st));
+        if (p->m.method == TREC_ADMM) LCHK(p->path == PATH_ROWS ? issue_admm(*p, st) : issue_admm_general(*p, st));
     std::vector<int> stage(plans.size(), 0);
     int S = 0;
     for (auto *p : plans) S = std::max(S, p->stages);

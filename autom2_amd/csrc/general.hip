@@ -182,6 +182,9 @@ __global__ __launch_bounds__(WG) void k_general_gather(const GeneralJob g, int t
 }
 
 // rval <- p - rval / rwt (rwt above the threshold), in place.
+// QUOT (ADMM's operator, main-mpiz.cpp:700-760): rval <- -rval / rwt where
+// rwt exceeds the threshold and 0 elsewhere, a tap the reference skips.
+template <bool QUOT>
 __global__ __launch_bounds__(WG) void k_general_residual(const GeneralJob g, int theta0, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
@@ -189,6 +192,10 @@ __global__ __launch_bounds__(WG) void k_general_residual(const GeneralJob g, int
     const size_t k = (size_t)theta0 * g.Y * g.X + i;
     float r = g.rval[k];
     const float w = g.rwt[k];
+    if (QUOT) {
+        g.rval[k] = w > g.thr_res ? -(r / w) : 0.f;
+        return;
+    }
     if (w > g.thr_res) r /= w;  // weights are >= 0: thr_res = 0 is the reference's w != 0
     g.rval[k] = g.stack[k] - r;
 }
@@ -232,12 +239,16 @@ __global__ __launch_bounds__(WG) void k_general_back(const GeneralJob g, int mod
         general_map(g.tg[th], x, y, z, xm, ym, xd, yd);
         float s = 0.f, c = 0.f;
         general_bilinear(img + (size_t)th * g.Y * g.X, g.X, g.Y, xm, ym, xd, yd, s, c);
-        if (mode == BACK_SIRT) {
+        if (mode == BACK_SIRT || mode == BACK_LIN) {
             val += s;
             wt += c;
         } else if (c > g.thr_upd) {
             out += (mode == BACK_SART) ? (s / c) * g.gamma : s / c;
         }
+    }
+    if (mode == BACK_LIN) {
+        g.out[idx] = g.lin_a * val + g.lin_b * out;
+        return;
     }
     if (mode == BACK_SIRT) {
         if (wt > g.thr_upd) out += val / wt * g.gamma;
@@ -429,7 +440,7 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
             const float2 p1 = *(const float2 *)(im + (o1[q] + oc));
             const float s = wy0[q] * fmaf(w1, p0.y, w0 * p0.x) + wy1[q] * fmaf(w1, p1.y, w0 * p1.x);
             const float c = (w0 + w1) * (wy0[q] + wy1[q]);
-            if (MODE == BACK_SIRT) {
+            if (MODE == BACK_SIRT || MODE == BACK_LIN) {
                 val[q] += s;
                 wt[q] += c;
             } else if (c > g.thr_upd) {
@@ -440,6 +451,10 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
 #pragma unroll
     for (int q = 0; q < GB_Q; ++q) {
         const int x = xb + 64 * q;
+        if (MODE == BACK_LIN) {
+            if (x < g.X) g.out[(size_t)row * g.X + x] = g.lin_a * val[q] + g.lin_b * out[q];
+            continue;
+        }
         if (MODE == BACK_SIRT && wt[q] > g.thr_upd) out[q] += val[q] / wt[q] * g.gamma;
         if (x < g.X) vout[x] = out[q];
     }
@@ -556,7 +571,7 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
             const int li = min(max((int)ur, 0), lhi);
             const float dx = (float)(ur - (double)li);
             const float w0 = hat(dx), w1 = hat(dx - 1.f);
-            if (MODE == BACK_SIRT) {
+            if (MODE == BACK_SIRT || MODE == BACK_LIN) {
                 h0[q] = fmaf(w1, lds[li + 1], fmaf(w0, lds[li], h0[q]));
                 h1[q] = fmaf(w1, lds[GL_PITCH + li + 1], fmaf(w0, lds[GL_PITCH + li], h1[q]));
                 hw[q] += w0 + w1;
@@ -572,6 +587,10 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
 #pragma unroll
     for (int q = 0; q < GL_Q; ++q) {
         const int x = xb0 + lane + 64 * q;
+        if (MODE == BACK_LIN) {
+            if (x < X) g.out[(size_t)row * g.X + x] = g.lin_a * (wy0 * h0[q] + wy1 * h1[q]) + g.lin_b * out[q];
+            continue;
+        }
         if (MODE == BACK_SIRT) {
             const float val = wy0 * h0[q] + wy1 * h1[q], wt = wys * hw[q];
             if (wt > g.thr_upd) out[q] += val / wt * g.gamma;
@@ -649,10 +668,11 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     return hipGetLastError();
 }
 
-hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, hipStream_t st)
+hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, bool quot, hipStream_t st)
 {
     const size_t n = (size_t)ntheta * g.Y * g.X;
-    hipLaunchKernelGGL(k_general_residual, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n);
+    if (quot) hipLaunchKernelGGL(k_general_residual<true>, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n);
+    else hipLaunchKernelGGL(k_general_residual<false>, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n);
     return hipGetLastError();
 }
 
@@ -671,6 +691,8 @@ hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, 
         const dim3 grid((unsigned)nblk_lds);
         if (mode == BACK_SIRT)
             hipLaunchKernelGGL(k_general_back_lds<BACK_SIRT>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        else if (mode == BACK_LIN)
+            hipLaunchKernelGGL(k_general_back_lds<BACK_LIN>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
         else if (mode == BACK_SART)
             hipLaunchKernelGGL(k_general_back_lds<BACK_SART>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
         else
@@ -681,6 +703,8 @@ hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, 
         const dim3 grid((unsigned)nblk);
         if (mode == BACK_SIRT)
             hipLaunchKernelGGL(k_general_back_rows<BACK_SIRT>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        else if (mode == BACK_LIN)
+            hipLaunchKernelGGL(k_general_back_rows<BACK_LIN>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
         else if (mode == BACK_SART)
             hipLaunchKernelGGL(k_general_back_rows<BACK_SART>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
         else

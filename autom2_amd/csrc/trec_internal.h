@@ -164,11 +164,16 @@ struct GeneralJob {
     float *vol;             // layout Y [Y][T][X] or Z [T][Y][X]
     float *rval;            // [P][Y][X]
     float *rwt;             // [P][Y][X]
+    // BACK_LIN (ADMM): out = lin_a * sum_theta s_theta(img) + lin_b * vol
+    float *out;
+    float lin_a, lin_b;
 };
 // host_tg: the same tilt table on the host, for the choice of kernel
 hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
                                   hipStream_t st);
-hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, hipStream_t st);
+// quot: -rval / rwt above the threshold, 0 elsewhere (ADMM's operator on a
+// zero stack); else stack - rval / rwt
+hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, bool quot, hipStream_t st);
 hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, int mode, int theta0, int ntheta,
                                const float *img, hipStream_t st);
 

@@ -80,7 +80,7 @@ typedef enum {
     TREC_WBP = 2,  /* R-weighted back-projection */
     TREC_SIRT = 3,
     TREC_SART = 4,
-    TREC_ADMM = 5  /* TiltRecZ's ADMM (pitch 0, |tilt| <= ~78 degrees) */
+    TREC_ADMM = 5  /* TiltRecZ-mpi's ADMM (main-mpiz.cpp:958-1013), any geometry */
 } trec_method_kind;
 
 typedef enum {

@@ -94,16 +94,38 @@ def test_admm_initial_volume(torch_cuda, trec, ctx, orc):
     assert e < TOL_F32
 
 
-def test_admm_refuses_general_path(torch_cuda, trec, ctx, orc):
+# The general path -- a pitch, a pitch with an offset, tilts past about 78
+# degrees, or pitch 0 forced onto it -- runs the same recursion with its own
+# operator: the general forward projection, the quotient -R / w (0 where w
+# is at most 10e-6) and the back-projection -sum s + mu x.
+@pytest.mark.parametrize("pitch,offset,span,layout,force", [
+    (5.0, 0.0, 60.0, 0, False),  # row-window forward, staged back-projection
+    (4.0, 2.0, 60.0, 1, False),  # pitch and offset: per-pixel gather, per-voxel back-projection
+    (0.0, 0.0, 80.0, 0, False),  # tilts past about 78 degrees
+    (0.0, 0.0, 60.0, 1, True)])  # pitch 0 forced onto the general path
+def test_admm_general_path(torch_cuda, trec, ctx, orc, monkeypatch, pitch, offset, span, layout, force):
     torch = torch_cuda
-    P, Y, T, X = 5, 8, 6, 20
-    g = trec.make_geom(X, Y, P, T, pitch=5.0)
-    m = trec.make_method("ADMM,1,1,0.5,0.0")
-    s = torch.zeros((P, Y, X), device="cuda")
-    out = torch.empty((Y, T, X), device="cuda")
-    with pytest.raises(trec.TrecError) as ei:
-        ctx.reconstruct(g, np.linspace(-40, 40, P), s, m, out)
-    assert ei.value.status == trec.ERR_UNSUPPORTED
+    P, Y, T, X = 9, 18, 10, 40
+    ang = tilts(P, span)
+    coef = orc.coeffs_mpi(ang, offset, pitch, 0.0)
+    org = orc.origin(X, T, T / 2)
+    stack = noisy_stack(orc, coef, smooth_volume(Y, T, X, 80), org, 81)
+    if force:
+        monkeypatch.setenv("TREC_FORCE_GENERAL", "1")
+    g = trec.make_geom(X, Y, P, T, offset, pitch, 0.0)
+    m = trec.make_method("ADMM,3,3,0.5,0.01", layout=layout)
+    out = torch.empty((Y, T, X) if layout == 0 else (T, Y, X), dtype=torch.float32, device="cuda")
+    print(f"ADMM general path: pitch {pitch} offset {offset} span {span} layout {layout}", flush=True)
+    ctx.reconstruct(g, ang, dev(torch, stack), m, out)
+    torch.cuda.synchronize()
+    ref = orc.admm_f32(coef, stack, T, org, 3, 3, 0.5, 0.01)
+    ref64 = orc.admm_f64(coef, stack.astype(np.float64), T, org, 3, 3, 0.5, 0.01)
+    if layout == 1:
+        ref, ref64 = to_layout_z(ref), to_layout_z(ref64)
+    got = out.cpu().numpy()
+    e = orc.rel_l2(got, ref)
+    print(f"  rel L2 vs FP32 oracle {e:.3e}, vs FP64 {orc.rel_l2(got, ref64):.3e}")
+    assert e < TOL_F32
 
 
 def test_cli_admm(tmp_path, torch_cuda, orc):
