@@ -58,30 +58,6 @@ __device__ __forceinline__ void general_map(const TiltGeneral &t, int x, int y, 
     yd = (float)(v - fv);
 }
 
-// Splat of one voxel column (x, z) over all rows into rval (or rwt with ones),
-// CuReprojectKernel / CuReWeightKernel semantics with float atomics.
-__global__ __launch_bounds__(WG) void k_general_forward(const GeneralJob g, int theta0, int ones)
-{
-    const int idx = blockIdx.x * WG + threadIdx.x;
-    if (idx >= g.X * g.T) return;
-    const int th = theta0 + blockIdx.y;
-    const int z = idx / g.X, x = idx % g.X;
-    const TiltGeneral &t = g.tg[th];
-    float *out = (ones ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
-    for (int y = 0; y < g.Y; ++y) {
-        const size_t vi = g.layout == TREC_LAYOUT_Y ? ((size_t)y * g.T + z) * g.X + x : ((size_t)z * g.Y + y) * g.X + x;
-        const float v = ones ? 1.f : g.vol[vi];
-        int xm, ym;
-        float xd, yd;
-        general_map(t, x, y, z, xm, ym, xd, yd);
-        if (xm >= 0 && xm < g.X && ym >= 0 && ym < g.Y) atomicAdd(out + (size_t)ym * g.X + xm, (1 - xd) * (1 - yd) * v);
-        if (xm + 1 >= 0 && xm + 1 < g.X && ym >= 0 && ym < g.Y) atomicAdd(out + (size_t)ym * g.X + xm + 1, xd * (1 - yd) * v);
-        if (xm >= 0 && xm < g.X && ym + 1 >= 0 && ym + 1 < g.Y) atomicAdd(out + (size_t)(ym + 1) * g.X + xm, (1 - xd) * yd * v);
-        if (xm + 1 >= 0 && xm + 1 < g.X && ym + 1 >= 0 && ym + 1 < g.Y)
-            atomicAdd(out + (size_t)(ym + 1) * g.X + xm + 1, xd * yd * v);
-    }
-}
-
 // Integer range [lo, hi] of k with (k - o) in [a, b) (rounding slack eps),
 // clamped to [0, n).
 __device__ __forceinline__ void gp_range(double a, double b, double o, int n, int &lo, int &hi)
@@ -658,13 +634,11 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
             return hipGetLastError();
         }
     }
+    // one work-group per 256 pixels of a row: fewer than 2^31 for any stack
+    // that fits in memory (X, Y <= 2^20 and P * Y * X floats in HBM)
     const size_t nblk = nib * (size_t)g.Y;
-    if (nblk <= 0x7fffffffu) {
-        hipLaunchKernelGGL(k_general_gather, dim3((unsigned)nblk, ntheta), dim3(WG), 0, st, g, theta0, ones ? 1 : 0);
-    } else {
-        hipLaunchKernelGGL(k_general_forward, dim3(cdiv((size_t)g.X * g.T, WG), ntheta), dim3(WG), 0, st, g, theta0,
-                           ones ? 1 : 0);
-    }
+    if (nblk > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_general_gather, dim3((unsigned)nblk, ntheta), dim3(WG), 0, st, g, theta0, ones ? 1 : 0);
     return hipGetLastError();
 }
 
