@@ -68,13 +68,14 @@ __device__ __forceinline__ void gp_range(double a, double b, double o, int n, in
     hi = h >= n ? n - 1 : (h < -1.0 ? -1 : (int)h);
 }
 
-// The splat above as a gather: pixel (i, j) of tilt th receives from voxel
-// (x, y, z) the weight max(0, 1 - |u - i|) max(0, 1 - |v - j|), which is the
-// splat's (1 - xd) or xd times (1 - yd) or yd, up to float rounding.  Sums
-// run over z, then y, then x ascending with no atomics, so the result is
-// deterministic.  Per layer the rows come from the v condition |v - j| < 1
-// (exactly when v does not depend on x, else from the inverse of the map's
-// (x, y) part on the square's corners); per row, as in k_forward, the first
+// Reproject's splat (main-mpi.cpp:286-344) as a gather, for any geometry:
+// pixel (i, j) of tilt th receives from voxel (x, y, z) the weight
+// max(0, 1 - |u - i|) max(0, 1 - |v - j|), which is the splat's (1 - xd) or
+// xd times (1 - yd) or yd, up to float rounding.  Sums run over z, then y,
+// then x ascending with no atomics, so the result is deterministic.  Per
+// layer the rows come from the v condition |v - j| < 1 (exactly when v does
+// not depend on x, else from the inverse of the map's (x, y) part on the
+// square's corners); per row, as in the row-blocked k_forward, the first
 // column with u > i - 1 and its distance d0 from the pixel follow from one
 // double evaluation, and candidate c sits at d0 + a1 c.
 __global__ __launch_bounds__(WG) void k_general_gather(const GeneralJob g, int theta0, int ones)
@@ -441,10 +442,12 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
 // voxel of a wave's (y, z) row.  Per tilt the wave stages the columns of
 // those two rows that its GL_Q * 64 voxels reach into its own LDS, with
 // coalesced loads: from c = min(max(trunc(u), 0), X - 2) at its least u, at
-// most (64 GL_Q - 1) |a1| + 3 <= GL_W columns.  It takes its taps from there.  That is
-// half the bytes of reading the taps from memory, where neighbouring lanes
-// read the same columns again.  The next tilt's window is loaded while the
-// current one is used.  Weights and sums are k_general_back_rows'.
+// most (64 GL_Q - 1) |a1| + 3 <= GL_W columns.  It takes its taps from
+// there: half the bytes of reading them from memory, where neighbouring
+// lanes read the same columns again.  The next tilt's window is loaded while
+// the current one is used.  The weights are k_general_back_rows'; SIRT and
+// ADMM sum each detector row's column-weighted taps over the tilts and apply
+// the row weights, the same in every tilt, once per voxel.
 constexpr int GL_Q = 8;        // voxels per lane, 64 apart
 constexpr int GL_W = 516;      // staged columns per row
 constexpr int GL_PITCH = 520;  // floats between the two staged rows
