@@ -626,14 +626,19 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
         b2min = std::min(b2min, t.b[2]);
     }
     nc = nc <= 4 ? nc : 6;
-    const int rj = b2min >= 15.0 / 16.0 ? 16 : (b2min >= 7.0 / 8.0 ? 8 : 0);
+    // 16 detector rows per thread, 8 for pitches between about 20 and 29
+    // degrees; SART's launches of one tilt take 4, which gives the GPU four
+    // times the waves of 16 at twice the slots per row
+    int rj = b2min >= 15.0 / 16.0 ? 16 : (b2min >= 7.0 / 8.0 ? 8 : 0);
+    if (ntheta <= 4 && b2min >= 3.0 / 4.0) rj = 4;
     rows = rows && rj > 0 && g.X >= nc;
     const size_t nib = cdiv((size_t)g.X, WG);
     if (rows) {
         const size_t nblk = nib * cdiv((size_t)g.Y, rj) * (size_t)ntheta;
         if (nblk <= 0x7fffffffu) {
             if (rj == 16) launch_rows_nc<16>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
-            else launch_rows_nc<8>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
+            else if (rj == 8) launch_rows_nc<8>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
+            else launch_rows_nc<4>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
             return hipGetLastError();
         }
     }
