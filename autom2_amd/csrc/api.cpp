@@ -494,6 +494,7 @@ size_t carve(JobPlan &p, char *base)
         q.rval = uses_forward(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
         q.rwt = uses_forward(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
         p.stack_copy = is_filtered(k) ? (float *)take(4 * P * Y * X) : nullptr;
+        q.part = k == TREC_SART ? (float *)take(4 * (size_t)GZ_CHUNKS * Y * X) : nullptr;
         if (k == TREC_ADMM) {
             // the CG and ADMM vectors, each a volume in the job's layout
             AdmmVecs &a = p.av;
@@ -691,8 +692,9 @@ hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
     case TREC_SART: {
         const int th = s % (int)P;
         LCHK(launch_zero(q.rval + (size_t)th * Y * X, Y * X, st));
-        LCHK(launch_general_forward(q, p.tg.data(), th, 1, false, st));
-        LCHK(launch_general_residual(q, th, 1, false, st));
+        int nzc = 1;
+        LCHK(launch_general_forward(q, p.tg.data(), th, 1, false, st, &nzc));
+        LCHK(launch_general_residual(q, th, 1, false, st, nzc));
         return launch_general_back(q, p.tg.data(), BACK_SART, th, 1, q.rval, st);
     }
     default:

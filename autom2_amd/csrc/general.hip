@@ -161,13 +161,21 @@ __global__ __launch_bounds__(WG) void k_general_gather(const GeneralJob g, int t
 // rval <- p - rval / rwt (rwt above the threshold), in place.
 // QUOT (ADMM's operator, main-mpiz.cpp:700-760): rval <- -rval / rwt where
 // rwt exceeds the threshold and 0 elsewhere, a tap the reference skips.
+// With nzc > 1 the forward projection is the sum of nzc partials in
+// g.part, added in chunk order.
 template <bool QUOT>
-__global__ __launch_bounds__(WG) void k_general_residual(const GeneralJob g, int theta0, size_t n)
+__global__ __launch_bounds__(WG) void k_general_residual(const GeneralJob g, int theta0, size_t n, int nzc)
 {
     const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
     if (i >= n) return;
     const size_t k = (size_t)theta0 * g.Y * g.X + i;
-    float r = g.rval[k];
+    float r;
+    if (nzc > 1) {
+        r = g.part[i];
+        for (int c = 1; c < nzc; ++c) r += g.part[(size_t)c * n + i];
+    } else {
+        r = g.rval[k];
+    }
     const float w = g.rwt[k];
     if (QUOT) {
         g.rval[k] = w > g.thr_res ? -(r / w) : 0.f;
@@ -261,7 +269,7 @@ __global__ __launch_bounds__(WG) void k_general_back(const GeneralJob g, int mod
 // in x order.  Work-groups are ordered tilt first, so those in flight
 // together read the same rows of the volume.
 template <int RJ, int NC, bool ONES>
-__global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int theta0, int ntheta)
+__global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int theta0, int ntheta, int nzc)
 {
     constexpr int NS = RJ + 4;
     constexpr int NE = 5 * NS;             // slot/row pairs: slot q / 5, row q / 5 - 4 + q % 5
@@ -275,7 +283,12 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
     // of them; pixels beyond the row repeat its last one and are not stored
     const int i = (int)(rest % nib) * WG + threadIdx.x;
     const int ic = min(i, g.X - 1);
-    const int j0 = (int)(rest / nib) * RJ;
+    const unsigned njb = (g.Y + RJ - 1) / RJ;
+    const int j0 = (int)((rest / nib) % njb) * RJ;
+    // with nzc > 1, layers zc * zlen .. of a chunk, summed into g.part
+    const int zc = (int)(rest / nib / njb);
+    const int zlen = (g.T + nzc - 1) / nzc;
+    const int z0 = zc * zlen, z1 = min(g.T, z0 + zlen);
     const TiltGeneral &t = g.tg[th];
     const double a1 = t.a[1], a2 = t.a[2], b2 = t.b[2];
     const double ra1 = 1.0 / a1, rb2 = 1.0 / b2;
@@ -306,7 +319,7 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
     float acc[RJ];
 #pragma unroll
     for (int r = 0; r < RJ; ++r) acc[r] = 0.f;
-    for (int z = 0; z < g.T; ++z) {
+    for (int z = z0; z < z1; ++z) {
         const double Zc = (double)((float)z - (float)t.oz);
         const double cu = t.a[0] + t.a[3] * Zc + t.ox;  // u = cu + a1 X + a2 Y
         const double cv = t.b[0] + t.b[3] * Zc + t.oy;  // v = cv + b2 Y
@@ -350,7 +363,8 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
         }
     }
     if (i >= g.X) return;
-    float *out = (ONES ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+    float *out = nzc > 1 ? g.part + ((size_t)zc * ntheta + (th - theta0)) * g.Y * g.X
+                         : (ONES ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
 #pragma unroll
     for (int r = 0; r < RJ; ++r)
         if (j0 + r < g.Y) out[(size_t)(j0 + r) * g.X + i] = acc[r];
@@ -588,30 +602,32 @@ bool same_rows(const TiltGeneral &a, const TiltGeneral &b)
 }
 
 template <int RJ, int NC>
-void launch_rows(const GeneralJob &g, int theta0, int ntheta, bool ones, unsigned nblk, hipStream_t st)
+void launch_rows(const GeneralJob &g, int theta0, int ntheta, bool ones, int nzc, unsigned nblk, hipStream_t st)
 {
     if (ones)
-        hipLaunchKernelGGL((k_general_rows<RJ, NC, true>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta);
+        hipLaunchKernelGGL((k_general_rows<RJ, NC, true>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc);
     else
-        hipLaunchKernelGGL((k_general_rows<RJ, NC, false>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta);
+        hipLaunchKernelGGL((k_general_rows<RJ, NC, false>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc);
 }
 
 template <int RJ>
-void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, int nc, unsigned nblk, hipStream_t st)
+void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, int nc, int nzc, unsigned nblk,
+                    hipStream_t st)
 {
     switch (nc) {
-    case 2: launch_rows<RJ, 2>(g, theta0, ntheta, ones, nblk, st); break;
-    case 3: launch_rows<RJ, 3>(g, theta0, ntheta, ones, nblk, st); break;
-    case 4: launch_rows<RJ, 4>(g, theta0, ntheta, ones, nblk, st); break;
-    default: launch_rows<RJ, 6>(g, theta0, ntheta, ones, nblk, st); break;
+    case 2: launch_rows<RJ, 2>(g, theta0, ntheta, ones, nzc, nblk, st); break;
+    case 3: launch_rows<RJ, 3>(g, theta0, ntheta, ones, nzc, nblk, st); break;
+    case 4: launch_rows<RJ, 4>(g, theta0, ntheta, ones, nzc, nblk, st); break;
+    default: launch_rows<RJ, 6>(g, theta0, ntheta, ones, nzc, nblk, st); break;
     }
 }
 
 }  // namespace
 
 hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
-                                  hipStream_t st)
+                                  hipStream_t st, int *nzc_used)
 {
+    if (nzc_used) *nzc_used = 1;
     // row windows: v independent of x and the same in every tilt launched,
     // 7/8 <= b2 <= 1, at most 6 candidate columns (|a1| >= 1/3)
     bool rows = ntheta > 0;
@@ -634,11 +650,15 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     rows = rows && rj > 0 && g.X >= nc;
     const size_t nib = cdiv((size_t)g.X, WG);
     if (rows) {
-        const size_t nblk = nib * cdiv((size_t)g.Y, rj) * (size_t)ntheta;
+        // a single-tilt value projection whose caller takes partials splits
+        // the layers into chunks, for GZ_CHUNKS times the waves
+        const int nzc = nzc_used && g.part && ntheta == 1 && !ones ? std::min(GZ_CHUNKS, g.T) : 1;
+        const size_t nblk = nib * cdiv((size_t)g.Y, rj) * (size_t)ntheta * (size_t)nzc;
         if (nblk <= 0x7fffffffu) {
-            if (rj == 16) launch_rows_nc<16>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
-            else if (rj == 8) launch_rows_nc<8>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
-            else launch_rows_nc<4>(g, theta0, ntheta, ones, nc, (unsigned)nblk, st);
+            if (rj == 16) launch_rows_nc<16>(g, theta0, ntheta, ones, nc, nzc, (unsigned)nblk, st);
+            else if (rj == 8) launch_rows_nc<8>(g, theta0, ntheta, ones, nc, nzc, (unsigned)nblk, st);
+            else launch_rows_nc<4>(g, theta0, ntheta, ones, nc, nzc, (unsigned)nblk, st);
+            if (nzc_used) *nzc_used = nzc;
             return hipGetLastError();
         }
     }
@@ -650,11 +670,11 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     return hipGetLastError();
 }
 
-hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, bool quot, hipStream_t st)
+hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, bool quot, hipStream_t st, int nzc)
 {
     const size_t n = (size_t)ntheta * g.Y * g.X;
-    if (quot) hipLaunchKernelGGL(k_general_residual<true>, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n);
-    else hipLaunchKernelGGL(k_general_residual<false>, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n);
+    if (quot) hipLaunchKernelGGL(k_general_residual<true>, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n, nzc);
+    else hipLaunchKernelGGL(k_general_residual<false>, dim3(cdiv(n, WG)), dim3(WG), 0, st, g, theta0, n, nzc);
     return hipGetLastError();
 }
 

@@ -167,13 +167,18 @@ struct GeneralJob {
     // BACK_LIN (ADMM): out = lin_a * sum_theta s_theta(img) + lin_b * vol
     float *out;
     float lin_a, lin_b;
+    float *part;  // [GZ_CHUNKS][Y][X] layer-chunk partials of a single-tilt forward (SART), or null
 };
+constexpr int GZ_CHUNKS = 4;
 // host_tg: the same tilt table on the host, for the choice of kernel
+// nzc_used (optional): the caller takes layer-chunk partials in g.part; the
+// number of chunks written there (1: the projection is in rval itself)
 hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
-                                  hipStream_t st);
+                                  hipStream_t st, int *nzc_used = nullptr);
 // quot: -rval / rwt above the threshold, 0 elsewhere (ADMM's operator on a
 // zero stack); else stack - rval / rwt
-hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, bool quot, hipStream_t st);
+hipError_t launch_general_residual(const GeneralJob &g, int theta0, int ntheta, bool quot, hipStream_t st,
+                                   int nzc = 1);
 hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, int mode, int theta0, int ntheta,
                                const float *img, hipStream_t st);
 
