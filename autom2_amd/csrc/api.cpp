@@ -333,6 +333,34 @@ int pick_nzc(trec_context *ctx, const trec_geom &g, int storage)
     return std::max(1, std::min(nzc, g.thickness));
 }
 
+// The widest union of voxel bands that k_forward_g stages for a tilt group
+// (union_block there): over layers z, the group's tilts k and its pixel
+// block at i0, from min_k floor(s_k(i0 - 1, z)) + 1 to max_k floor(s_k(i0 +
+// FW_PIX - 2, z)) + 1 + ncand_k, s_k(i, z) = (i - c0_k) / a1_k + slope_k z.
+// That is a maximum of affine functions of (z, i0) minus a minimum of them,
+// convex, so it peaks at a corner of the ranges; floor() moves it by one.
+int widest_union(const std::vector<TiltConst> &tc, const trec_geom &g)
+{
+    const int P = g.nproj, npb = (g.nx + FW_PIX - 1) / FW_PIX;
+    int widest = 0;
+    for (int th0 = 0; th0 < P; th0 += FG) {
+        for (int i0 : {0, (npb - 1) * FW_PIX}) {
+            for (int z : {0, g.thickness - 1}) {
+                int lo = INT_MAX, hi = INT_MIN;
+                for (int k = 0; k < FG; ++k) {
+                    const TiltConst &t = tc[std::min(th0 + k, P - 1)];
+                    const double slo = ((double)(i0 - 1) - t.c0) * t.inv_a1;
+                    const double shi = ((double)(i0 + FW_PIX - 2) - t.c0) * t.inv_a1;
+                    lo = std::min(lo, (int)std::floor(t.slope * z + slo) + 1);
+                    hi = std::max(hi, (int)std::floor(t.slope * z + shi) + 1 + t.ncand);
+                }
+                widest = std::max(widest, hi - lo);
+            }
+        }
+    }
+    return widest;
+}
+
 // Fills the host side of a plan (no device memory yet).
 trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const trec_method &m, const float *angles,
                      const float *stack, const float *initial, float *volume)
@@ -411,6 +439,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             // candidate whose weight is below 1.2e-7
             t.ncand = (int)std::ceil(2.0 / c[1] - 1e-6);
         }
+        p.fg_small = p.m.storage == TREC_STORE_F32 && widest_union(p.tc, g) + FW_G + 2 <= FGS_SMALL;
         filter_taps(p, m.method, g.nx);
     } else {
         GeneralJob &q = p.gj;
@@ -742,7 +771,7 @@ struct Batcher {
 
 hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &stage, hipStream_t st)
 {
-    Batcher fp_fused[3], fp_part[3], comb[3], back[3][3];
+    Batcher fp_fused[3], fp_part[3], comb[3], back[3][3], fp_small;
     for (size_t j = 0; j < plans.size(); ++j) {
         JobPlan &p = *plans[j];
         const int s = stage[j];
@@ -757,7 +786,7 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
         switch (p.m.method) {
         case TREC_SIRT:
             if (h.nzc == 1) {
-                fp_fused[sto].add(p.d, fwd_blocks(h, h.P), 0, 0);
+                (p.fg_small ? fp_small : fp_fused[sto]).add(p.d, fwd_blocks(h, h.P), 0, 0);
             } else {
                 fp_part[sto].add(p.d, fwd_blocks(h, h.P), 0, 0);
                 comb[sto].add(p.d, combine_blocks(h, h.P), 0, 0);
@@ -780,6 +809,8 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
         for (int sto = 0; sto < 3; ++sto)
             LCHK(fp_fused[sto].flush(
                 [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, true, st); }));
+        LCHK(fp_small.flush(
+            [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, TREC_STORE_F32, true, st, true); }));
         for (int sto = 0; sto < 3; ++sto)
             LCHK(fp_part[sto].flush(
                 [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, false, st); }));

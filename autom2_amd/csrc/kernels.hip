@@ -532,11 +532,13 @@ unsigned forward_group_blocks(const JobDesc &h, int ntheta)
 // The union of layer z is [ulo, ulo + uw) with ulo the least first cell and
 // ulo + uw the greatest end over the group's tilts, each computed with the
 // expressions the pixels use, so every lane's candidates lie inside it.
-template <typename PT, bool FUSED, bool HV>
-__global__ __launch_bounds__(FW_PIX *FG) __attribute__((amdgpu_waves_per_eu(6))) void k_forward_g(
+// S is the band's length in cells: FGS (six groups per CU), or FGS_SMALL
+// (seven) when the host has found every union of the launch to fit it.
+template <typename PT, bool FUSED, bool HV, int S = FGS>
+__global__ __launch_bounds__(FW_PIX *FG) __attribute__((amdgpu_waves_per_eu(S <= FGS_SMALL ? 7 : 6))) void k_forward_g(
     const LaunchList L, unsigned nblk)
 {
-    __shared__ float4 band[4][FGS];
+    __shared__ float4 band[4][S];
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
     const int j = find_job(L, b0);
     KJob &J = *kconst(L.job[j]);
@@ -606,7 +608,7 @@ __global__ __launch_bounds__(FW_PIX *FG) __attribute__((amdgpu_waves_per_eu(6)))
 #pragma unroll
     for (int r = 0; r < RB; ++r) acc[r] = 0.f;
 
-    if (umax + FW_G > FGS) {
+    if (umax + FW_G > S) {
         if (mine) {
             for (int z = z0; z < z1; ++z) {
                 const double s = fma(slope, (double)z, sbase);
@@ -627,7 +629,7 @@ __global__ __launch_bounds__(FW_PIX *FG) __attribute__((amdgpu_waves_per_eu(6)))
     } else {
         // Chunk c of the union (16 cells) is loaded by wave c % FG; lane l
         // carries plane l / 16 of cell 16 c + l % 16.
-        constexpr int NLW = (FGS / FW_G + FG - 1) / FG;
+        constexpr int NLW = (S / FW_G + FG - 1) / FG;
         float4 pre[NLW];
 #pragma unroll
         for (int m = 0; m < NLW; ++m) pre[m] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -707,9 +709,9 @@ __global__ __launch_bounds__(FW_PIX *FG) __attribute__((amdgpu_waves_per_eu(6)))
                     const int cl = (int)fl + 1 - ulo;
                     const float d0 = fmaf(-a1f, frac, a1m1);
                     if constexpr (HV)
-                        fw_gather_h<NC, FGS>(band, cl, ncand, d0, a1f, acc);
+                        fw_gather_h<NC, S>(band, cl, ncand, d0, a1f, acc);
                     else
-                        fw_gather<NC, FGS>(band, cl, ncand, d0, a1f, acc);
+                        fw_gather<NC, S>(band, cl, ncand, d0, a1f, acc);
                 }
 #pragma unroll
                 for (int r = 0; r < RB; ++r) asm volatile("" : "+v"(acc[r]));
@@ -1490,7 +1492,7 @@ unsigned fwd_value_blocks(const JobDesc &h, int ntheta)
     return forward_direct() ? forward_blocks(h, ntheta) : forward_group_blocks(h, ntheta);
 }
 
-hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st)
+hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st, bool small)
 {
     // storage < 0 selects the splat of ones (Rwt)
     if (storage < 0) {
@@ -1506,7 +1508,10 @@ hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool 
             switch (storage) {
             case TREC_STORE_F16: hipLaunchKernelGGL((k_forward_g<_Float16, true, true>), dim3(nblk), b, 0, st, L, nblk); break;
             case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward_g<__bf16, true, false>), dim3(nblk), b, 0, st, L, nblk); break;
-            default: hipLaunchKernelGGL((k_forward_g<float, true, false>), dim3(nblk), b, 0, st, L, nblk); break;
+            default:
+                if (small) hipLaunchKernelGGL((k_forward_g<float, true, false, FGS_SMALL>), dim3(nblk), b, 0, st, L, nblk);
+                else hipLaunchKernelGGL((k_forward_g<float, true, false>), dim3(nblk), b, 0, st, L, nblk);
+                break;
             }
         }
     } else {

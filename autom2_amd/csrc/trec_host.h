@@ -29,6 +29,7 @@ struct JobPlan {
     JobDesc *d = nullptr;
     JobDesc *d_ones = nullptr;
     std::vector<TiltConst> tc;
+    bool fg_small = false;  // every union of the forward fits the FGS_SMALL band
     double *sumsq = nullptr;
     float *hk = nullptr;
     int N = 0;

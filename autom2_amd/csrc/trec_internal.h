@@ -34,6 +34,10 @@ constexpr int FW_G = 16;
 constexpr int FG = 4;
 constexpr int FGS = 400;  // 25 KB of LDS per group: six groups (24 waves) fit a CU
 static_assert(FGS % FW_G == 0, "the group band is loaded in whole chunks");
+// a band of 22.5 KB: seven groups (28 waves) fit a CU, for launches whose
+// unions all fit it
+constexpr int FGS_SMALL = 352;
+static_assert(FGS_SMALL % FW_G == 0 && FGS_SMALL < FGS, "the small band is loaded in whole chunks");
 // Back-projection tile: 64 voxels along x (one wave) by BP_Z along z; each
 // thread carries BP_ZT voxels of one x column (z, z+4, ...).  Residuals of
 // BP_G tilts are staged in LDS per pass, BP_W cells per tilt: with a1 <= 1
@@ -120,7 +124,10 @@ hipError_t launch_unblock_volume(const JobDesc &h, const JobDesc *d, float *dst,
 hipError_t launch_weights(const JobDesc &h, const JobDesc *d, hipStream_t st);
 // builds hvol from vol
 hipError_t launch_shadow(const JobDesc &h, const JobDesc *d, hipStream_t st);
-hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st);
+// small: the FP32 fused forward with the FGS_SMALL band (every job of the
+// launch has fg_small set)
+hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st,
+                          bool small = false);
 hipError_t launch_combine(const LaunchList &L, unsigned nblk, int storage, hipStream_t st);
 hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode, hipStream_t st);
 hipError_t launch_rweight(float *data, const float *hk, int N, int X, long long nrows,
