@@ -439,7 +439,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             // candidate whose weight is below 1.2e-7
             t.ncand = (int)std::ceil(2.0 / c[1] - 1e-6);
         }
-        p.fg_small = p.m.storage == TREC_STORE_F32 && widest_union(p.tc, g) + FW_G + 2 <= FGS_SMALL;
+        p.fg_small = widest_union(p.tc, g) + FW_G + 2 <= FGS_SMALL;
         filter_taps(p, m.method, g.nx);
     } else {
         GeneralJob &q = p.gj;
@@ -771,7 +771,7 @@ struct Batcher {
 
 hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &stage, hipStream_t st)
 {
-    Batcher fp_fused[3], fp_part[3], comb[3], back[3][3], fp_small;
+    Batcher fp_fused[3], fp_part[3], comb[3], back[3][3], fp_small[3];
     for (size_t j = 0; j < plans.size(); ++j) {
         JobPlan &p = *plans[j];
         const int s = stage[j];
@@ -786,7 +786,7 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
         switch (p.m.method) {
         case TREC_SIRT:
             if (h.nzc == 1) {
-                (p.fg_small ? fp_small : fp_fused[sto]).add(p.d, fwd_blocks(h, h.P), 0, 0);
+                (p.fg_small ? fp_small[sto] : fp_fused[sto]).add(p.d, fwd_blocks(h, h.P), 0, 0);
             } else {
                 fp_part[sto].add(p.d, fwd_blocks(h, h.P), 0, 0);
                 comb[sto].add(p.d, combine_blocks(h, h.P), 0, 0);
@@ -809,8 +809,9 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
         for (int sto = 0; sto < 3; ++sto)
             LCHK(fp_fused[sto].flush(
                 [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, true, st); }));
-        LCHK(fp_small.flush(
-            [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, TREC_STORE_F32, true, st, true); }));
+        for (int sto = 0; sto < 3; ++sto)
+            LCHK(fp_small[sto].flush(
+                [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, true, st, true); }));
         for (int sto = 0; sto < 3; ++sto)
             LCHK(fp_part[sto].flush(
                 [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, false, st); }));

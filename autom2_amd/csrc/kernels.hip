@@ -1506,8 +1506,14 @@ hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool 
             else hipLaunchKernelGGL((k_forward_g<float, false, false>), dim3(nblk), b, 0, st, L, nblk);
         } else {
             switch (storage) {
-            case TREC_STORE_F16: hipLaunchKernelGGL((k_forward_g<_Float16, true, true>), dim3(nblk), b, 0, st, L, nblk); break;
-            case TREC_STORE_BF16: hipLaunchKernelGGL((k_forward_g<__bf16, true, false>), dim3(nblk), b, 0, st, L, nblk); break;
+            case TREC_STORE_F16:
+                if (small) hipLaunchKernelGGL((k_forward_g<_Float16, true, true, FGS_SMALL>), dim3(nblk), b, 0, st, L, nblk);
+                else hipLaunchKernelGGL((k_forward_g<_Float16, true, true>), dim3(nblk), b, 0, st, L, nblk);
+                break;
+            case TREC_STORE_BF16:
+                if (small) hipLaunchKernelGGL((k_forward_g<__bf16, true, false, FGS_SMALL>), dim3(nblk), b, 0, st, L, nblk);
+                else hipLaunchKernelGGL((k_forward_g<__bf16, true, false>), dim3(nblk), b, 0, st, L, nblk);
+                break;
             default:
                 if (small) hipLaunchKernelGGL((k_forward_g<float, true, false, FGS_SMALL>), dim3(nblk), b, 0, st, L, nblk);
                 else hipLaunchKernelGGL((k_forward_g<float, true, false>), dim3(nblk), b, 0, st, L, nblk);

@@ -124,8 +124,8 @@ hipError_t launch_unblock_volume(const JobDesc &h, const JobDesc *d, float *dst,
 hipError_t launch_weights(const JobDesc &h, const JobDesc *d, hipStream_t st);
 // builds hvol from vol
 hipError_t launch_shadow(const JobDesc &h, const JobDesc *d, hipStream_t st);
-// small: the FP32 fused forward with the FGS_SMALL band (every job of the
-// launch has fg_small set)
+// small: the fused forward with the FGS_SMALL band (every job of the launch
+// has fg_small set)
 hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st,
                           bool small = false);
 hipError_t launch_combine(const LaunchList &L, unsigned nblk, int storage, hipStream_t st);
