@@ -296,3 +296,17 @@ def test_half_volume_copy_edges(torch_cuda, trec, ctx, orc, method):
     e64, e32 = orc.rel_l2(got, ref64), orc.rel_l2(got, f32)
     print(f"{method} half copy: rel L2 vs FP64 {e64:.3e}, vs float storage {e32:.3e}")
     assert e64 < 1e-3 and e32 < 1e-3
+
+
+# The fused forward stages each tilt group's band union in LDS: 352 cells
+# when every union of the job fits (the small shapes above), else 400, and
+# a work-group whose union exceeds even that gathers from L2 directly.
+# Thick volumes widen the unions: +-70 degrees at T = 200 reaches 372 cells
+# (and 6 candidates per layer), +-60 degrees at T = 400 reaches 437.
+@pytest.mark.parametrize("shape,span", [((9, 16, 200, 64), 70.0), ((9, 16, 400, 96), 60.0)])
+def test_sirt_wide_unions(torch_cuda, trec, ctx, orc, shape, span):
+    P, Y, T, X = shape
+    ang, coef, org, stack = case(orc, P, Y, T, X, span=span, seed=33)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,2,1.0")
+    ref = orc.sirt_f32(coef, stack, T, org, 2, 1.0)
+    assert report(f"SIRT x2 T={T} span {span}", got, ref) < TOL_F32
