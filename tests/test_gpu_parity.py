@@ -310,3 +310,14 @@ def test_sirt_wide_unions(torch_cuda, trec, ctx, orc, shape, span):
     got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,2,1.0")
     ref = orc.sirt_f32(coef, stack, T, org, 2, 1.0)
     assert report(f"SIRT x2 T={T} span {span}", got, ref) < TOL_F32
+
+
+def test_f16_wide_unions(torch_cuda, trec, ctx, orc):
+    """The half forward with the 400-cell band (unions of 372 cells)."""
+    P, Y, T, X = 9, 16, 200, 64
+    ang, coef, org, stack = case(orc, P, Y, T, X, span=70.0, seed=34)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,3,1.0", storage="f16")
+    ref64 = orc.sirt_f64(coef, stack.astype(np.float64), T, org, 3, 1.0)
+    e = orc.rel_l2(got, ref64)
+    print(f"SIRT x3 f16 T={T} span 70: rel L2 vs FP64 {e:.3e}")
+    assert e < 1e-3
