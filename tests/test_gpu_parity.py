@@ -321,3 +321,15 @@ def test_f16_wide_unions(torch_cuda, trec, ctx, orc):
     e = orc.rel_l2(got, ref64)
     print(f"SIRT x3 f16 T={T} span 70: rel L2 vs FP64 {e:.3e}")
     assert e < 1e-3
+
+
+@pytest.mark.parametrize("shape", [(5, 3, 4, 5), (7, 2, 6, 9), (3, 5, 3, 4)])
+def test_pitch_tiny_shapes(torch_cuda, trec, ctx, orc, shape):
+    """Pitched SIRT and BPT where every work-group is partial: fewer rows
+    than a row block, as few columns as candidates, a few layers."""
+    P, Y, T, X = shape
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=35, pitch=5.0)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,2,1.0", pitch=5.0)
+    assert report(f"pitch 5 SIRT x2 {shape}", got, orc.sirt_f32(coef, stack, T, org, 2, 1.0)) < TOL_F32
+    got = run(torch_cuda, trec, ctx, stack, ang, T, "BPT", pitch=5.0)
+    assert report(f"pitch 5 BPT {shape}", got, orc.bpt_f32(coef, stack, T, org)) < TOL_F32
