@@ -125,6 +125,13 @@ struct trec_context {
     char *staging = nullptr;  // pinned host memory for descriptor uploads
     size_t staging_size = 0;
     hipEvent_t staging_ev = nullptr;
+    // The arena is reused by every call.  A call's device work waits for the
+    // previous call's (arena_ev, recorded on that call's stream), so calls on
+    // different streams do not overwrite descriptors or workspace that
+    // earlier kernels still read.
+    hipEvent_t arena_ev = nullptr;
+    hipStream_t arena_stream = nullptr;  // stream of the call holding the arena
+    bool arena_held = false;
     hipStream_t cap = nullptr;  // capture stream
     hipGraphExec_t gexec = nullptr;
     std::string gkey;
@@ -363,7 +370,7 @@ int widest_union(const std::vector<TiltConst> &tc, const trec_geom &g)
 
 // Fills the host side of a plan (no device memory yet).
 trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const trec_method &m, const float *angles,
-                     const float *stack, const float *initial, float *volume)
+                     const float *stack, const float *initial, float *volume, int force_nzc, bool force_general)
 {
     p.g = g;
     p.m = m;
@@ -378,7 +385,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
 
     // Row-blocked path: pitch 0 (b = (0,0,1,0), a2 = 0 exactly) and every
     // tilt's a1 >= 0.2 (|angle + offset| <= ~78 degrees).
-    bool rows = !getenv("TREC_FORCE_GENERAL");
+    bool rows = !force_general && !getenv("TREC_FORCE_GENERAL");
     for (int i = 0; i < g.nproj && rows; ++i) {
         const double *c = &p.coef[8 * (size_t)i];
         if (!(c[2] == 0.0 && c[4] == 0.0 && c[5] == 0.0 && c[6] == 1.0 && c[7] == 0.0 && c[1] >= 0.2)) rows = false;
@@ -413,7 +420,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
         h.P = g.nproj;
         h.nyb = (g.ny + RB - 1) / RB;
         h.nzc = 1;
-        if (m.method == TREC_SIRT) h.nzc = pick_nzc(ctx, g, p.m.storage);
+        if (m.method == TREC_SIRT) h.nzc = force_nzc > 0 ? std::min(force_nzc, g.thickness) : pick_nzc(ctx, g, p.m.storage);
         h.zc_len = (g.thickness + h.nzc - 1) / h.nzc;
         h.nzc = (g.thickness + h.zc_len - 1) / h.zc_len;
         h.storage = p.m.storage;
@@ -503,7 +510,7 @@ size_t carve(JobPlan &p, char *base)
         h.inv_rwt = uses_forward(k) ? (float *)take(4 * P * X) : nullptr;
         h.inv_c = k == TREC_SIRT ? (float *)take(4 * T * X) : nullptr;
         h.rwt = p.want_rwt ? (float *)take(4 * P * X) : nullptr;
-        p.sumsq = p.m.storage != TREC_STORE_F32 ? (double *)take(sizeof(double)) : nullptr;
+        p.sumsq = p.m.storage != TREC_STORE_F32 ? (double *)take(sizeof(double) * SUMSQ_BLOCKS) : nullptr;
         p.hk = is_filtered(k) ? (float *)take(4 * p.hk_host.size()) : nullptr;
         if (k == TREC_ADMM) {
             // the CG and ADMM vectors, each a row-blocked volume, and the
@@ -539,9 +546,19 @@ size_t carve(JobPlan &p, char *base)
     return off;
 }
 
-trec_status ensure_arena(trec_context *ctx, size_t bytes)
+trec_status wait_arena(trec_context *ctx, hipStream_t st)
 {
-    if (ctx->arena_size >= bytes) return TREC_OK;
+    HIPCHK(ctx, hipStreamWaitEvent(st, ctx->arena_ev, 0));
+    ctx->arena_stream = st;
+    ctx->arena_held = true;
+    return TREC_OK;
+}
+
+// Makes the arena at least `bytes` long and orders `st` after the previous
+// call's use of it.
+trec_status ensure_arena(trec_context *ctx, size_t bytes, hipStream_t st)
+{
+    if (ctx->arena_size >= bytes) return wait_arena(ctx, st);
     HIPCHK(ctx, hipDeviceSynchronize());
     if (ctx->gexec) {
         (void)hipGraphExecDestroy(ctx->gexec);
@@ -559,8 +576,21 @@ trec_status ensure_arena(trec_context *ctx, size_t bytes)
         return fail(ctx, TREC_ERR_NOMEM, "device workspace of %zu bytes: %s", want, hipGetErrorString(e));
     }
     ctx->arena_size = want;
-    return TREC_OK;
+    return wait_arena(ctx, st);
 }
+
+// Records the end of a call's use of the arena on its stream (arena_ev).
+struct ArenaGuard {
+    trec_context *ctx;
+    explicit ArenaGuard(trec_context *c) : ctx(c) {}
+    ~ArenaGuard()
+    {
+        if (ctx && ctx->arena_held) {
+            (void)hipEventRecord(ctx->arena_ev, ctx->arena_stream);
+            ctx->arena_held = false;
+        }
+    }
+};
 
 // Copies the host-side descriptors of all plans to the device in one pass
 // through the pinned staging buffer.
@@ -649,9 +679,8 @@ hipError_t issue_prep(JobPlan &p, hipStream_t st)
     if (p.path == PATH_ROWS) {
         JobDesc &h = p.h;
         if (p.sumsq) {
-            LCHK(launch_zero((float *)p.sumsq, 2, st));
             LCHK(launch_sumsq(p.stack, TREC_STORE_F32, P * Y * X, p.sumsq, st));
-            LCHK(launch_set_scale(p.d, p.sumsq, (double)(P * Y * X), st));
+            LCHK(launch_set_scale(p.d, p.sumsq, P * Y * X, st));
         }
         LCHK(launch_block_stack(h, p.d, p.stack, st));
         if (p.initial) LCHK(launch_block_volume(h, p.d, p.initial, p.m.layout, st));
@@ -957,18 +986,34 @@ hipError_t issue_all(const std::vector<JobPlan *> &plans, hipStream_t st)
     return hipSuccess;
 }
 
+// 64-bit FNV-1a of a byte range.
+uint64_t fnv1a(const void *data, size_t n, uint64_t h = 1469598103934665603ull)
+{
+    const unsigned char *b = static_cast<const unsigned char *>(data);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// Everything a captured schedule depends on: sizes, method, buffers, and the
+// per-tilt geometry, because the host picks kernels and launch shapes from
+// the coefficients (the row or general path, the general kernels' row
+// windows, candidate and slot counts, the forward's LDS band).
 std::string graph_key(trec_context *ctx, const std::vector<JobPlan *> &plans)
 {
     std::string k;
-    char b[512];
+    char b[640];
     snprintf(b, sizeof b, "%p|", (void *)ctx->arena);
     k += b;
     for (auto *pp : plans) {
         const JobPlan &p = *pp;
-        snprintf(b, sizeof b, "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%a,%p,%p,%p,%d,%d,%d,%a|", p.g.nx, p.g.ny, p.g.nproj,
-                 p.g.thickness, p.g.rules, p.m.method, p.m.iterations, p.m.storage, p.m.layout, p.path,
+        const uint64_t hc = fnv1a(p.coef.data(), sizeof(double) * p.coef.size());
+        snprintf(b, sizeof b,
+                 "%d,%d,%d,%d,%d,%d,%d,%d,%d,%d,%a,%p,%p,%p,%d,%d,%d,%a,%a,%a,%a,%d,%016llx|", p.g.nx, p.g.ny,
+                 p.g.nproj, p.g.thickness, p.g.rules, p.m.method, p.m.iterations, p.m.storage, p.m.layout, p.path,
                  (double)p.m.gamma, (const void *)p.stack, (const void *)p.initial, (void *)p.volume,
-                 p.path == PATH_ROWS ? p.h.nzc : 0, (int)p.want_rwt, p.m.cg_iterations, (double)p.m.soft);
+                 p.path == PATH_ROWS ? p.h.nzc : 0, (int)p.want_rwt, p.m.cg_iterations, (double)p.m.soft,
+                 (double)p.g.offset, (double)p.g.pitch, (double)p.g.zshift, (int)p.fg_small,
+                 (unsigned long long)hc);
         k += b;
     }
     return k;
@@ -981,7 +1026,7 @@ trec_status run_jobs(trec_context *ctx, std::vector<JobPlan> &plan_store, hipStr
     for (auto &p : plan_store) plans.push_back(&p);
     size_t total = 0;
     for (auto *p : plans) total += carve(*p, nullptr);
-    trec_status s = ensure_arena(ctx, total);
+    trec_status s = ensure_arena(ctx, total, st);
     if (s != TREC_OK) return s;
     size_t off = 0;
     for (auto *p : plans) {
@@ -1025,14 +1070,26 @@ trec_status run_jobs(trec_context *ctx, std::vector<JobPlan> &plan_store, hipStr
     return TREC_OK;
 }
 
-size_t request_bytes(const trec_geom &g, const trec_method &m)
+size_t request_bytes(const trec_geom &g, const trec_method &m, const float *angles, int *nzc)
 {
-    std::vector<float> ang(std::max(g.nproj, 1), 0.f);
+    if (nzc) *nzc = 0;
     JobPlan p;
-    if (validate(nullptr, &g, &m, ang.data()) != TREC_OK) return 0;
-    if (plan_job(nullptr, p, g, m, ang.data(), nullptr, nullptr, nullptr) != TREC_OK) return 0;
+    if (validate(nullptr, &g, &m, angles) != TREC_OK) return 0;
+    if (plan_job(nullptr, p, g, m, angles, nullptr, nullptr, nullptr) != TREC_OK) return 0;
+    if (nzc && p.path == PATH_ROWS) *nzc = p.h.nzc;
     const size_t X = g.nx, Y = g.ny, T = g.thickness, P = g.nproj;
     return carve(p, nullptr) + align_up(4 * P * Y * X) + align_up(4 * X * Y * T);
+}
+
+size_t request_bytes_bound(const trec_geom &g, const trec_method &m)
+{
+    std::vector<float> ang(std::max(g.nproj, 1), 0.f);
+    const size_t rows = request_bytes(g, m, ang.data(), nullptr);
+    if (rows == 0) return 0;
+    JobPlan p;
+    if (plan_job(nullptr, p, g, m, ang.data(), nullptr, nullptr, nullptr, 0, true) != TREC_OK) return 0;
+    const size_t X = g.nx, Y = g.ny, T = g.thickness, P = g.nproj;
+    return std::max(rows, carve(p, nullptr) + align_up(4 * P * Y * X) + align_up(4 * X * Y * T));
 }
 
 }  // namespace trec
@@ -1145,7 +1202,8 @@ trec_status trec_context_create(int device, trec_context **out)
     if (!c) return TREC_ERR_NOMEM;
     c->device = device;
     if (hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->staging_ev, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->staging_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->arena_ev, hipEventDisableTiming) != hipSuccess) {
         trec_context_destroy(c);
         return TREC_ERR_DEVICE;
     }
@@ -1162,6 +1220,7 @@ void trec_context_destroy(trec_context *ctx)
     if (ctx->arena) (void)hipFree(ctx->arena);
     if (ctx->staging) (void)hipHostFree(ctx->staging);
     if (ctx->staging_ev) (void)hipEventDestroy(ctx->staging_ev);
+    if (ctx->arena_ev) (void)hipEventDestroy(ctx->arena_ev);
     if (ctx->cap) (void)hipStreamDestroy(ctx->cap);
     for (auto &r : ctx->prof.pending) {
         (void)hipEventDestroy(r.a);
@@ -1212,6 +1271,7 @@ trec_status trec_reconstruct(trec_context *ctx, const trec_geom *geom, const flo
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     trec_status s = validate(ctx, geom, method, angles_deg);
     if (s != TREC_OK) return s;
     if (!stack || !volume) return fail(ctx, TREC_ERR_INVALID, "null stack or volume");
@@ -1228,6 +1288,7 @@ trec_status trec_reconstruct_batch(trec_context *ctx, int njobs, const trec_job 
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     if (njobs < 0 || (njobs > 0 && !jobs)) return fail(ctx, TREC_ERR_INVALID, "bad job list");
     if (njobs == 0) return TREC_OK;
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1247,12 +1308,14 @@ trec_status trec_filter_rweight(trec_context *ctx, float *rows, int nx, long lon
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     if (!rows || nx < 2 || nrows < 0) return fail(ctx, TREC_ERR_INVALID, "bad filter arguments");
     if (nrows == 0) return TREC_OK;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int N = 0;
     std::vector<float> hk = rweight_kernel(nx, N);
-    trec_status s = ensure_arena(ctx, align_up(sizeof(float) * N));
+    hipStream_t st = (hipStream_t)stream;
+    trec_status s = ensure_arena(ctx, align_up(sizeof(float) * N), st);
     if (s != TREC_OK) return s;
     HIPCHK(ctx, hipEventSynchronize(ctx->staging_ev));
     if (ctx->staging_size < sizeof(float) * N) {
@@ -1263,7 +1326,6 @@ trec_status trec_filter_rweight(trec_context *ctx, float *rows, int nx, long lon
         ctx->staging_size = sizeof(float) * N;
     }
     memcpy(ctx->staging, hk.data(), sizeof(float) * N);
-    hipStream_t st = (hipStream_t)stream;
     HIPCHK(ctx, hipMemcpyAsync(ctx->arena, ctx->staging, sizeof(float) * N, hipMemcpyHostToDevice, st));
     HIPCHK(ctx, hipEventRecord(ctx->staging_ev, st));
     HIPCHK(ctx, launch_rweight(rows, (const float *)ctx->arena, N, nx, nrows, 0, 1, 1, nx, st));
@@ -1275,6 +1337,7 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     if (nx < 1 || ny < 1 || nsec < 1 || nsec > 65535)
         return fail(ctx, TREC_ERR_INVALID, "bad stack size %d x %d x %d", nx, ny, nsec);
     if ((size_t)nx * ny > ((size_t)1 << 29))
@@ -1335,7 +1398,7 @@ trec_status trec_apply_xf(trec_context *ctx, int nx, int ny, int nsec, const flo
     const size_t b_rows = align_up(sizeof(int2) * ny * nsec), b_cols = align_up(sizeof(int2) * nx * nsec);
     const size_t b_part = align_up(sizeof(double) * nb * nsec), b_cval = align_up(sizeof(float) * nsec);
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    trec_status s = ensure_arena(ctx, b_xc + b_tab + b_rows + b_cols + b_part + b_cval);
+    trec_status s = ensure_arena(ctx, b_xc + b_tab + b_rows + b_cols + b_part + b_cval, (hipStream_t)stream);
     if (s != TREC_OK) return s;
     XfLaunch a{};
     a.W = nx;
@@ -1390,6 +1453,7 @@ trec_status trec_geopar(trec_context *ctx, const float *volume, int nx, int ny, 
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     if (!volume) return fail(ctx, TREC_ERR_INVALID, "null volume");
     if (nx < 1 || ny < 1 || nz < 1) return fail(ctx, TREC_ERR_INVALID, "bad volume size %d x %d x %d", nx, ny, nz);
     if (axis != TREC_GP_VIEW_X && axis != TREC_GP_VIEW_Y) return fail(ctx, TREC_ERR_INVALID, "bad view axis %d", axis);
@@ -1414,7 +1478,7 @@ trec_status trec_geopar(trec_context *ctx, const float *volume, int nx, int ny, 
     const size_t b_views = views ? 0 : align_up(sizeof(float) * 3 * (size_t)nviews);
     const size_t b_small = align_up(sizeof(float) * 4 + sizeof(int) * 4);
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    trec_status s = ensure_arena(ctx, 2 * b_part + b_ab + b_prof + b_chips + b_views + b_small);
+    trec_status s = ensure_arena(ctx, 2 * b_part + b_ab + b_prof + b_chips + b_views + b_small, (hipStream_t)stream);
     if (s != TREC_OK) return s;
     char *base = ctx->arena;
     GpLaunch a{};
@@ -1494,9 +1558,10 @@ trec_status trec_float_to_byte(trec_context *ctx, const float *in, size_t n, uns
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     if (!in || !out || n == 0) return fail(ctx, TREC_ERR_INVALID, "bad conversion arguments");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    trec_status s = ensure_arena(ctx, align_up(sizeof(double) * 2048) + align_up(2 * sizeof(float)));
+    trec_status s = ensure_arena(ctx, align_up(sizeof(double) * 2048) + align_up(2 * sizeof(float)), (hipStream_t)stream);
     if (s != TREC_OK) return s;
     double *part = (double *)ctx->arena;
     float *stats = (float *)(ctx->arena + align_up(sizeof(double) * 2048));
@@ -1513,6 +1578,7 @@ trec_status trec_filter_fbp(trec_context *ctx, float *rows, int nx, long long nr
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     if (!rows || nx < 1 || nrows < 0 || (mode != 0 && mode != 1))
         return fail(ctx, TREC_ERR_INVALID, "bad filter arguments");
     if (nrows == 0) return TREC_OK;
@@ -1520,7 +1586,7 @@ trec_status trec_filter_fbp(trec_context *ctx, float *rows, int nx, long long nr
     int N = 0;
     std::vector<float> h = spatial_taps(mode, nx, N);
     const size_t bytes = sizeof(float) * h.size();
-    trec_status s = ensure_arena(ctx, align_up(bytes));
+    trec_status s = ensure_arena(ctx, align_up(bytes), (hipStream_t)stream);
     if (s != TREC_OK) return s;
     HIPCHK(ctx, hipEventSynchronize(ctx->staging_ev));
     if (ctx->staging_size < bytes) {
@@ -1543,6 +1609,7 @@ trec_status trec_forward_project(trec_context *ctx, const trec_geom *geom, const
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     trec_method m{TREC_SIRT, 0, 1.0f, TREC_STORE_F32, TREC_LAYOUT_Y};
     trec_status s = validate(ctx, geom, &m, angles_deg);
     if (s != TREC_OK) return s;
@@ -1561,7 +1628,7 @@ trec_status trec_forward_project(trec_context *ctx, const trec_geom *geom, const
         p.m.method = TREC_SIRT;
         // the partial buffer of one z chunk holds the raw projection
         size_t bytes = carve(p, nullptr) + align_up(4 * (size_t)p.h.nyb * P * X * RB);
-        s = ensure_arena(ctx, bytes);
+        s = ensure_arena(ctx, bytes, st);
         if (s != TREC_OK) return s;
         carve(p, ctx->arena);
         p.h.part = (float *)(ctx->arena + p.bytes);
@@ -1586,7 +1653,7 @@ trec_status trec_forward_project(trec_context *ctx, const trec_geom *geom, const
         }
         return TREC_OK;
     }
-    s = ensure_arena(ctx, carve(p, nullptr));
+    s = ensure_arena(ctx, carve(p, nullptr), st);
     if (s != TREC_OK) return s;
     carve(p, ctx->arena);
     s = upload(ctx, {&p}, st);
@@ -1625,7 +1692,7 @@ trec_status trec_request_footprint(const trec_geom *geom, const trec_method *met
     std::vector<float> ang(std::max(geom->nproj, 1), 0.f);
     trec_status s = validate(nullptr, geom, method, ang.data());
     if (s != TREC_OK) return s;
-    *bytes = request_bytes(*geom, *method);
+    *bytes = request_bytes_bound(*geom, *method);
     return TREC_OK;
 }
 
@@ -1673,6 +1740,7 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
 {
     if (!ctx) return TREC_ERR_INVALID;
     ctx->err.clear();
+    ArenaGuard arena_guard(ctx);
     if (!geom || !method) return fail(ctx, TREC_ERR_INVALID, "null geometry or method");
     std::vector<float> ang(std::max(geom->nproj, 1));
     for (int i = 0; i < geom->nproj; ++i) ang[i] = geom->nproj > 1 ? -60.f + 120.f * i / (geom->nproj - 1) : 0.f;
@@ -1699,18 +1767,20 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
     (void)hipEventCreate(&e1);
     int best = 1;
     float best_ms = 1e30f;
+    constexpr int REPS = 4;
     const int cands[] = {1, 2, 4, 8, 16, 32};
     trec_status res = TREC_OK;
     for (int nzc : cands) {
         if (nzc > (int)T) break;
         JobPlan p;
         trec_method m = *method;
-        m.iterations = 3;
+        // one warm-up step and REPS timed steps, each a whole SIRT iteration
+        m.iterations = 1 + REPS;
         if ((res = plan_job(ctx, p, *geom, m, ang.data(), stack, nullptr, vol)) != TREC_OK) break;
         if (p.path != PATH_ROWS) break;
         p.h.zc_len = (int)((T + nzc - 1) / nzc);
         p.h.nzc = (int)((T + p.h.zc_len - 1) / p.h.zc_len);
-        if ((res = ensure_arena(ctx, carve(p, nullptr))) != TREC_OK) break;
+        if ((res = ensure_arena(ctx, carve(p, nullptr), st)) != TREC_OK) break;
         carve(p, ctx->arena);
         if ((res = upload(ctx, {&p}, st)) != TREC_OK) break;
         std::vector<JobPlan *> pl{&p};
@@ -1720,7 +1790,7 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
             break;
         }
         float ms = 1e30f;
-        for (int rep = 0; rep < 4 && res == TREC_OK; ++rep) {
+        for (int rep = 0; rep < REPS && res == TREC_OK; ++rep) {
             float t = 0.f;
             if (hipEventRecord(e0, st) != hipSuccess || issue_step(pl, stage, st) != hipSuccess ||
                 hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {

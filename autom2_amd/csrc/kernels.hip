@@ -1340,10 +1340,12 @@ __global__ __launch_bounds__(WG) void k_to_byte(const float *__restrict__ a, siz
     }
 }
 
-// Sum of squares in double: the stack's scale for the 16-bit forms and the
-// residual norms (one atomic per work-group).
+// Sum of squares in double: the stack's scale for the 16-bit forms.  Each
+// work-group writes its own partial, part[blockIdx.x]; k_set_scale adds the
+// partials in a fixed order, so the scale does not depend on the order in
+// which work-groups finish.
 template <typename T>
-__global__ __launch_bounds__(WG) void k_sumsq(const T *__restrict__ a, size_t n, double *__restrict__ out)
+__global__ __launch_bounds__(WG) void k_sumsq(const T *__restrict__ a, size_t n, double *__restrict__ part)
 {
     __shared__ double red[WG];
     double s = 0.0;
@@ -1357,15 +1359,23 @@ __global__ __launch_bounds__(WG) void k_sumsq(const T *__restrict__ a, size_t n,
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(out, red[0]);
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 // ------------------------------------------------------------------ small helpers
 
-__global__ void k_set_scale(JobDesc *d, const double *sumsq, double n)
+// scale = RMS of the stack from the nb partials of k_sumsq: lane l adds
+// partials l, l + 64, ... in order, then a fixed shuffle tree adds the lanes.
+__global__ __launch_bounds__(64) void k_set_scale(JobDesc *d, const double *__restrict__ part, int nb, double n)
 {
-    const double rms = sqrt(*sumsq / n);
-    d->scale = (rms > 0.0 && rms < 3.0e38) ? (float)rms : 1.0f;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nb; k += 64) acc += part[k];
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) acc += __shfl_down(acc, w, 64);
+    if (threadIdx.x == 0) {
+        const double rms = sqrt(acc / n);
+        d->scale = (rms > 0.0 && rms < 3.0e38) ? (float)rms : 1.0f;
+    }
 }
 
 // blocked [nyb][P][X][RB] float -> [P][Y][X]
@@ -1402,9 +1412,10 @@ __global__ void k_copy(float *__restrict__ dst, const float *__restrict__ src, s
 
 // ------------------------------------------------------------------ launchers
 
-hipError_t launch_set_scale(JobDesc *d, const double *sumsq, double n, hipStream_t st)
+hipError_t launch_set_scale(JobDesc *d, const double *part, size_t n, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(1), 0, st, d, sumsq, n);
+    const int nb = (int)sumsq_blocks(n);
+    hipLaunchKernelGGL(k_set_scale, dim3(1), dim3(64), 0, st, d, part, nb, (double)n);
     return hipGetLastError();
 }
 
@@ -1678,17 +1689,19 @@ hipError_t launch_to_byte(const float *a, size_t n, double *part, float *stats, 
     return hipGetLastError();
 }
 
-hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipStream_t st)
+unsigned sumsq_blocks(size_t n) { return (unsigned)std::min<size_t>(std::max<size_t>(cdiv(n, WG), 1), SUMSQ_BLOCKS); }
+
+hipError_t launch_sumsq(const void *a, int storage, size_t n, double *part, hipStream_t st)
 {
-    const unsigned blocks = (unsigned)std::min<size_t>(cdiv(n, WG), 4096);
+    const unsigned blocks = sumsq_blocks(n);
     switch (storage) {
     case TREC_STORE_F16:
-        hipLaunchKernelGGL(k_sumsq<_Float16>, dim3(blocks), dim3(WG), 0, st, (const _Float16 *)a, n, out);
+        hipLaunchKernelGGL(k_sumsq<_Float16>, dim3(blocks), dim3(WG), 0, st, (const _Float16 *)a, n, part);
         break;
     case TREC_STORE_BF16:
-        hipLaunchKernelGGL(k_sumsq<__bf16>, dim3(blocks), dim3(WG), 0, st, (const __bf16 *)a, n, out);
+        hipLaunchKernelGGL(k_sumsq<__bf16>, dim3(blocks), dim3(WG), 0, st, (const __bf16 *)a, n, part);
         break;
-    default: hipLaunchKernelGGL(k_sumsq<float>, dim3(blocks), dim3(WG), 0, st, (const float *)a, n, out); break;
+    default: hipLaunchKernelGGL(k_sumsq<float>, dim3(blocks), dim3(WG), 0, st, (const float *)a, n, part); break;
     }
     return hipGetLastError();
 }

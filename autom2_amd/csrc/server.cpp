@@ -151,6 +151,7 @@ struct Request {
     const float *stack_host = nullptr;
     float *volume_out = nullptr;
     size_t footprint = 0;
+    int nzc = 0;  // z chunks the footprint was planned with
     double submit_s = 0, admit_s = 0, done_s = 0;
     trec_status status = TREC_OK;
     bool done = false;
@@ -314,8 +315,14 @@ void trec_server::run_copier(Copier &c, bool in)
 // Plans an arrived request and queues its preparation kernels.
 trec_status trec_server::prepare(Request &r)
 {
-    trec_status s = plan_job(ctx, r.plan, r.g, r.m, r.angles.data(), r.dstack(), nullptr, r.dvolume());
+    trec_status s = plan_job(ctx, r.plan, r.g, r.m, r.angles.data(), r.dstack(), nullptr, r.dvolume(), r.nzc);
     if (s != TREC_OK) return s;
+    // the block was sized at submit from the same angles and z chunks; the
+    // workspace must not run into the next block
+    const size_t room = r.footprint - align_up(r.stack_bytes()) - align_up(r.volume_bytes());
+    if (carve(r.plan, nullptr) > room)
+        return fail(ctx, TREC_ERR_INVALID, "request %llu: workspace exceeds its admitted footprint",
+                    (unsigned long long)r.id);
     carve(r.plan, r.workspace());
     s = upload(ctx, {&r.plan}, stream);
     if (s != TREC_OK) return s;
@@ -466,7 +473,7 @@ trec_status trec_server_submit(trec_server *srv, const trec_request *req, uint64
     r->angles.assign(req->angles_deg, req->angles_deg + req->geom.nproj);
     r->stack_host = req->stack;
     r->volume_out = req->volume_out;
-    r->footprint = request_bytes(req->geom, req->method);
+    r->footprint = request_bytes(req->geom, req->method, req->angles_deg, &r->nzc);
     if (r->footprint == 0) return TREC_ERR_INVALID;
     if (align_up(r->footprint) > srv->arena.size()) return TREC_ERR_BUSY;  // can never fit
     r->submit_s = now_s();

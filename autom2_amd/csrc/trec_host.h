@@ -57,8 +57,11 @@ std::vector<float> rweight_kernel(int X, int &N);
 std::vector<float> spatial_taps(int mode, int X, int &N);
 
 trec_status validate(trec_context *ctx, const trec_geom *g, const trec_method *m, const float *angles);
+// force_nzc > 0 fixes the SIRT forward's z chunks (else the tuned or default
+// count); force_general takes the general path whatever the geometry.
 trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const trec_method &m, const float *angles,
-                     const float *stack, const float *initial, float *volume);
+                     const float *stack, const float *initial, float *volume, int force_nzc = 0,
+                     bool force_general = false);
 // Assigns the plan's device buffers from `base` (nullptr: sizing only).
 size_t carve(JobPlan &p, char *base);
 // Copies the plans' descriptors to the device through pinned staging.
@@ -71,7 +74,10 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
 hipError_t issue_finish(JobPlan &p, hipStream_t st);
 
 // Bytes a request holds on the device in the serving loop: workspace plus
-// its stack and volume.
-size_t request_bytes(const trec_geom &g, const trec_method &m);
+// its stack and volume, planned with its own tilt angles; *nzc receives the
+// z chunks that plan uses (0 on the general path), which the request keeps.
+size_t request_bytes(const trec_geom &g, const trec_method &m, const float *angles, int *nzc);
+// The same without angles: the larger of the row and the general path.
+size_t request_bytes_bound(const trec_geom &g, const trec_method &m);
 
 }  // namespace trec

@@ -142,8 +142,11 @@ hipError_t launch_fbp_filter(float *data, const float *taps, int N, int X, long 
 hipError_t launch_to_byte(const float *a, size_t n, double *part, float *stats, unsigned char *out, hipStream_t st);
 // trec_convert's element conversion (types are trec_dtype values).
 hipError_t launch_convert(const void *in, int in_type, void *out, int out_type, size_t n, hipStream_t st);
-// out += sum of squares (in double) of n values of the storage type.
-hipError_t launch_sumsq(const void *a, int storage, size_t n, double *out, hipStream_t st);
+// Sum of squares (in double) of n values of the storage type, as one partial
+// per work-group in part[0 .. sumsq_blocks(n)) (at most SUMSQ_BLOCKS).
+constexpr int SUMSQ_BLOCKS = 1024;
+unsigned sumsq_blocks(size_t n);
+hipError_t launch_sumsq(const void *a, int storage, size_t n, double *part, hipStream_t st);
 
 // Work-group counts of the multi-job kernels for one job.
 unsigned forward_blocks(const JobDesc &h, int ntheta);
@@ -268,7 +271,8 @@ hipError_t launch_gp_search(const GpLaunch &a, hipStream_t st);
 hipError_t launch_gp_views(const GpLaunch &a, hipStream_t st);
 
 // Small helpers.
-hipError_t launch_set_scale(JobDesc *d, const double *sumsq, double n, hipStream_t st);
+// d->scale = sqrt(sum of launch_sumsq's partials / n), summed in a fixed order
+hipError_t launch_set_scale(JobDesc *d, const double *part, size_t n, hipStream_t st);
 hipError_t launch_unblock_proj(const float *src, int P, int X, int Y, int nyb, float *dst, hipStream_t st);
 hipError_t launch_broadcast_rows(const float *src, int P, int X, int Y, float *dst, hipStream_t st);
 hipError_t launch_copy(float *dst, const float *src, size_t n, hipStream_t st);

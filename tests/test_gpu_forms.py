@@ -159,3 +159,74 @@ def test_server_admission_by_memory(torch_cuda, trec, orc):
     for stack, out in zip(stacks, outs):
         ref = orc.sirt_f32(orc.coeffs_mpi(tilts(P)), stack, T, orc.origin(X, T, T / 2), 3, 1.0)
         assert orc.rel_l2(out, ref) < 1e-5
+
+
+def test_batch_graph_follows_geometry(torch_cuda, trec, orc):
+    """The cached graph is keyed on the per-tilt geometry: the same buffers
+    reconstructed with another pitch, offset or set of angles must not
+    replay the first call's kernel choices (row windows, candidate counts,
+    the general or row path)."""
+    ctx = trec.Context(0)
+    P, Y, T, X = 9, 24, 12, 48
+    stack = np.random.default_rng(5).random((P, Y, X)).astype(np.float32)
+    dstack = torch_cuda.from_numpy(stack).cuda()
+    vol = torch_cuda.empty((Y, T, X), device="cuda")
+    cases = [(0.0, 4.0, tilts(P)), (0.0, 12.0, tilts(P)), (2.0, 4.0, tilts(P)), (0.0, 0.0, tilts(P)),
+             (0.0, 4.0, tilts(P, 75.0)), (0.0, 0.0, tilts(P, 80.0))]
+    for offset, pitch, ang in cases:
+        g = trec.make_geom(X, Y, P, T, offset, pitch, 0.0)
+        ctx.reconstruct_batch([trec.Series(g, trec.make_method("SIRT,3,1.0"), ang, dstack, vol)])
+        torch_cuda.cuda.synchronize()
+        ref = orc.sirt_f32(orc.coeffs_mpi(ang, offset, pitch, 0.0), stack, T, orc.origin(X, T, T / 2), 3, 1.0)
+        err = orc.rel_l2(vol.cpu().numpy(), ref)
+        print(f"offset {offset} pitch {pitch} span {ang[-1]}: rel L2 {err:.2e}")
+        assert err < 1e-5, (offset, pitch, ang[-1])
+    ctx.close()
+
+
+def test_calls_on_two_streams_share_the_workspace(torch_cuda, trec, orc):
+    """Back-to-back calls on one context but different streams: the second
+    call's descriptors and workspace must wait for the first call's kernels."""
+    torch = torch_cuda
+    ctx = trec.Context(0)
+    a = make_series(orc, 31, 64, 64, 256, 110)
+    b = make_series(orc, 9, 20, 12, 40, 111)
+    refs = [alone(torch, trec, ctx, a[0], a[1], 64, "SIRT,8,1.0", "f32"),
+            alone(torch, trec, ctx, b[0], b[1], 12, "SIRT,2,1.0", "f16")]
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for rep in range(3):
+        outs = []
+        for (ang, stack), T, meth, sto, st in ((a, 64, "SIRT,8,1.0", "f32", s1), (b, 12, "SIRT,2,1.0", "f16", s2)):
+            P, Y, X = stack.shape
+            d = torch.from_numpy(stack).cuda()
+            o = torch.empty((Y, T, X), device="cuda")
+            torch.cuda.synchronize()
+            ctx.reconstruct(trec.make_geom(X, Y, P, T), ang, d, trec.make_method(meth, storage=sto), o, stream=st)
+            outs.append((o, d))
+        torch.cuda.synchronize()
+        for (o, _), ref in zip(outs, refs):
+            np.testing.assert_array_equal(o.cpu().numpy(), ref, err_msg=f"rep {rep}")
+    ctx.close()
+
+
+def test_server_steep_tilts_and_thin_volume(torch_cuda, trec, orc):
+    """Requests whose real angles take the general path (tilts past about 78
+    degrees) are sized with those angles at submit: a thin SART volume and a
+    SIRT volume, each checked against the oracle."""
+    srv = trec.Server(0, budget=256 << 20, max_active=4)
+    cases = [(9, 16, 3, 40, "SART,2,0.7", 80.0), (11, 20, 10, 36, "SIRT,3,1.0", 82.0)]
+    ids, outs, data = [], [], []
+    for k, (P, Y, T, X, meth, span) in enumerate(cases):
+        ang, stack = make_series(orc, P, Y, T, X, 120 + k, span)
+        out = np.zeros((Y, T, X), np.float32)
+        outs.append(out)
+        data.append((ang, stack))
+        ids.append(srv.submit(trec.make_geom(X, Y, P, T), trec.make_method(meth), ang, stack, out))
+    recs = [srv.wait(i) for i in ids]
+    srv.close()
+    for (P, Y, T, X, meth, span), (ang, stack), out, rec in zip(cases, data, outs, recs):
+        assert rec["status"] == 0
+        it, gam = int(meth.split(",")[1]), float(meth.split(",")[2])
+        fn = orc.sart_f32 if meth.startswith("SART") else orc.sirt_f32
+        ref = fn(orc.coeffs_mpi(ang), stack, T, orc.origin(X, T, T / 2), it, gam)
+        assert orc.rel_l2(out, ref) < 1e-5, meth
