@@ -141,6 +141,13 @@ struct trec_context {
 
 namespace trec {
 
+void clear_error(trec_context *ctx)
+{
+    if (ctx) ctx->err.clear();
+}
+
+int context_device(const trec_context *ctx) { return ctx->device; }
+
 trec_status fail(trec_context *ctx, trec_status s, const char *fmt, ...)
 {
     if (ctx) {
@@ -1201,6 +1208,13 @@ trec_status trec_context_create(int device, trec_context **out)
     trec_context *c = new (std::nothrow) trec_context;
     if (!c) return TREC_ERR_NOMEM;
     c->device = device;
+    // the no-allocation entry points' device array: its address on this
+    // device, taken once per device with the device current
+    if (static_scratch_init(device) != hipSuccess) {
+        (void)hipGetLastError();
+        delete c;
+        return TREC_ERR_DEVICE;
+    }
     if (hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->staging_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->arena_ev, hipEventDisableTiming) != hipSuccess) {

@@ -48,6 +48,12 @@ struct JobPlan {
 };
 
 trec_status fail(trec_context *ctx, trec_status s, const char *fmt, ...);
+void clear_error(trec_context *ctx);
+int context_device(const trec_context *ctx);
+// rweight_static.hip: takes the device address of the no-allocation entry
+// points' file-scope array on `device`, which must be current (once per
+// device; trec_context_create calls it).
+hipError_t static_scratch_init(int device);
 
 // Coefficients of main-mpi.cpp:85-183 (coef: 8 per tilt).
 void coefficients(const trec_geom &g, const float *angles, double *coef);

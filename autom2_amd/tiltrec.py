@@ -77,6 +77,7 @@ SIGNATURES = [
     ("trec_workspace_size", _I, [ctypes.POINTER(Geom), ctypes.POINTER(Method), ctypes.POINTER(ctypes.c_size_t)]),
     ("trec_reconstruct", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, ctypes.POINTER(Method), _P, _P]),
     ("trec_filter_rweight", _I, [_P, _P, _I, ctypes.c_longlong, _P]),
+    ("trec_filter_rweight_static", _I, [_P, _P, _I, ctypes.c_longlong, _P]),
     ("trec_filter_fbp", _I, [_P, _P, _I, ctypes.c_longlong, _I, _P]),
     ("trec_float_to_byte", _I, [_P, _P, ctypes.c_size_t, _P, _PF, _P]),
     ("trec_forward_project", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, _P, _P]),
@@ -283,6 +284,12 @@ class Context:
     def filter_rweight(self, rows, stream=None):
         nx = rows.shape[-1]
         _check(lib().trec_filter_rweight(self._h, _ptr(rows), nx, rows.numel() // nx, _stream(stream)), self._h)
+
+    def filter_rweight_static(self, rows, stream=None):
+        """The R-weighting without any device allocation (rows up to 7448 pixels)."""
+        nx = rows.shape[-1]
+        _check(lib().trec_filter_rweight_static(self._h, _ptr(rows), nx, rows.numel() // nx, _stream(stream)),
+               self._h)
 
     def filter_fbp(self, rows, mode=0, stream=None):
         """FBP's spatial filter in place on [..., nx] rows (0: Shepp-Logan, 1: Ram-Lak)."""

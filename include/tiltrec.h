@@ -172,6 +172,18 @@ trec_status trec_reconstruct(trec_context *ctx, const trec_geom *geom,
 trec_status trec_filter_rweight(trec_context *ctx, float *rows, int nx,
                                 long long nrows, void *stream);
 
+/* trec_filter_rweight for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The same filter; its kernel
+ * table is built on the device in a fixed array of the library (128 KiB per
+ * device, held until the process ends), so the call allocates nothing.
+ * Rows are limited to nx + nx / 10 <= 8192 taps (nx <= 7448, the widest rows
+ * the filter holds in LDS); wider rows return TREC_ERR_INVALID.  All calls on one device share that array and it
+ * keeps the last call's table: calls that overlap on one device (different
+ * streams or threads) race, so issue them on one stream. */
+trec_status trec_filter_rweight_static(trec_context *ctx, float *rows, int nx,
+                                       long long nrows, void *stream);
+
 /* FBP's spatial filter of nrows rows of nx floats in place (ApplyFilterInplace
  * modes 0 and 1, filter_prj.cpp:137-178, 266-446): mode 0 Shepp-Logan (what
  * FBP uses), 1 Ram-Lak; 251 taps over the reference's symmetrised and
