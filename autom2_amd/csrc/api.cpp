@@ -399,19 +399,27 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
     }
     p.path = rows ? PATH_ROWS : PATH_GENERAL;
 
-    // numeric rules (see trec_rules)
+    // numeric rules (see trec_rules).  The reference compares float sums with
+    // double literals (w > 1e-7); the kernels compare floats with floats, so
+    // each literal becomes the largest float not above it: for a float w,
+    // w > that float exactly when w > the literal.
+    auto at_most = [](double t) {
+        float f = (float)t;
+        if ((double)f > t) f = std::nextafter(f, 0.0f);
+        return f;
+    };
     float thr_upd = 0.f, thr_res = 0.f;
     if (g.rules == TREC_RULES_CUDA) {
-        if (m.method == TREC_SIRT) thr_upd = 1e-5f;
-        if (m.method == TREC_BPT || is_filtered(m.method)) thr_upd = 1e-5f;
+        if (m.method == TREC_SIRT) thr_upd = at_most(10e-6);                           // basic.cu:184
+        if (m.method == TREC_BPT || is_filtered(m.method)) thr_upd = at_most(1e-5);   // basic.cu:409
         if (m.method == TREC_SART) {
-            thr_upd = 1e-6f;
-            thr_res = 1e-7f;
+            thr_upd = at_most(10e-7);  // basic.cu:434
+            thr_res = at_most(1e-7);   // basic.cu:170
         }
     }
     // ADMM: a tap contributes where the ray weight exceeds 10e-6 (both
     // reference binaries, main-mpiz.cpp:740-785, baiscz.cu:446-480)
-    if (m.method == TREC_ADMM) thr_res = 10e-6f;
+    if (m.method == TREC_ADMM) thr_res = at_most(10e-6);
     const int iters = is_iterative(m.method) ? m.iterations : 0;
     if (m.method == TREC_SIRT) p.stages = iters;
     else if (m.method == TREC_SART) p.stages = iters * g.nproj;

@@ -37,16 +37,16 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_coeffs_mpi.argtypes = [_F, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _D]
         L.orc_coeffs_cuda.argtypes = [_F, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _F]
-        for n in ("orc_bpt_f32",):
-            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _F, _F]
+        _DB = ctypes.c_double
+        L.orc_bpt_f32.argtypes = [_D, _I, _I, _I, _I, _F, _F, _F, _DB]
         for n in ("orc_sirt_f32", "orc_sart_f32"):
-            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _F, _I, ctypes.c_float, _F]
+            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _F, _I, ctypes.c_float, _F, _DB, _DB]
         L.orc_reproject_f32.argtypes = [_D, _I, _I, _I, _F, _F, _F, _F]
         L.orc_reproject_f64.argtypes = [_D, _I, _I, _I, _F, _D, _D, _D]
         L.orc_backproject_sums_f64.argtypes = [_D, _I, _I, _I, _F, _D, _D, _D]
-        L.orc_bpt_f64.argtypes = [_D, _I, _I, _I, _I, _F, _D, _D]
+        L.orc_bpt_f64.argtypes = [_D, _I, _I, _I, _I, _F, _D, _D, _DB]
         for n in ("orc_sirt_f64", "orc_sart_f64"):
-            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, ctypes.c_double, _D]
+            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, ctypes.c_double, _D, _DB, _DB]
         L.orc_rweight_f64.argtypes = [_D, _I, _I]
         L.orc_optimal_dft_size.argtypes = [_I]
         L.orc_optimal_dft_size.restype = _I
@@ -113,6 +113,21 @@ def origin(X, T, oy):
     return (np.float32(X * 0.5), np.float32(oy), np.float32(T * 0.5))
 
 
+# TiltRec-cuda's weight thresholds (update, residual) per method, as the
+# double literals its kernels compare float sums with: BPT/WBP/FBP c > 1e-5
+# (basic.cu:409), SIRT wtvol > 10e-6 (:184) with an exact residual test
+# (:152), SART c > 10e-7 (:434) and reproj_wt > 1e-7 (:170).  TiltRec-mpi's
+# tests are exact: (0, 0).
+CUDA_THRESHOLDS = {"BPT": (1e-5, 0.0), "WBP": (1e-5, 0.0), "FBP": (1e-5, 0.0),
+                   "SIRT": (10e-6, 0.0), "SART": (10e-7, 1e-7)}
+
+
+def rules_origin(X, Y, T, rules):
+    """TiltRec-mpi puts the y origin at T/2 (main-mpi.cpp:743-745), the CUDA
+    and Z binaries at Y/2 (main-cuda.cpp:259-261)."""
+    return origin(X, T, T / 2 if rules == 0 else Y / 2)
+
+
 def _dims(proj, T):
     P, rows, X = proj.shape
     return P, X, rows, T
@@ -150,56 +165,60 @@ def backproject_sums_f64(coef1, img, T, org):
     return s, c
 
 
-def bpt_f32(coef, proj, T, org, vol=None):
+def bpt_f32(coef, proj, T, org, vol=None, thr=(0.0, 0.0)):
+    """thr = (update, residual) thresholds; BPT reads the update one."""
     proj = np.ascontiguousarray(proj, dtype=np.float32)
     P, X, rows, _ = _dims(proj, T)
     out = np.zeros((rows, T, X), np.float32) if vol is None else np.array(vol, np.float32)
     c = np.ascontiguousarray(coef, dtype=np.float64)
-    lib().orc_bpt_f32(_dp(c), P, X, rows, T, _org(org), _fp(proj), _fp(out))
+    lib().orc_bpt_f32(_dp(c), P, X, rows, T, _org(org), _fp(proj), _fp(out), float(thr[0]))
     return out
 
 
-def bpt_f64(coef, proj, T, org, vol=None):
+def bpt_f64(coef, proj, T, org, vol=None, thr=(0.0, 0.0)):
     proj = np.ascontiguousarray(proj, dtype=np.float64)
     P, X, rows, _ = _dims(proj, T)
     out = np.zeros((rows, T, X), np.float64) if vol is None else np.array(vol, np.float64)
     c = np.ascontiguousarray(coef, dtype=np.float64)
-    lib().orc_bpt_f64(_dp(c), P, X, rows, T, _org(org), _dp(proj), _dp(out))
+    lib().orc_bpt_f64(_dp(c), P, X, rows, T, _org(org), _dp(proj), _dp(out), float(thr[0]))
     return out
 
 
-def _iter_f32(name, coef, proj, T, org, iters, gamma, vol):
+def _iter_f32(name, coef, proj, T, org, iters, gamma, vol, thr):
     proj = np.ascontiguousarray(proj, dtype=np.float32)
     P, X, rows, _ = _dims(proj, T)
     out = np.zeros((rows, T, X), np.float32) if vol is None else np.array(vol, np.float32)
     c = np.ascontiguousarray(coef, dtype=np.float64)
-    getattr(lib(), name)(_dp(c), P, X, rows, T, _org(org), _fp(proj), iters, gamma, _fp(out))
+    getattr(lib(), name)(_dp(c), P, X, rows, T, _org(org), _fp(proj), iters, gamma, _fp(out), float(thr[0]),
+                         float(thr[1]))
     return out
 
 
-def _iter_f64(name, coef, proj, T, org, iters, gamma, vol):
+def _iter_f64(name, coef, proj, T, org, iters, gamma, vol, thr):
     proj = np.ascontiguousarray(proj, dtype=np.float64)
     P, X, rows, _ = _dims(proj, T)
     out = np.zeros((rows, T, X), np.float64) if vol is None else np.array(vol, np.float64)
     c = np.ascontiguousarray(coef, dtype=np.float64)
-    getattr(lib(), name)(_dp(c), P, X, rows, T, _org(org), _dp(proj), iters, gamma, _dp(out))
+    getattr(lib(), name)(_dp(c), P, X, rows, T, _org(org), _dp(proj), iters, gamma, _dp(out), float(thr[0]),
+                         float(thr[1]))
     return out
 
 
-def sirt_f32(coef, proj, T, org, iters, gamma=1.0, vol=None):
-    return _iter_f32("orc_sirt_f32", coef, proj, T, org, iters, gamma, vol)
+def sirt_f32(coef, proj, T, org, iters, gamma=1.0, vol=None, thr=(0.0, 0.0)):
+    """thr = (update, residual) thresholds: (0, 0) TiltRec-mpi, CUDA_THRESHOLDS TiltRec-cuda."""
+    return _iter_f32("orc_sirt_f32", coef, proj, T, org, iters, gamma, vol, thr)
 
 
-def sart_f32(coef, proj, T, org, iters, gamma=1.0, vol=None):
-    return _iter_f32("orc_sart_f32", coef, proj, T, org, iters, gamma, vol)
+def sart_f32(coef, proj, T, org, iters, gamma=1.0, vol=None, thr=(0.0, 0.0)):
+    return _iter_f32("orc_sart_f32", coef, proj, T, org, iters, gamma, vol, thr)
 
 
-def sirt_f64(coef, proj, T, org, iters, gamma=1.0, vol=None):
-    return _iter_f64("orc_sirt_f64", coef, proj, T, org, iters, gamma, vol)
+def sirt_f64(coef, proj, T, org, iters, gamma=1.0, vol=None, thr=(0.0, 0.0)):
+    return _iter_f64("orc_sirt_f64", coef, proj, T, org, iters, gamma, vol, thr)
 
 
-def sart_f64(coef, proj, T, org, iters, gamma=1.0, vol=None):
-    return _iter_f64("orc_sart_f64", coef, proj, T, org, iters, gamma, vol)
+def sart_f64(coef, proj, T, org, iters, gamma=1.0, vol=None, thr=(0.0, 0.0)):
+    return _iter_f64("orc_sart_f64", coef, proj, T, org, iters, gamma, vol, thr)
 
 
 def rweight_f64(rows):
@@ -260,11 +279,11 @@ def fbp_filter_f64(rows, mode=0):
     return a.reshape(shp)
 
 
-def fbp_f32(coef, proj, T, org, mode=0, vol=None):
+def fbp_f32(coef, proj, T, org, mode=0, vol=None, thr=(0.0, 0.0)):
     """TiltRec's FBP: the spatial filter (FP64, rounded to float as the
     reference stores it) followed by the FP32 back-projection."""
     filt = fbp_filter_f64(proj.astype(np.float64), mode).astype(np.float32)
-    return bpt_f32(coef, filt, T, org, vol)
+    return bpt_f32(coef, filt, T, org, vol, thr)
 
 def float_to_byte(values):
     """--f2b conversion (recon_z.cu:253-262): (bytes uint8, (mean, sd))."""

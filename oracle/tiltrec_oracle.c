@@ -26,6 +26,17 @@
  *   *_f64 -- the same loops with every weight and accumulator in double: the
  *            FP64 oracle that states the error of a float result.
  *
+ * Numeric rules.  Every update and residual test takes a threshold that the
+ * weight sum must exceed, compared in double as the reference compares a
+ * float with a double literal:
+ *   TiltRec-mpi:  0 everywhere (its `if (c)` / `if (rw[n])` tests; the sums
+ *                 are never negative), main-mpi.cpp:442,571,600,645;
+ *   TiltRec-cuda: BPT/WBP/FBP c > 1e-5 (basic.cu:409), SIRT wtvol > 10e-6
+ *                 (basic.cu:184) with an exact residual test (:152), SART
+ *                 c > 10e-7 (:434) and reproj_wt > 1e-7 (:170).
+ * TiltRec-cuda also puts the y origin at Y/2 (main-cuda.cpp:259-261), which
+ * callers pass in org[1].
+ *
  * Data layouts follow the TiltRec-mpi y path after RotateX
  * (mrcstack.cpp:566-579): projections are [P][rows][X] and the volume is
  * [rows][T][X] (one XZ section per row, the order WriteBlock('z') writes).
@@ -217,7 +228,7 @@ void orc_reproject_f32(const double *coef, int W, int len, int T,
 /* BackProject, main-mpi.cpp:411-450 (BPT).  proj [P][len][W], vol [len][T][W]
  * is added to (the reference starts from zero or from --initial). */
 void orc_bpt_f32(const double *coef, int P, int W, int len, int T,
-                 const float org[3], const float *proj, float *vol)
+                 const float org[3], const float *proj, float *vol, double thr_upd)
 {
     for (int idx = 0; idx < P; idx++) {
         const float *one = proj + (size_t)W * len * idx;
@@ -230,7 +241,7 @@ void orc_bpt_f32(const double *coef, int P, int W, int len, int T,
                     float s = 0, c = 0;
                     valcoef(org, x, y, z, coef + 8 * (size_t)idx, &w);
                     bilinear_f32(W, len, one, &w, &s, &c);
-                    if (c) vz[x] += (float)(s / c);
+                    if ((double)c > thr_upd) vz[x] += (float)(s / c);
                 }
             }
         }
@@ -240,7 +251,7 @@ void orc_bpt_f32(const double *coef, int P, int W, int len, int T,
 /* SIRT, main-mpi.cpp:614-664 with UpdateWeightsByProjDiff (:526-562). */
 void orc_sirt_f32(const double *coef, int P, int W, int len, int T,
                   const float org[3], const float *proj, int iters, float gamma,
-                  float *vol)
+                  float *vol, double thr_upd, double thr_res)
 {
     size_t px = (size_t)len * W, size = (size_t)len * W * T;
     float *rv = (float *)malloc(sizeof(float) * px);
@@ -257,8 +268,9 @@ void orc_sirt_f32(const double *coef, int P, int W, int len, int T,
             reproject_f32(org, vol, c, rv, rw, T, len, W);
             const float *pd = proj + (size_t)W * len * idx;
             for (size_t n = 0; n < px; n++) {
-                if (rw[n]) rv[n] /= rw[n];
-                rv[n] = pd[n] - rv[n];
+                /* diff = 0 unless the ray weight passes the test (basic.cu:142-175) */
+                const float diff = (double)rw[n] > thr_res ? rv[n] / rw[n] : 0.f;
+                rv[n] = pd[n] - diff;
             }
             for (int y = 0; y < len; y++) {
                 float *vy = valvol + (size_t)y * W * T;
@@ -276,7 +288,7 @@ void orc_sirt_f32(const double *coef, int P, int W, int len, int T,
             }
         }
         for (size_t i = size; i--;) {
-            if (wtvol[i]) vol[i] += valvol[i] / wtvol[i] * gamma;
+            if ((double)wtvol[i] > thr_upd) vol[i] += valvol[i] / wtvol[i] * gamma;
         }
     }
     free(rv);
@@ -288,7 +300,7 @@ void orc_sirt_f32(const double *coef, int P, int W, int len, int T,
 /* SART, main-mpi.cpp:578-612 with UpdateVolumeByProjDiff (:492-524). */
 void orc_sart_f32(const double *coef, int P, int W, int len, int T,
                   const float org[3], const float *proj, int iters, float gamma,
-                  float *vol)
+                  float *vol, double thr_upd, double thr_res)
 {
     size_t px = (size_t)len * W;
     float *rv = (float *)malloc(sizeof(float) * px);
@@ -301,8 +313,9 @@ void orc_sart_f32(const double *coef, int P, int W, int len, int T,
             reproject_f32(org, vol, c, rv, rw, T, len, W);
             const float *pd = proj + (size_t)W * len * idx;
             for (size_t n = 0; n < px; n++) {
-                if (rw[n] != 0) rv[n] /= rw[n];
-                rv[n] = pd[n] - rv[n];
+                /* diff = 0 unless the ray weight passes the test (basic.cu:142-175) */
+                const float diff = (double)rw[n] > thr_res ? rv[n] / rw[n] : 0.f;
+                rv[n] = pd[n] - diff;
             }
             for (int y = 0; y < len; y++) {
                 float *vy = vol + (size_t)y * W * T;
@@ -312,7 +325,7 @@ void orc_sart_f32(const double *coef, int P, int W, int len, int T,
                         float s = 0, cc = 0;
                         valcoef(org, x, y, z, c, &w);
                         bilinear_f32(W, len, rv, &w, &s, &cc);
-                        if (cc) vy[(size_t)z * W + x] += (float)(s / cc) * gamma;
+                        if ((double)cc > thr_upd) vy[(size_t)z * W + x] += (float)(s / cc) * gamma;
                     }
                 }
             }
@@ -419,7 +432,7 @@ void orc_backproject_sums_f64(const double *coef, int W, int len, int T,
 }
 
 void orc_bpt_f64(const double *coef, int P, int W, int len, int T,
-                 const float org[3], const double *proj, double *vol)
+                 const float org[3], const double *proj, double *vol, double thr_upd)
 {
     for (int idx = 0; idx < P; idx++) {
         const double *one = proj + (size_t)W * len * idx;
@@ -430,14 +443,14 @@ void orc_bpt_f64(const double *coef, int P, int W, int len, int T,
                     double s = 0, c = 0;
                     valcoef(org, x, y, z, coef + 8 * (size_t)idx, &w);
                     bilinear_f64(W, len, one, &w, &s, &c);
-                    if (c) vol[((size_t)y * T + z) * W + x] += s / c;
+                    if (c > thr_upd) vol[((size_t)y * T + z) * W + x] += s / c;
                 }
     }
 }
 
 void orc_sirt_f64(const double *coef, int P, int W, int len, int T,
                   const float org[3], const double *proj, int iters, double gamma,
-                  double *vol)
+                  double *vol, double thr_upd, double thr_res)
 {
     size_t px = (size_t)len * W, size = (size_t)len * W * T;
     double *rv = (double *)malloc(sizeof(double) * px);
@@ -454,8 +467,8 @@ void orc_sirt_f64(const double *coef, int P, int W, int len, int T,
             reproject_f64(org, vol, c, rv, rw, T, len, W);
             const double *pd = proj + (size_t)W * len * idx;
             for (size_t n = 0; n < px; n++) {
-                if (rw[n]) rv[n] /= rw[n];
-                rv[n] = pd[n] - rv[n];
+                const double diff = rw[n] > thr_res ? rv[n] / rw[n] : 0.0;
+                rv[n] = pd[n] - diff;
             }
             for (int y = 0; y < len; y++)
                 for (int z = 0; z < T; z++)
@@ -470,7 +483,7 @@ void orc_sirt_f64(const double *coef, int P, int W, int len, int T,
                     }
         }
         for (size_t i = size; i--;) {
-            if (wtvol[i]) vol[i] += valvol[i] / wtvol[i] * gamma;
+            if (wtvol[i] > thr_upd) vol[i] += valvol[i] / wtvol[i] * gamma;
         }
     }
     free(rv);
@@ -481,7 +494,7 @@ void orc_sirt_f64(const double *coef, int P, int W, int len, int T,
 
 void orc_sart_f64(const double *coef, int P, int W, int len, int T,
                   const float org[3], const double *proj, int iters, double gamma,
-                  double *vol)
+                  double *vol, double thr_upd, double thr_res)
 {
     size_t px = (size_t)len * W;
     double *rv = (double *)malloc(sizeof(double) * px);
@@ -494,8 +507,8 @@ void orc_sart_f64(const double *coef, int P, int W, int len, int T,
             reproject_f64(org, vol, c, rv, rw, T, len, W);
             const double *pd = proj + (size_t)W * len * idx;
             for (size_t n = 0; n < px; n++) {
-                if (rw[n] != 0) rv[n] /= rw[n];
-                rv[n] = pd[n] - rv[n];
+                const double diff = rw[n] > thr_res ? rv[n] / rw[n] : 0.0;
+                rv[n] = pd[n] - diff;
             }
             for (int y = 0; y < len; y++)
                 for (int z = 0; z < T; z++)
@@ -504,7 +517,7 @@ void orc_sart_f64(const double *coef, int P, int W, int len, int T,
                         double s = 0, cc = 0;
                         valcoef(org, x, y, z, c, &w);
                         bilinear_f64(W, len, rv, &w, &s, &cc);
-                        if (cc) vol[((size_t)y * T + z) * W + x] += s / cc * gamma;
+                        if (cc > thr_upd) vol[((size_t)y * T + z) * W + x] += s / cc * gamma;
                     }
         }
     }

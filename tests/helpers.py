@@ -35,3 +35,42 @@ def noisy_stack(orc, coef, vol, org, seed, noise=0.1):
 def to_layout_z(vol_y):
     """[Y][T][X] -> [T][Y][X]"""
     return np.ascontiguousarray(np.transpose(vol_y, (1, 0, 2)))
+
+
+def voxel_u(coef, x, z, X, T):
+    """Detector column u of voxel (x, z) at pitch 0 for each row of `coef`,
+    in ValCoef's double operations (main-mpi.cpp:204-226)."""
+    ox, oz = np.float32(X * 0.5), np.float32(T * 0.5)
+    Xc = float(np.float32(np.float32(x) - ox))
+    Zc = float(np.float32(np.float32(z) - oz))
+    c = np.asarray(coef, np.float64)
+    return ((c[:, 0] + c[:, 1] * Xc) + c[:, 2] * 0.0 + c[:, 3] * Zc) + float(ox)
+
+
+def edge_tilt(orc, X, T, wmin, wmax, zshift=0.0):
+    """A float tilt angle near 3 degrees at which corner voxel (0, T-1) of an
+    X-wide, T-thick volume maps just right of detector column -1: its only
+    in-bounds tap (column 0) weighs w = frac(u) with wmin < w <= wmax.  That
+    weight is then the voxel's whole weight sum for the tilt, which is what
+    TiltRec-cuda's thresholds test (basic.cu:184,409,434).  The root of
+    u = -1 is bracketed by bisection, then neighbouring float angles are
+    scanned for a qualifying weight."""
+    f = lambda a: voxel_u(orc.coeffs_mpi(np.array([a], np.float32), 0.0, 0.0, zshift), 0, T - 1, X, T)[0] + 1.0
+    lo, hi = 0.5, 30.0
+    assert f(lo) > 0 > f(hi)
+    for _ in range(60):
+        m = 0.5 * (lo + hi)
+        lo, hi = (m, hi) if f(m) > 0 else (lo, m)
+    base = np.float32(lo)
+    cand = [base]
+    up = dn = base
+    for _ in range(4000):
+        up = np.nextafter(up, np.float32(90))
+        dn = np.nextafter(dn, np.float32(0))
+        cand += [up, dn]
+    ang = np.array(cand, np.float32)
+    u = voxel_u(orc.coeffs_mpi(ang, 0.0, 0.0, zshift), 0, T - 1, X, T)
+    w = (u - np.floor(u)).astype(np.float32)
+    ok = (np.floor(u) == -1) & (w > wmin) & (w <= wmax)
+    assert ok.any(), "no float angle gives the wanted tap weight"
+    return np.float32(ang[ok][np.argmax(w[ok])])

@@ -251,14 +251,67 @@ def test_16bit_storage_vs_fp64(torch_cuda, trec, ctx, orc, storage, bound):
     assert e < bound
 
 
-def test_cuda_rules_thresholds(torch_cuda, trec, ctx, orc):
-    """TREC_RULES_CUDA skips voxels whose weight sum is at most 1e-5
-    (basic.cu:184,409); interior voxels are unchanged."""
-    P, Y, T, X = 9, 16, 12, 40
-    ang, coef, org, stack = case(orc, P, Y, T, X, seed=14)
-    a = run(torch_cuda, trec, ctx, stack, ang, T, "BPT", rules=0)
-    b = run(torch_cuda, trec, ctx, stack, ang, T, "BPT", rules=1)
-    np.testing.assert_allclose(a[:, :, 12:28], b[:, :, 12:28], rtol=1e-6, atol=1e-6)
+def _cuda_ref(orc, method, coef, stack, T, org, rules):
+    """The FP32 oracle under either binary's thresholds."""
+    name = method.split(",")[0]
+    thr = orc.CUDA_THRESHOLDS[name] if rules == 1 else (0.0, 0.0)
+    if name == "BPT":
+        return orc.bpt_f32(coef, stack, T, org, thr=thr)
+    if name == "WBP":
+        return orc.bpt_f32(coef, orc.rweight_f64(stack.astype(np.float64)).astype(np.float32), T, org, thr=thr)
+    if name == "FBP":
+        return orc.fbp_f32(coef, stack, T, org, thr=thr)
+    it, gam = int(method.split(",")[1]), float(method.split(",")[2])
+    fn = orc.sirt_f32 if name == "SIRT" else orc.sart_f32
+    return fn(coef, stack, T, org, it, gam, thr=thr)
+
+
+@pytest.mark.parametrize("method", ["BPT", "WBP", "FBP", "SIRT,3,1.0", "SART,2,0.7"])
+def test_cuda_rules_tiny_weights(torch_cuda, trec, ctx, orc, method):
+    """TiltRec-cuda's thresholds where they change the volume.  One tilt maps
+    corner voxel (0, T-1) just right of detector column -1, so its only tap
+    (column 0) weighs w in (1e-6, 1e-5] (SART: (1e-7, 1e-6]); the other
+    tilts cannot see that voxel at all.  Under TREC_RULES_CUDA the voxel
+    skips that tilt (c > 1e-5, basic.cu:409; SIRT wtvol > 10e-6, :184; SART
+    c > 10e-7, :434); under TREC_RULES_MPI it takes the full s / c.  Both
+    rules must match the oracle at those voxels and over the volume."""
+    from tests.helpers import edge_tilt, voxel_u
+    X, T, Y = 32, 40, 4
+    name = method.split(",")[0]
+    lo, hi = (1e-7, 1e-6) if name == "SART" else (1e-6, 1e-5)
+    te = edge_tilt(orc, X, T, lo, hi)
+    ang = np.array([te, 20.0, 27.0, 35.0, 45.0, 52.0, 60.0], np.float32)
+    coef = orc.coeffs_mpi(ang)
+    u = voxel_u(coef, 0, T - 1, X, T)
+    assert -1 < u[0] <= -1 + hi and np.all(u[1:] < -1)
+    org = orc.origin(X, T, T / 2)
+    stack = np.random.default_rng(31).random((len(ang), Y, X)).astype(np.float32) + 0.5
+    refs = {r: _cuda_ref(orc, method, coef, stack, T, org, r) for r in (0, 1)}
+    corner = (slice(None), T - 1, 0)
+    gap = np.abs(refs[0][corner] - refs[1][corner])
+    assert np.all(gap > 1e-3 * np.abs(refs[1]).max()), f"the rules do not differ at the corner: {gap}"
+    for rules in (0, 1):
+        got = run(torch_cuda, trec, ctx, stack, ang, T, method, rules=rules)
+        e = report(f"{method} rules {rules}", got, refs[rules])
+        assert e < TOL_F32
+        np.testing.assert_allclose(got[corner], refs[rules][corner], rtol=1e-5, atol=1e-5,
+                                   err_msg=f"corner voxels, rules {rules}")
+
+
+@pytest.mark.parametrize("method", ["BPT", "WBP", "SIRT,3,1.0", "SART,1,0.7"])
+def test_cuda_rules_pitch_origin(torch_cuda, trec, ctx, orc, method):
+    """With a pitch, TiltRec-cuda's y origin (Y/2, main-cuda.cpp:259-261)
+    differs from TiltRec-mpi's (T/2, main-mpi.cpp:743-745): the general path
+    under TREC_RULES_CUDA must follow Y/2 and the CUDA thresholds."""
+    P, Y, T, X = 9, 24, 10, 40
+    ang = tilts(P)
+    coef = orc.coeffs_mpi(ang, 0.0, 4.0, 0.0)
+    stack = noisy_stack(orc, coef, smooth_volume(Y, T, X, 41), orc.origin(X, T, Y / 2), 42)
+    ref_cuda = _cuda_ref(orc, method, coef, stack, T, orc.rules_origin(X, Y, T, 1), 1)
+    ref_mpi = _cuda_ref(orc, method, coef, stack, T, orc.rules_origin(X, Y, T, 0), 1)
+    assert orc.rel_l2(ref_mpi, ref_cuda) > 1e-3  # the origin matters here
+    got = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=4.0, rules=1)
+    assert report(f"{method} pitch 4 rules CUDA", got, ref_cuda) < TOL_F32
 
 
 def test_bad_arguments_fail_loudly(torch_cuda, trec, ctx):

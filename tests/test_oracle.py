@@ -199,6 +199,104 @@ def test_sirt_sart_match_matrix_form(orc, pitch):
     assert orc.rel_l2(g32, _matrix_sirt(Ws, ps, x0, 3, 0.8).reshape(g32.shape)) < 1e-6
 
 
+def _cut(values, frac):
+    """A threshold midway between two neighbouring distinct positive values,
+    about `frac` of the way up their sorted list, so that rounding in how a
+    sum was formed cannot move a value across it."""
+    v = np.unique(values[values > 0])
+    k = int(frac * (len(v) - 1))
+    while k + 1 < len(v) and v[k + 1] - v[k] < 1e-9 * v[k + 1]:
+        k += 1
+    return 0.5 * (v[k] + v[k + 1])
+
+
+def _matrix_sirt_thr(Ws, ps, x, iters, gamma, tu, tr):
+    for _ in range(iters):
+        val = np.zeros_like(x)
+        wt = np.zeros_like(x)
+        for W, p in zip(Ws, ps):
+            rw = W.sum(1)
+            q = np.where(rw > tr, (W @ x) / np.where(rw > tr, rw, 1), 0.0)
+            val += W.T @ (p - q)
+            wt += W.sum(0)
+        x = x + gamma * np.where(wt > tu, val / np.where(wt > tu, wt, 1), 0)
+    return x
+
+
+def _matrix_sart_thr(Ws, ps, x, iters, gamma, tu, tr):
+    for _ in range(iters):
+        for W, p in zip(Ws, ps):
+            rw = W.sum(1)
+            r = p - np.where(rw > tr, (W @ x) / np.where(rw > tr, rw, 1), 0.0)
+            c = W.sum(0)
+            x = x + gamma * np.where(c > tu, (W.T @ r) / np.where(c > tu, c, 1), 0)
+    return x
+
+
+@pytest.mark.parametrize("pitch,rules", [(0.0, 0), (5.0, 0), (5.0, 1)])
+def test_thresholds_match_masked_matrix_forms(orc, pitch, rules):
+    """The update and residual thresholds (TiltRec-cuda's 1e-5 / 1e-6 / 1e-7
+    tests, basic.cu:170,184,409,434) equal the matrix forms with every
+    weight sum at or below its threshold masked out.  The thresholds here
+    sit inside the spread of a thick volume's weight sums so that the masks
+    cut many voxels and pixels; rules 1 puts the y origin at Y/2 as the CUDA
+    binaries do (main-cuda.cpp:259-261)."""
+    rng = np.random.default_rng(15)
+    P, rows, T, X = 5, 4, 12, 8
+    ang = tilt_set(P, 50.0)
+    c = orc.coeffs_mpi(ang, 0.0, pitch, 0.0)
+    org = orc.rules_origin(X, rows, T, rules)
+    proj = rng.random((P, rows, X))
+    Ws = [sysmat(c[k], X, rows, T, org) for k in range(P)]
+    ps = [proj[k].ravel() for k in range(P)]
+    C = sum(W.sum(0) for W in Ws)
+    c1 = np.concatenate([W.sum(0) for W in Ws])
+    rw = np.concatenate([W.sum(1) for W in Ws])
+    tu_sirt, tu_sart, tr = _cut(C, 0.3), _cut(c1, 0.3), _cut(rw, 0.3)
+    assert ((C > 0) & (C <= tu_sirt)).sum() > 10 and ((rw > 0) & (rw <= tr)).sum() > 5
+    x0 = np.zeros(rows * T * X)
+    ref = _matrix_sirt_thr(Ws, ps, x0, 3, 0.8, tu_sirt, tr)
+    got = orc.sirt_f64(c, proj, T, org, 3, 0.8, thr=(tu_sirt, tr))
+    np.testing.assert_allclose(got.ravel(), ref, rtol=1e-11, atol=1e-11)
+    assert not np.allclose(ref, _matrix_sirt_thr(Ws, ps, x0, 3, 0.8, 0.0, 0.0))  # the masks matter
+    ref = _matrix_sart_thr(Ws, ps, x0, 2, 0.7, tu_sart, tr)
+    got = orc.sart_f64(c, proj, T, org, 2, 0.7, thr=(tu_sart, tr))
+    np.testing.assert_allclose(got.ravel(), ref, rtol=1e-11, atol=1e-11)
+    bref = sum(np.where(W.sum(0) > tu_sart, (W.T @ p) / np.where(W.sum(0) > tu_sart, W.sum(0), 1), 0)
+               for W, p in zip(Ws, ps))
+    np.testing.assert_allclose(orc.bpt_f64(c, proj, T, org, thr=(tu_sart, 0.0)).ravel(), bref, rtol=1e-12, atol=1e-12)
+    g32 = orc.sirt_f32(c, proj.astype(np.float32), T, org, 3, 0.8, thr=(tu_sirt, tr))
+    assert orc.rel_l2(g32, _matrix_sirt_thr(Ws, ps, x0, 3, 0.8, tu_sirt, tr).reshape(g32.shape)) < 1e-6
+
+
+def test_cuda_thresholds_known_answer(orc):
+    """A stack of ones and a tilt at which corner voxel (0, T-1) reaches
+    column 0 only, with weight w in (1e-6, 1e-5]: BPT adds s / c = 1 for that
+    tilt under TiltRec-mpi's exact test and nothing under TiltRec-cuda's
+    c > 1e-5 (basic.cu:409).  With only tilts that cannot see the voxel
+    besides, SIRT's weight sum is w as well: the voxel moves by the residual
+    under the exact test and stays 0 under wtvol > 10e-6 (basic.cu:184)."""
+    from tests.helpers import edge_tilt, voxel_u
+    X, T, rows = 32, 40, 2
+    te = edge_tilt(orc, X, T, 1e-6, 1e-5)
+    ang = np.array([te, 20.0, 30.0, 45.0, 60.0], np.float32)
+    c = orc.coeffs_mpi(ang)
+    u = voxel_u(c, 0, T - 1, X, T)
+    assert -1 < u[0] <= -1 + 1e-5 and np.all(u[1:] < -1)   # only tilt 0 sees the voxel
+    org = orc.origin(X, T, T / 2)
+    ones = np.ones((len(ang), rows, X), np.float32)
+    mpi = orc.bpt_f32(c, ones, T, org)
+    cuda = orc.bpt_f32(c, ones, T, org, thr=orc.CUDA_THRESHOLDS["BPT"])
+    assert np.all(mpi[:, T - 1, 0] == 1.0) and np.all(cuda[:, T - 1, 0] == 0.0)
+    same = np.ones_like(mpi, bool)
+    same[:, T - 1, 0] = False
+    np.testing.assert_array_equal(mpi[same], cuda[same])
+    proj = np.random.default_rng(3).random((len(ang), rows, X)).astype(np.float32) + 0.5
+    s_mpi = orc.sirt_f32(c, proj, T, org, 1)
+    s_cuda = orc.sirt_f32(c, proj, T, org, 1, thr=orc.CUDA_THRESHOLDS["SIRT"])
+    assert np.all(np.abs(s_mpi[:, T - 1, 0]) > 0.1) and np.all(s_cuda[:, T - 1, 0] == 0.0)
+
+
 def test_sirt_initial_volume_is_continued(orc):
     """--initial seeds the iteration: k then m iterations equal k+m."""
     rng = np.random.default_rng(6)
