@@ -133,6 +133,7 @@ struct trec_context {
     hipStream_t arena_stream = nullptr;  // stream of the call holding the arena
     bool arena_held = false;
     hipStream_t cap = nullptr;  // capture stream
+    size_t ws_limit = 0;        // workspace budget of one call (0: the free device memory)
     hipGraphExec_t gexec = nullptr;
     std::string gkey;
     std::map<std::string, int> tuned;  // nzc per geometry
@@ -434,6 +435,9 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
         h.T = g.thickness;
         h.P = g.nproj;
         h.nyb = (g.ny + RB - 1) / RB;
+        p.nyb_total = h.nyb;
+        p.slab_rb = h.nyb;
+        p.nslab = 1;
         h.nzc = 1;
         if (m.method == TREC_SIRT) h.nzc = force_nzc > 0 ? std::min(force_nzc, g.thickness) : pick_nzc(ctx, g, p.m.storage);
         h.zc_len = (g.thickness + h.nzc - 1) / h.nzc;
@@ -507,7 +511,8 @@ size_t carve(JobPlan &p, char *base)
         JobDesc &h = p.h;
         const size_t es = p.m.storage == TREC_STORE_F32 ? 4 : 2;
         const size_t nyb = h.nyb;
-        p.d = (JobDesc *)take(sizeof(JobDesc));
+        p.d_all = (JobDesc *)take(sizeof(JobDesc) * p.nslab);
+        p.d = p.d_all;
         p.d_ones = (JobDesc *)take(sizeof(JobDesc));
         h.tc = (TiltConst *)take(sizeof(TiltConst) * P);
         // the forward projection's band loads may run up to FGS cells
@@ -618,14 +623,22 @@ trec_status upload(trec_context *ctx, const std::vector<JobPlan *> &plans, hipSt
     };
     std::vector<Piece> pcs;
     std::vector<JobDesc> ones(plans.size());
+    std::vector<std::vector<JobDesc>> slabs(plans.size());
     for (size_t i = 0; i < plans.size(); ++i) {
         JobPlan &p = *plans[i];
         if (p.path == PATH_ROWS) {
             ones[i] = p.h;
+            ones[i].y0 = 0;
             ones[i].nyb = 1;
             ones[i].nzc = 1;
             ones[i].zc_len = p.h.T;
-            pcs.push_back({p.d, &p.h, sizeof(JobDesc)});
+            for (int sl = 0; sl < p.nslab; ++sl) {
+                JobDesc hs = p.h;
+                hs.y0 = sl * p.slab_rb * RB;
+                hs.nyb = std::min(p.slab_rb, p.nyb_total - sl * p.slab_rb);
+                slabs[i].push_back(hs);
+            }
+            pcs.push_back({p.d_all, slabs[i].data(), sizeof(JobDesc) * slabs[i].size()});
             pcs.push_back({p.d_ones, &ones[i], sizeof(JobDesc)});
             if (p.m.method == TREC_ADMM) {
                 // 0: htb = A^T b (the stack as the image, no x term);
@@ -670,6 +683,37 @@ trec_status upload(trec_context *ctx, const std::vector<JobPlan *> &plans, hipSt
     }
     HIPCHK(ctx, hipEventRecord(ctx->staging_ev, st));
     return TREC_OK;
+}
+
+bool split_slabs(JobPlan &p, size_t limit)
+{
+    if (p.path != PATH_ROWS || p.m.method == TREC_ADMM) return carve(p, nullptr) <= limit;
+    auto bytes_for = [&](int rb) {
+        p.h.nyb = rb;
+        p.slab_rb = rb;
+        p.nslab = (p.nyb_total + rb - 1) / rb;
+        return carve(p, nullptr);
+    };
+    int lo = 1, hi = p.nyb_total;
+    if (bytes_for(hi) <= limit) return true;
+    if (bytes_for(lo) > limit) {
+        bytes_for(p.nyb_total);
+        return false;
+    }
+    while (lo < hi) {  // largest slab whose workspace fits
+        const int mid = lo + (hi - lo + 1) / 2;
+        if (bytes_for(mid) <= limit) lo = mid;
+        else hi = mid - 1;
+    }
+    bytes_for(lo);
+    return true;
+}
+
+void select_slab(JobPlan &p, int s)
+{
+    p.h.y0 = s * p.slab_rb * RB;
+    p.h.nyb = std::min(p.slab_rb, p.nyb_total - s * p.slab_rb);
+    p.d = p.d_all + s;
 }
 
 // ------------------------------------------------------------------ issue
@@ -990,6 +1034,19 @@ hipError_t issue_admm_general(JobPlan &p, hipStream_t st)
 
 hipError_t issue_all(const std::vector<JobPlan *> &plans, hipStream_t st)
 {
+    if (plans.size() == 1 && plans[0]->nslab > 1) {
+        // one job in y slabs, one after another in the same workspace
+        JobPlan &p = *plans[0];
+        for (int sl = 0; sl < p.nslab; ++sl) {
+            select_slab(p, sl);
+            LCHK(issue_prep(p, st));
+            std::vector<int> stage(1, 0);
+            for (int s = 0; s < p.stages; ++s) LCHK(issue_step(plans, stage, st));
+            LCHK(issue_finish(p, st));
+        }
+        select_slab(p, 0);
+        return hipSuccess;
+    }
     for (auto *p : plans) LCHK(issue_prep(*p, st));
     for (auto *p : plans)
         if (p->m.method == TREC_ADMM) LCHK(p->path == PATH_ROWS ? issue_admm(*p, st) : issue_admm_general(*p, st));
@@ -1034,11 +1091,31 @@ std::string graph_key(trec_context *ctx, const std::vector<JobPlan *> &plans)
     return k;
 }
 
-// Allocates, uploads and issues a list of planned jobs on `st`.
+// Workspace one call may take: the context's limit, or 90 % of what the
+// device has free plus the arena the context already holds.
+size_t workspace_budget(trec_context *ctx)
+{
+    if (ctx->ws_limit) return ctx->ws_limit;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        (void)hipGetLastError();
+        return SIZE_MAX;
+    }
+    return (fr + ctx->arena_size) / 10 * 9;
+}
+
+// Allocates, uploads and issues a list of planned jobs on `st`.  A single
+// row-path job whose workspace exceeds the budget runs in y slabs.
 trec_status run_jobs(trec_context *ctx, std::vector<JobPlan> &plan_store, hipStream_t st, bool use_graph)
 {
     std::vector<JobPlan *> plans;
     for (auto &p : plan_store) plans.push_back(&p);
+    if (plans.size() == 1) {
+        const size_t budget = workspace_budget(ctx);
+        if (carve(*plans[0], nullptr) > budget && !split_slabs(*plans[0], budget))
+            return fail(ctx, TREC_ERR_NOMEM, "workspace of one row block exceeds the budget of %zu bytes", budget);
+        if (plans[0]->nslab > 1) use_graph = false;
+    }
     size_t total = 0;
     for (auto *p : plans) total += carve(*p, nullptr);
     trec_status s = ensure_arena(ctx, total, st);
@@ -1706,6 +1783,41 @@ trec_status trec_sirt_iterate(trec_context *ctx, const trec_geom *geom, const fl
 {
     trec_method m{TREC_SIRT, 1, gamma, TREC_STORE_F32, TREC_LAYOUT_Y};
     return trec_reconstruct(ctx, geom, angles_deg, stack, volume, &m, volume, stream);
+}
+
+trec_status trec_set_workspace_limit(trec_context *ctx, size_t bytes)
+{
+    if (!ctx) return TREC_ERR_INVALID;
+    ctx->ws_limit = bytes;
+    return TREC_OK;
+}
+
+trec_status trec_plan(trec_context *ctx, const trec_geom *geom, const float *angles_deg, const trec_method *method,
+                      trec_plan_info *info)
+{
+    if (!ctx || !info) return TREC_ERR_INVALID;
+    ctx->err.clear();
+    trec_status s = validate(ctx, geom, method, angles_deg);
+    if (s != TREC_OK) return s;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    JobPlan p;
+    s = plan_job(ctx, p, *geom, *method, angles_deg, nullptr, nullptr, nullptr);
+    if (s != TREC_OK) return s;
+    const size_t budget = workspace_budget(ctx);
+    if (carve(p, nullptr) > budget && !split_slabs(p, budget))
+        return fail(ctx, TREC_ERR_NOMEM, "workspace of one row block exceeds the budget of %zu bytes", budget);
+    memset(info, 0, sizeof *info);
+    info->path = p.path;
+    info->workspace = carve(p, nullptr);
+    info->slabs = p.nslab;
+    info->slab_rows = p.path == PATH_ROWS ? std::min(p.slab_rb * RB, geom->ny) : geom->ny;
+    if (p.path == PATH_ROWS) {
+        info->zchunks = p.h.nzc;
+        if (uses_forward(p.m.method))
+            info->band_cells = (p.m.method == TREC_SIRT && p.h.nzc == 1 && p.fg_small) ? FGS_SMALL : FGS;
+        info->back_tilts = p.m.storage == TREC_STORE_F32 ? BP_G32 : BP_G;
+    }
+    return TREC_OK;
 }
 
 trec_status trec_request_footprint(const trec_geom *geom, const trec_method *method, size_t *bytes)

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(WG) void k_block_stack(const JobDesc *__restrict__ 
     const float inv = 1.0f / J.scale;
     for (int k = threadIdx.x; k < RB * 64; k += WG) {
         const int r = k / 64, xx = k % 64;
-        const int y = yb * RB + r, x = x0 + xx;
+        const int y = J.y0 + yb * RB + r, x = x0 + xx;
         float v = 0.f;
         if (y < J.Y && x < J.X) v = stack[((size_t)th * J.Y + y) * J.X + x] * inv;
         tile[r][xx] = v;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(WG) void k_block_volume(const JobDesc *__restrict__
     const float inv = 1.0f / J.scale;
     for (int k = threadIdx.x; k < RB * 64; k += WG) {
         const int r = k / 64, xx = k % 64;
-        const int y = yb * RB + r, x = x0 + xx;
+        const int y = J.y0 + yb * RB + r, x = x0 + xx;
         float v = 0.f;
         if (y < J.Y && x < J.X) {
             const size_t o = layout == TREC_LAYOUT_Y ? ((size_t)y * J.T + z) * J.X + x : ((size_t)z * J.Y + y) * J.X + x;
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(WG) void k_unblock_volume(const JobDesc *__restrict
     __syncthreads();
     for (int k = threadIdx.x; k < RB * 64; k += WG) {
         const int r = k / 64, xx = k % 64;
-        const int y = yb * RB + r, x = x0 + xx;
+        const int y = J.y0 + yb * RB + r, x = x0 + xx;
         if (y < J.Y && x < J.X) {
             const size_t o = layout == TREC_LAYOUT_Y ? ((size_t)y * J.T + z) * J.X + x : ((size_t)z * J.Y + y) * J.X + x;
             dst[o] = tile[r][xx] * J.scale;

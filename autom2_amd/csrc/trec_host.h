@@ -24,9 +24,16 @@ struct JobPlan {
     int stages = 0;         // SIRT: iterations; SART: iterations * tilts; BPT/WBP: 1
     bool want_rwt = false;  // keep the raw ray weights (forward-projection API)
     std::vector<double> coef;
-    // row-blocked path
+    // row-blocked path.  A job may run as nslab y slabs of slab_rb row
+    // blocks, one after another in the same workspace (single-job calls whose
+    // whole workspace exceeds the limit): h and d then describe the slab being
+    // issued (select_slab), d_all holds every slab's descriptor.
     JobDesc h{};
     JobDesc *d = nullptr;
+    JobDesc *d_all = nullptr;
+    int nyb_total = 0;
+    int slab_rb = 0;
+    int nslab = 1;
     JobDesc *d_ones = nullptr;
     std::vector<TiltConst> tc;
     bool fg_small = false;  // every union of the forward fits the FGS_SMALL band
@@ -72,6 +79,12 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
 size_t carve(JobPlan &p, char *base);
 // Copies the plans' descriptors to the device through pinned staging.
 trec_status upload(trec_context *ctx, const std::vector<JobPlan *> &plans, hipStream_t st);
+
+// Splits a row-path job into y slabs whose workspace fits `limit` bytes
+// (false: not even one row block fits).  Only single-job calls use slabs.
+bool split_slabs(JobPlan &p, size_t limit);
+// Points h and d at slab s.
+void select_slab(JobPlan &p, int s);
 
 hipError_t issue_prep(JobPlan &p, hipStream_t st);
 // Advances every job whose stage[j] < stages by one stage, merging the jobs'

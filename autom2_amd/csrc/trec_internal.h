@@ -80,8 +80,9 @@ enum PathKind { PATH_ROWS = 0, PATH_GENERAL = 1 };
 
 // One reconstruction job as the kernels see it (device memory).
 struct JobDesc {
-    int X, Y, T, P;
-    int nyb;       // row blocks
+    int X, Y, T, P;  // Y: rows of the caller's stack and volume
+    int y0;        // first caller row of this slab (slab-wise jobs; else 0)
+    int nyb;       // row blocks of the slab (the whole job when y0 = 0 and one slab)
     int nzc;       // z chunks of the forward projection (1 = fused residual)
     int zc_len;    // layers per z chunk
     int storage;   // trec_storage of proj / res

@@ -47,6 +47,12 @@ class Method(ctypes.Structure):
                 ("soft", ctypes.c_float)]
 
 
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("path", ctypes.c_int), ("zchunks", ctypes.c_int), ("band_cells", ctypes.c_int),
+                ("back_tilts", ctypes.c_int), ("slabs", ctypes.c_int), ("slab_rows", ctypes.c_int),
+                ("workspace", ctypes.c_size_t)]
+
+
 class Job(ctypes.Structure):
     _fields_ = [("geom", Geom), ("method", Method), ("angles_deg", ctypes.POINTER(ctypes.c_float)),
                 ("stack", ctypes.c_void_p), ("initial", ctypes.c_void_p), ("volume", ctypes.c_void_p)]
@@ -84,6 +90,8 @@ SIGNATURES = [
     ("trec_backproject", _I, [_P, ctypes.POINTER(Geom), _PF, _P, _P, _P]),
     ("trec_sirt_iterate", _I, [_P, ctypes.POINTER(Geom), _PF, _P, ctypes.c_float, _P, _P]),
     ("trec_reconstruct_batch", _I, [_P, _I, ctypes.POINTER(Job), _P]),
+    ("trec_set_workspace_limit", _I, [_P, ctypes.c_size_t]),
+    ("trec_plan", _I, [_P, ctypes.POINTER(Geom), _PF, ctypes.POINTER(Method), ctypes.POINTER(PlanInfo)]),
     ("trec_autotune", _I, [_P, ctypes.POINTER(Geom), ctypes.POINTER(Method)]),
     ("trec_profile_enable", _I, [_P, _I]),
     ("trec_profile_read", _I, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
@@ -319,6 +327,18 @@ class Context:
         pa, _keep = _angles(angles)
         _check(lib().trec_sirt_iterate(self._h, ctypes.byref(geom), pa, _ptr(stack), gamma, _ptr(volume),
                                        _stream(stream)), self._h)
+
+    def set_workspace_limit(self, nbytes):
+        """Workspace budget of a call (0: the free device memory); larger
+        pitch-0 series run in y slabs."""
+        _check(lib().trec_set_workspace_limit(self._h, int(nbytes)), self._h)
+
+    def plan(self, geom, angles, method):
+        """How reconstruct() will run the series: a dict of trec_plan_info."""
+        info = PlanInfo()
+        pa, _keep = _angles(angles)
+        _check(lib().trec_plan(self._h, ctypes.byref(geom), pa, ctypes.byref(method), ctypes.byref(info)), self._h)
+        return {f: getattr(info, f) for f, _ in PlanInfo._fields_}
 
     def autotune(self, geom, method):
         _check(lib().trec_autotune(self._h, ctypes.byref(geom), ctypes.byref(method)), self._h)

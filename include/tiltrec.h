@@ -158,6 +158,31 @@ trec_status trec_coefficients(const trec_geom *geom, const float *angles_deg,
 trec_status trec_workspace_size(const trec_geom *geom, const trec_method *method,
                                 size_t *bytes);
 
+/* Workspace budget of one trec_reconstruct call on this context, bytes
+ * (0, the default: 90 % of the device memory free at the call plus the
+ * workspace the context already holds).  A pitch-0 series whose workspace
+ * (trec_workspace_size) exceeds it is reconstructed in y slabs, one after
+ * another in a workspace that fits, as TiltRec-cuda streams y slabs
+ * (recon_y.cu:33-62, 312-420); the volume is the same, bit for bit. */
+trec_status trec_set_workspace_limit(trec_context *ctx, size_t bytes);
+
+/* How trec_reconstruct will run a series on this context (after any
+ * autotuning), for tools and benchmarks. */
+typedef struct {
+    int path;          /* 0: row-blocked (pitch 0, |tilt| <= ~78 deg), 1: general */
+    int zchunks;       /* z chunks of the SIRT forward projection; 1: the
+                          forward writes the residual itself */
+    int band_cells;    /* LDS band of the tilt-group forward: 352 or 400 cells
+                          (0: no forward projection) */
+    int back_tilts;    /* tilts the row-path back-projection stages per pass */
+    int slabs;         /* y slabs the volume is reconstructed in */
+    int slab_rows;     /* volume rows per slab */
+    size_t workspace;  /* device workspace bytes of the call */
+} trec_plan_info;
+trec_status trec_plan(trec_context *ctx, const trec_geom *geom,
+                      const float *angles_deg, const trec_method *method,
+                      trec_plan_info *info);
+
 /* Reconstructs one tilt series.  stack is read only.  initial (may be NULL)
  * seeds iterative methods and is the starting volume of BPT/WBP, as
  * --initial does in TiltRec-mpi (main-mpi.cpp:778-791); it may alias

@@ -386,3 +386,41 @@ def test_pitch_tiny_shapes(torch_cuda, trec, ctx, orc, shape):
     assert report(f"pitch 5 SIRT x2 {shape}", got, orc.sirt_f32(coef, stack, T, org, 2, 1.0)) < TOL_F32
     got = run(torch_cuda, trec, ctx, stack, ang, T, "BPT", pitch=5.0)
     assert report(f"pitch 5 BPT {shape}", got, orc.bpt_f32(coef, stack, T, org)) < TOL_F32
+
+
+@pytest.mark.parametrize("method,storage,layout", [("SIRT,3,1.0", "f32", 0), ("SIRT,3,1.0", "f16", 1),
+                                                   ("SART,1,0.7", "f32", 0), ("WBP", "f32", 1),
+                                                   ("BPT", "f32", 0)])
+def test_slabs_equal_whole_volume(torch_cuda, trec, orc, method, storage, layout):
+    """A workspace limit below the whole job's workspace makes trec_reconstruct
+    run the volume in y slabs, one after another (the last one partial here);
+    the volume must equal the one-slab result bit for bit and the oracle."""
+    torch = torch_cuda
+    P, Y, T, X = 9, 83, 12, 40
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=51)
+    init = smooth_volume(Y, T, X, 52)
+    ctx = trec.Context(0)
+    whole = run(torch, trec, ctx, stack, ang, T, method, initial=to_layout_z(init) if layout else init,
+                layout=layout, storage=storage)
+    g = trec.make_geom(X, Y, P, T)
+    m = trec.make_method(method, storage=storage, layout=layout)
+    ws = trec.workspace_size(g, m)
+    ctx.set_workspace_limit(ws // 3)
+    info = ctx.plan(g, ang, m)
+    print(f"{method} {storage} layout {layout}: {info}")
+    assert info["slabs"] >= 3 and info["workspace"] <= ws // 3
+    sl = run(torch, trec, ctx, stack, ang, T, method, initial=to_layout_z(init) if layout else init,
+             layout=layout, storage=storage)
+    ctx.close()
+    np.testing.assert_array_equal(sl, whole)
+    if storage == "f32":
+        name = method.split(",")[0]
+        if name in ("SIRT", "SART"):
+            it, gam = int(method.split(",")[1]), float(method.split(",")[2])
+            ref = (orc.sirt_f32 if name == "SIRT" else orc.sart_f32)(coef, stack, T, org, it, gam, vol=init)
+        elif name == "WBP":
+            ref = orc.bpt_f32(coef, orc.rweight_f64(stack.astype(np.float64)).astype(np.float32), T, org, vol=init)
+        else:
+            ref = orc.bpt_f32(coef, stack, T, org, vol=init)
+        got = np.transpose(sl, (1, 0, 2)) if layout else sl
+        assert report(f"{method} in slabs", got, ref) < TOL_F32
