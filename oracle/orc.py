@@ -34,51 +34,61 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        L.orc_coeffs_mpi.argtypes = [_F, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _D]
-        L.orc_coeffs_cuda.argtypes = [_F, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _F]
-        _DB = ctypes.c_double
-        L.orc_bpt_f32.argtypes = [_D, _I, _I, _I, _I, _F, _F, _F, _DB]
-        for n in ("orc_sirt_f32", "orc_sart_f32"):
-            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _F, _I, ctypes.c_float, _F, _DB, _DB]
-        L.orc_reproject_f32.argtypes = [_D, _I, _I, _I, _F, _F, _F, _F]
-        L.orc_reproject_f64.argtypes = [_D, _I, _I, _I, _F, _D, _D, _D]
-        L.orc_backproject_sums_f64.argtypes = [_D, _I, _I, _I, _F, _D, _D, _D]
-        L.orc_bpt_f64.argtypes = [_D, _I, _I, _I, _I, _F, _D, _D, _DB]
-        for n in ("orc_sirt_f64", "orc_sart_f64"):
-            getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, ctypes.c_double, _D, _DB, _DB]
-        L.orc_rweight_f64.argtypes = [_D, _I, _I]
-        L.orc_optimal_dft_size.argtypes = [_I]
-        L.orc_optimal_dft_size.restype = _I
-        L.orc_spatial_taps.argtypes = [_I, _I, _F]
-        L.orc_fbp_filter_f64.argtypes = [_D, _I, _I, _I]
-        L.orc_admm_f32.argtypes = [_D, _I, _I, _I, _I, _F, _F, _I, _I, ctypes.c_float, ctypes.c_float, _F]
-        L.orc_admm_f64.argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, _I, ctypes.c_double, ctypes.c_double, _D]
-        L.orc_bytes_from_stats.argtypes = [_F, ctypes.c_long, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
-        L.orc_float_to_byte.argtypes = [_F, ctypes.c_long, ctypes.c_void_p, _F]
-        L.orc_xf_table.argtypes = [_I, _F]
-        L.orc_xf_inverse.argtypes = [_F, _I, _I, _D]
-        L.orc_image_mean.argtypes = [_F, ctypes.c_long]
-        L.orc_image_mean.restype = ctypes.c_double
-        L.orc_xf_coord.argtypes = [_D, _I, _I, ctypes.POINTER(ctypes.c_int)]
-        L.orc_warp_affine.argtypes = [_F, _I, _I, _D, _I, ctypes.c_float, _F]
-        L.orc_apply_xf.argtypes = [_F, _I, _I, _I, _F, _I, _F, _F]
-        L.orc_gp_nchips.argtypes = [_I]
-        L.orc_gp_nchips.restype = _I
-        L.orc_gp_npairs.argtypes = [_I]
-        L.orc_gp_npairs.restype = ctypes.c_longlong
-        L.orc_gp_view.argtypes = [_F, _I, _I, _I, _I, _I, _F]
-        L.orc_gp_normalize.argtypes = [_F, ctypes.c_longlong, _F]
-        L.orc_gp_profiles.argtypes = [_F, _I, _I, _F]
-        L.orc_gp_ncc.argtypes = [_F, _I, _I, _I]
-        L.orc_gp_ncc.restype = ctypes.c_double
-        _PI = ctypes.POINTER(ctypes.c_int)
-        L.orc_gp_search.argtypes = [_F, _I, _PI, _PI, _PI, _PI]
-        L.orc_gp_search.restype = ctypes.c_float
-        L.orc_gp_view_result.argtypes = [_PI, _PI, _F, _I, _F]
-        L.orc_geopar.argtypes = [_F, _I, _I, _I, _I, _F, _F, _F]
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+def variant(name):
+    """Loads lib/liborc_<name>.so, the same oracle built with other compiler
+    flags (the Makefile's O0 and native targets), for CPU timing only."""
+    return _load(os.path.join(_HERE, "lib", f"liborc_{name}.so"))
+
+
+def _load(path):
+    L = ctypes.CDLL(path)
+    L.orc_coeffs_mpi.argtypes = [_F, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _D]
+    L.orc_coeffs_cuda.argtypes = [_F, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _F]
+    _DB = ctypes.c_double
+    L.orc_bpt_f32.argtypes = [_D, _I, _I, _I, _I, _F, _F, _F, _DB]
+    for n in ("orc_sirt_f32", "orc_sart_f32"):
+        getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _F, _I, ctypes.c_float, _F, _DB, _DB]
+    L.orc_reproject_f32.argtypes = [_D, _I, _I, _I, _F, _F, _F, _F]
+    L.orc_reproject_f64.argtypes = [_D, _I, _I, _I, _F, _D, _D, _D]
+    L.orc_backproject_sums_f64.argtypes = [_D, _I, _I, _I, _F, _D, _D, _D]
+    L.orc_bpt_f64.argtypes = [_D, _I, _I, _I, _I, _F, _D, _D, _DB]
+    for n in ("orc_sirt_f64", "orc_sart_f64"):
+        getattr(L, n).argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, ctypes.c_double, _D, _DB, _DB]
+    L.orc_rweight_f64.argtypes = [_D, _I, _I]
+    L.orc_optimal_dft_size.argtypes = [_I]
+    L.orc_optimal_dft_size.restype = _I
+    L.orc_spatial_taps.argtypes = [_I, _I, _F]
+    L.orc_fbp_filter_f64.argtypes = [_D, _I, _I, _I]
+    L.orc_admm_f32.argtypes = [_D, _I, _I, _I, _I, _F, _F, _I, _I, ctypes.c_float, ctypes.c_float, _F]
+    L.orc_admm_f64.argtypes = [_D, _I, _I, _I, _I, _F, _D, _I, _I, ctypes.c_double, ctypes.c_double, _D]
+    L.orc_bytes_from_stats.argtypes = [_F, ctypes.c_long, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
+    L.orc_float_to_byte.argtypes = [_F, ctypes.c_long, ctypes.c_void_p, _F]
+    L.orc_xf_table.argtypes = [_I, _F]
+    L.orc_xf_inverse.argtypes = [_F, _I, _I, _D]
+    L.orc_image_mean.argtypes = [_F, ctypes.c_long]
+    L.orc_image_mean.restype = ctypes.c_double
+    L.orc_xf_coord.argtypes = [_D, _I, _I, ctypes.POINTER(ctypes.c_int)]
+    L.orc_warp_affine.argtypes = [_F, _I, _I, _D, _I, ctypes.c_float, _F]
+    L.orc_apply_xf.argtypes = [_F, _I, _I, _I, _F, _I, _F, _F]
+    L.orc_gp_nchips.argtypes = [_I]
+    L.orc_gp_nchips.restype = _I
+    L.orc_gp_npairs.argtypes = [_I]
+    L.orc_gp_npairs.restype = ctypes.c_longlong
+    L.orc_gp_view.argtypes = [_F, _I, _I, _I, _I, _I, _F]
+    L.orc_gp_normalize.argtypes = [_F, ctypes.c_longlong, _F]
+    L.orc_gp_profiles.argtypes = [_F, _I, _I, _F]
+    L.orc_gp_ncc.argtypes = [_F, _I, _I, _I]
+    L.orc_gp_ncc.restype = ctypes.c_double
+    _PI = ctypes.POINTER(ctypes.c_int)
+    L.orc_gp_search.argtypes = [_F, _I, _PI, _PI, _PI, _PI]
+    L.orc_gp_search.restype = ctypes.c_float
+    L.orc_gp_view_result.argtypes = [_PI, _PI, _F, _I, _F]
+    L.orc_geopar.argtypes = [_F, _I, _I, _I, _I, _F, _F, _F]
+    return L
 
 
 def _fp(a):
@@ -184,12 +194,12 @@ def bpt_f64(coef, proj, T, org, vol=None, thr=(0.0, 0.0)):
     return out
 
 
-def _iter_f32(name, coef, proj, T, org, iters, gamma, vol, thr):
+def _iter_f32(name, coef, proj, T, org, iters, gamma, vol, thr, L=None):
     proj = np.ascontiguousarray(proj, dtype=np.float32)
     P, X, rows, _ = _dims(proj, T)
     out = np.zeros((rows, T, X), np.float32) if vol is None else np.array(vol, np.float32)
     c = np.ascontiguousarray(coef, dtype=np.float64)
-    getattr(lib(), name)(_dp(c), P, X, rows, T, _org(org), _fp(proj), iters, gamma, _fp(out), float(thr[0]),
+    getattr(L or lib(), name)(_dp(c), P, X, rows, T, _org(org), _fp(proj), iters, gamma, _fp(out), float(thr[0]),
                          float(thr[1]))
     return out
 
@@ -204,9 +214,10 @@ def _iter_f64(name, coef, proj, T, org, iters, gamma, vol, thr):
     return out
 
 
-def sirt_f32(coef, proj, T, org, iters, gamma=1.0, vol=None, thr=(0.0, 0.0)):
-    """thr = (update, residual) thresholds: (0, 0) TiltRec-mpi, CUDA_THRESHOLDS TiltRec-cuda."""
-    return _iter_f32("orc_sirt_f32", coef, proj, T, org, iters, gamma, vol, thr)
+def sirt_f32(coef, proj, T, org, iters, gamma=1.0, vol=None, thr=(0.0, 0.0), L=None):
+    """thr = (update, residual) thresholds: (0, 0) TiltRec-mpi, CUDA_THRESHOLDS
+    TiltRec-cuda.  L: a library from variant() instead of the default build."""
+    return _iter_f32("orc_sirt_f32", coef, proj, T, org, iters, gamma, vol, thr, L)
 
 
 def sart_f32(coef, proj, T, org, iters, gamma=1.0, vol=None, thr=(0.0, 0.0)):
