@@ -424,7 +424,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
     const int iters = is_iterative(m.method) ? m.iterations : 0;
     if (m.method == TREC_SIRT) p.stages = iters;
     else if (m.method == TREC_SART) p.stages = iters * g.nproj;
-    else if (m.method == TREC_ADMM) p.stages = 0;  // issued by issue_admm
+    else if (m.method == TREC_ADMM) p.stages = 1;  // the whole CG recursion is one stage (issue_admm)
     else p.stages = 1;
 
     if (p.path == PATH_ROWS) {
@@ -795,6 +795,9 @@ hipError_t issue_finish(JobPlan &p, hipStream_t st)
     return hipSuccess;
 }
 
+hipError_t issue_admm(JobPlan &p, hipStream_t st);
+hipError_t issue_admm_general(JobPlan &p, hipStream_t st);
+
 // One stage of a general-path job (issued on its own).
 hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
 {
@@ -814,6 +817,8 @@ hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
         LCHK(launch_general_residual(q, th, 1, false, st, nzc));
         return launch_general_back(q, p.tg.data(), BACK_SART, th, 1, q.rval, st);
     }
+    case TREC_ADMM:
+        return issue_admm_general(p, st);
     default:
         return launch_general_back(q, p.tg.data(), BACK_BPT, 0, (int)P, is_filtered(p.m.method) ? p.stack_copy : p.stack, st);
     }
@@ -887,6 +892,9 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
             back[BACK_SART][sto].add(p.d, back_blocks(h), th, 1);
             break;
         }
+        case TREC_ADMM:
+            LCHK(issue_admm(p, st));
+            break;
         default:
             back[BACK_BPT][sto].add(p.d, back_blocks(h), 0, 0);
             break;
@@ -1048,8 +1056,6 @@ hipError_t issue_all(const std::vector<JobPlan *> &plans, hipStream_t st)
         return hipSuccess;
     }
     for (auto *p : plans) LCHK(issue_prep(*p, st));
-    for (auto *p : plans)
-        if (p->m.method == TREC_ADMM) LCHK(p->path == PATH_ROWS ? issue_admm(*p, st) : issue_admm_general(*p, st));
     std::vector<int> stage(plans.size(), 0);
     int S = 0;
     for (auto *p : plans) S = std::max(S, p->stages);

@@ -11,8 +11,8 @@
 //     2. prepare the requests whose stack arrived (plans, descriptors,
 //        layout kernels) and add them to the active set;
 //     3. advance every active request by one stage (a SIRT iteration, a
-//        SART tilt, a BPT/WBP pass): the stages of all active requests are
-//        merged into one launch per kernel kind;
+//        SART tilt, a BPT/WBP pass, a whole ADMM recursion): the stages of
+//        all active requests are merged into one launch per kernel kind;
 //     4. for requests whose last stage ran, queue the output kernels and
 //        hand the volume to the copier.
 //   two copier threads, one for stacks in and one for volumes out, move
@@ -55,7 +55,7 @@ constexpr size_t CHUNK = 32u << 20;
 // copies about 10 GB/s, well below the DMA rate)
 void par_memcpy(char *dst, const char *src, size_t n)
 {
-    constexpr int NT = 4;
+    constexpr int NT = 8;
     if (n < ((size_t)4 << 20)) {
         memcpy(dst, src, n);
         return;
@@ -148,22 +148,37 @@ struct Request {
     trec_geom g{};
     trec_method m{};
     std::vector<float> angles;
-    const float *stack_host = nullptr;
-    float *volume_out = nullptr;
+    const void *stack_host = nullptr;
+    void *volume_out = nullptr;
+    bool stack_half = false, volume_half = false;
     size_t footprint = 0;
     int nzc = 0;  // z chunks the footprint was planned with
-    double submit_s = 0, admit_s = 0, done_s = 0;
+    double submit_s = 0, admit_s = 0, done_s = 0, staged_s = 0, computed_s = 0;
     trec_status status = TREC_OK;
     bool done = false;
     char *block = nullptr;  // arena block: stack | volume | workspace
     hipEvent_t out_ready = nullptr;  // the volume is final on the device
     JobPlan plan;
 
-    size_t stack_bytes() const { return 4 * (size_t)g.nproj * g.ny * g.nx; }
-    size_t volume_bytes() const { return 4 * (size_t)g.nx * g.ny * g.thickness; }
+    size_t stack_elems() const { return (size_t)g.nproj * g.ny * g.nx; }
+    size_t volume_elems() const { return (size_t)g.nx * g.ny * g.thickness; }
+    size_t stack_bytes() const { return 4 * stack_elems(); }    // on the device, float
+    size_t volume_bytes() const { return 4 * volume_elems(); }  // on the device, float
+    // bytes crossing PCIe
+    size_t stack_io_bytes() const { return (stack_half ? 2 : 4) * stack_elems(); }
+    size_t volume_io_bytes() const { return (volume_half ? 2 : 4) * volume_elems(); }
+    // half staging for half input or output (after the workspace)
+    size_t half_bytes() const
+    {
+        return 2 * std::max(stack_half ? stack_elems() : 0, volume_half ? volume_elems() : 0);
+    }
     float *dstack() const { return (float *)block; }
     float *dvolume() const { return (float *)(block + align_up(stack_bytes())); }
     char *workspace() const { return block + align_up(stack_bytes()) + align_up(volume_bytes()); }
+    size_t workspace_room() const { return footprint - align_up(stack_bytes()) - align_up(volume_bytes()) - align_up(half_bytes()); }
+    void *dhalf() const { return block + footprint - align_up(half_bytes()); }
+    void *dstack_in() const { return stack_half ? dhalf() : (void *)dstack(); }
+    const void *dvolume_out() const { return volume_half ? dhalf() : (const void *)dvolume(); }
 };
 
 // One direction of host <-> device traffic.
@@ -230,8 +245,8 @@ struct trec_server {
 trec_status trec_server::copy_in(Copier &c, Request &r)
 {
     const char *src = reinterpret_cast<const char *>(r.stack_host);
-    char *dst = reinterpret_cast<char *>(r.dstack());
-    const size_t n = r.stack_bytes();
+    char *dst = reinterpret_cast<char *>(r.dstack_in());
+    const size_t n = r.stack_io_bytes();
     int b = 0;
     for (size_t off = 0; off < n; off += CHUNK, b ^= 1) {
         const size_t len = std::min(CHUNK, n - off);
@@ -248,9 +263,9 @@ trec_status trec_server::copy_in(Copier &c, Request &r)
 // overlapping chunk k's memcpy.
 trec_status trec_server::copy_out(Copier &c, Request &r)
 {
-    const char *src = reinterpret_cast<const char *>(r.dvolume());
+    const char *src = reinterpret_cast<const char *>(r.dvolume_out());
     char *dst = reinterpret_cast<char *>(r.volume_out);
-    const size_t n = r.volume_bytes();
+    const size_t n = r.volume_io_bytes();
     if (hipStreamWaitEvent(c.stream, r.out_ready, 0) != hipSuccess) return TREC_ERR_DEVICE;
     const size_t nchunk = (n + CHUNK - 1) / CHUNK;
     auto issue = [&](size_t k) {
@@ -262,6 +277,7 @@ trec_status trec_server::copy_out(Copier &c, Request &r)
     for (size_t k = 0; k < nchunk; ++k) {
         if (k + 1 < nchunk && !issue(k + 1)) return TREC_ERR_DEVICE;
         if (hipEventSynchronize(c.ev[k & 1]) != hipSuccess) return TREC_ERR_DEVICE;
+        if (k == 0) r.computed_s = now_s();
         const size_t off = k * CHUNK, len = std::min(CHUNK, n - off);
         par_memcpy(dst + off, c.pinned[k & 1], len);
     }
@@ -297,6 +313,7 @@ void trec_server::run_copier(Copier &c, bool in)
         Request &r = *job;
         if (in) {
             const trec_status st = copy_in(c, r);
+            r.staged_s = now_s();
             std::lock_guard<std::mutex> lk(mu);
             if (st == TREC_OK) {
                 ready.push_back(&r);
@@ -318,12 +335,15 @@ trec_status trec_server::prepare(Request &r)
     trec_status s = plan_job(ctx, r.plan, r.g, r.m, r.angles.data(), r.dstack(), nullptr, r.dvolume(), r.nzc);
     if (s != TREC_OK) return s;
     // the block was sized at submit from the same angles and z chunks; the
-    // workspace must not run into the next block
-    const size_t room = r.footprint - align_up(r.stack_bytes()) - align_up(r.volume_bytes());
-    if (carve(r.plan, nullptr) > room)
+    // workspace must not run into the half staging or the next block
+    if (carve(r.plan, nullptr) > r.workspace_room())
         return fail(ctx, TREC_ERR_INVALID, "request %llu: workspace exceeds its admitted footprint",
                     (unsigned long long)r.id);
     carve(r.plan, r.workspace());
+    // a half stack is widened to float on the device (copy_in has finished)
+    if (r.stack_half &&
+        launch_convert(r.dhalf(), TREC_DT_F16, r.dstack(), TREC_DT_F32, r.stack_elems(), stream) != hipSuccess)
+        return TREC_ERR_DEVICE;
     s = upload(ctx, {&r.plan}, stream);
     if (s != TREC_OK) return s;
     if (issue_prep(r.plan, stream) != hipSuccess) return TREC_ERR_DEVICE;
@@ -335,6 +355,14 @@ void trec_server::run()
     (void)hipSetDevice(device);
     std::vector<Request *> active;
     std::vector<int> stage;
+    // Two steps in flight: after issuing step k the loop waits for step k - 1
+    // only, so admission, preparation and the issue of step k + 1 happen on
+    // the host while step k runs on the device.
+    hipEvent_t step_ev[2] = {nullptr, nullptr};
+    for (auto &ev : step_ev)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+    int cur = 0;
+    bool prev_pending = false;
     for (;;) {
         std::vector<Request *> arrived;
         {
@@ -344,7 +372,11 @@ void trec_server::run()
             };
             if (active.empty())
                 cv_work.wait(lk, [&] { return (stop && queue.empty() && in_flight == 0) || !ready.empty() || admissible(); });
-            if (stop && queue.empty() && in_flight == 0 && active.empty()) return;
+            if (stop && queue.empty() && in_flight == 0 && active.empty()) {
+                for (auto ev : step_ev)
+                    if (ev) (void)hipEventDestroy(ev);
+                return;
+            }
             // first come, first served: stop at the first request that does not fit yet
             while (!queue.empty() && in_flight < max_active) {
                 Request *r = queue.front();
@@ -386,15 +418,26 @@ void trec_server::run()
             if (stage[k] < active[k]->plan.stages) continue;
             Request &r = *active[k];
             e = issue_finish(r.plan, stream);
+            if (e == hipSuccess && r.volume_half)
+                e = launch_convert(r.dvolume(), TREC_DT_F32, r.dhalf(), TREC_DT_F16, r.volume_elems(), stream);
             if (e == hipSuccess) e = hipEventRecord(r.out_ready, stream);
             finished.push_back(k);
         }
-        // one step in flight at a time: the host prepares the next step's
-        // launches while the device runs this one
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e == hipSuccess) {
+            if (!step_ev[0] || !step_ev[1]) {
+                e = hipStreamSynchronize(stream);  // no events: one step at a time
+            } else {
+                e = hipEventRecord(step_ev[cur], stream);
+                if (e == hipSuccess && prev_pending) e = hipEventSynchronize(step_ev[cur ^ 1]);
+                prev_pending = true;
+                cur ^= 1;
+            }
+        }
         if (e != hipSuccess) {
             // a device failure ends every active request with an error
             (void)hipGetLastError();
+            (void)hipStreamSynchronize(stream);
+            prev_pending = false;
             std::lock_guard<std::mutex> lk(mu);
             for (Request *r : active) finish(*r, TREC_ERR_DEVICE);
             active.clear();
@@ -464,17 +507,19 @@ trec_status trec_server_submit(trec_server *srv, const trec_request *req, uint64
     if (!req->stack || !req->volume_out) return TREC_ERR_INVALID;
     trec_status s = validate(nullptr, &req->geom, &req->method, req->angles_deg);
     if (s != TREC_OK) return s;
-    // the serving loop advances requests stage by stage; ADMM's nested CG
-    // runs as one sequence (trec_reconstruct)
-    if (req->method.method == TREC_ADMM) return TREC_ERR_UNSUPPORTED;
     auto r = std::make_unique<Request>();
     r->g = req->geom;
     r->m = req->method;
     r->angles.assign(req->angles_deg, req->angles_deg + req->geom.nproj);
     r->stack_host = req->stack;
     r->volume_out = req->volume_out;
+    if ((req->stack_half != 0 && req->stack_half != 1) || (req->volume_half != 0 && req->volume_half != 1))
+        return TREC_ERR_INVALID;
+    r->stack_half = req->stack_half == 1;
+    r->volume_half = req->volume_half == 1;
     r->footprint = request_bytes(req->geom, req->method, req->angles_deg, &r->nzc);
     if (r->footprint == 0) return TREC_ERR_INVALID;
+    r->footprint += align_up(r->half_bytes());
     if (align_up(r->footprint) > srv->arena.size()) return TREC_ERR_BUSY;  // can never fit
     r->submit_s = now_s();
     std::lock_guard<std::mutex> lk(srv->mu);
@@ -509,6 +554,8 @@ trec_status trec_server_wait(trec_server *srv, uint64_t id, trec_result *res)
         res->submit_s = r->submit_s;
         res->admit_s = r->admit_s;
         res->done_s = r->done_s;
+        res->staged_s = r->staged_s;
+        res->computed_s = r->computed_s;
         res->footprint = r->footprint;
     }
     const trec_status st = r->status;

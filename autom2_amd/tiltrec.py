@@ -60,12 +60,14 @@ class Job(ctypes.Structure):
 
 class Request(ctypes.Structure):
     _fields_ = [("geom", Geom), ("method", Method), ("angles_deg", ctypes.POINTER(ctypes.c_float)),
-                ("stack", ctypes.c_void_p), ("volume_out", ctypes.c_void_p)]
+                ("stack", ctypes.c_void_p), ("volume_out", ctypes.c_void_p), ("stack_half", ctypes.c_int),
+                ("volume_half", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
-                ("done_s", ctypes.c_double), ("footprint", ctypes.c_size_t)]
+                ("done_s", ctypes.c_double), ("footprint", ctypes.c_size_t), ("staged_s", ctypes.c_double),
+                ("computed_s", ctypes.c_double)]
 
 
 # (name, restype, argtypes) of every function include/tiltrec.h declares
@@ -397,10 +399,11 @@ class Server:
 
     def submit(self, geom, method, angles, stack, out):
         import numpy as np
-        assert stack.dtype == np.float32 and stack.flags["C_CONTIGUOUS"]
-        assert out.dtype == np.float32 and out.flags["C_CONTIGUOUS"]
+        assert stack.dtype in (np.float32, np.float16) and stack.flags["C_CONTIGUOUS"]
+        assert out.dtype in (np.float32, np.float16) and out.flags["C_CONTIGUOUS"]
         pa, keep = _angles(angles)
-        req = Request(geom, method, pa, stack.ctypes.data, out.ctypes.data)
+        req = Request(geom, method, pa, stack.ctypes.data, out.ctypes.data, int(stack.dtype == np.float16),
+                      int(out.dtype == np.float16))
         rid = ctypes.c_uint64()
         _check(lib().trec_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)))
         self._keep[rid.value] = (stack, out, keep)
@@ -417,7 +420,7 @@ class Server:
         self._keep.pop(rid, None)
         _check(st)
         return {"status": r.status, "submit_s": r.submit_s, "admit_s": r.admit_s, "done_s": r.done_s,
-                "footprint": r.footprint}
+                "footprint": r.footprint, "staged_s": r.staged_s, "computed_s": r.computed_s}
 
     def close(self):
         if self._h:

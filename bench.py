@@ -213,20 +213,43 @@ def serving_mix(n, seed):
     return reqs
 
 
-def serve_server(trec, reqs, arrivals, storage):
-    """Continuous-batching server: submit at the arrival times, wait all."""
+def out_pool(reqs, dtype, k=8):
+    """Host output buffers for the served volumes: k arrays of the largest
+    volume, request i writing into buffer i % k.  The bench checks no
+    values, and holding every request's volume at once would take more host
+    memory than a box gives one run."""
     import numpy as np
-    srv = trec.Server(0, budget=96 << 30, max_active=16)
+    big = max(r["X"] * r["T"] * r["X"] for r in reqs)
+    return [np.empty(big, dtype) for _ in range(min(k, len(reqs)))]
+
+
+def serve(trec, reqs, arrivals, storage, max_active, half_io=False):
+    """Submits the requests to a server at their arrival times and waits for
+    all: latencies (submission to volume in host memory) and the span from
+    the first submission to the last completion.  half_io: the stacks are
+    IEEE half (as MRC mode 12 files hold them) and the volumes come back as
+    half."""
+    import numpy as np
+    pool = out_pool(reqs, np.float16 if half_io else np.float32)
+    stacks = {}
+    if half_io:
+        for r in reqs:
+            if id(r["stack"]) not in stacks:
+                stacks[id(r["stack"])] = r["stack"].astype(np.float16)
+    # WBP keeps a float stack: its ramp filter would raise the stack's half
+    # rounding above 1e-3 of the volume
+    half_in = lambda r: half_io and not r["method"].startswith("WBP")
+    srv = trec.Server(0, budget=96 << 30, max_active=max_active)
     t0 = time.perf_counter()
-    ids, outs = [], []
-    for r, ta in zip(reqs, arrivals):
+    ids = []
+    for k, (r, ta) in enumerate(zip(reqs, arrivals)):
         dt = ta - (time.perf_counter() - t0)
         if dt > 0:
             time.sleep(dt)
-        out = np.empty((r["X"], r["T"], r["X"]), np.float32)
-        outs.append(out)
+        out = pool[k % len(pool)][:r["X"] * r["T"] * r["X"]]
+        st = stacks[id(r["stack"])] if half_in(r) else r["stack"]
         ids.append(srv.submit(trec.make_geom(r["X"], r["X"], r["P"], r["T"]),
-                              trec.make_method(r["method"], storage=storage), r["ang"], r["stack"], out))
+                              trec.make_method(r["method"], storage=storage), r["ang"], st, out))
     recs = [srv.wait(i) for i in ids]
     srv.close()
     lat = [(rec["done_s"] - rec["submit_s"]) * 1e3 for rec in recs]
@@ -234,50 +257,42 @@ def serve_server(trec, reqs, arrivals, storage):
     return lat, span
 
 
-def serve_sequential(trec, torch, reqs, arrivals):
-    """Plain FP32, one request at a time in arrival order (copy in,
-    reconstruct, copy out)."""
-    ctx = trec.Context(0)
-    t0 = time.perf_counter()
-    lat = []
-    for r, ta in zip(reqs, arrivals):
-        now = time.perf_counter() - t0
-        if ta > now:
-            time.sleep(ta - now)
-        s = torch.from_numpy(r["stack"]).cuda()
-        vol = torch.empty((r["X"], r["T"], r["X"]), dtype=torch.float32, device="cuda")
-        ctx.reconstruct(trec.make_geom(r["X"], r["X"], r["P"], r["T"]), r["ang"], s,
-                        trec.make_method(r["method"]), vol)
-        vol.cpu()
-        torch.cuda.synchronize()
-        lat.append((time.perf_counter() - t0 - ta) * 1e3)
-        del s, vol
-    span = time.perf_counter() - t0 - arrivals[0]
-    ctx.close()
-    torch.cuda.empty_cache()
-    return lat, span
-
-
 def serving_form(trec, torch, n, rate, seed=1004):
+    """The serving configuration: n requests of the batching mix, Poisson
+    arrivals at `rate` per second and all at once (saturated).  Every arm
+    runs through the same server and copy machinery (pinned double-buffered
+    staging, threaded host copies):
+      plain_f32       one request at a time (max_active 1), FP32 storage;
+      batched_f16     continuous batching (up to 16 active), FP16 storage;
+      batched_f16_half_io  the same with half volumes out (MRC mode 12) and,
+                      for SIRT, half stacks in: half the bytes over PCIe;
+      batched_f32     continuous batching, FP32 storage (saturated only).
+    Latency is from submission to the volume in host memory."""
     import numpy as np
     reqs = serving_mix(n, seed)
     gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, n)
     arrivals = list(np.cumsum(gaps) - gaps[0])
-    out = {"requests": n, "arrival_rate_per_s": rate, "storage": "f16"}
+    out = {"requests": n, "arrival_rate_per_s": rate,
+           "copies": "every arm through the server's pinned staging; latency includes host copies"}
+    stat = lambda lat, span: {"series_per_s": round(n / span, 3), "p50_ms": round(percentile(lat, 0.5), 1),
+                              "p99_ms": round(percentile(lat, 0.99), 1)}
+    arms = [("plain_f32", trec.STORE_F32, 1, False), ("batched_f16", trec.STORE_F16, 16, False),
+            ("batched_f16_half_io", trec.STORE_F16, 16, True)]
     for mode in ("poisson", "saturated"):
         arr = arrivals if mode == "poisson" else [0.0] * n
-        log(f"[bench] serving ({mode}): continuous batching, FP16")
-        lat_s, span_s = serve_server(trec, reqs, arr, trec.STORE_F16)
-        log(f"[bench] serving ({mode}): plain FP32 one at a time")
-        lat_q, span_q = serve_sequential(trec, torch, reqs, arr)
-        out[mode] = {
-            "batched_f16": {"series_per_s": round(n / span_s, 3), "p50_ms": round(percentile(lat_s, 0.5), 1),
-                            "p99_ms": round(percentile(lat_s, 0.99), 1)},
-            "plain_f32": {"series_per_s": round(n / span_q, 3), "p50_ms": round(percentile(lat_q, 0.5), 1),
-                          "p99_ms": round(percentile(lat_q, 0.99), 1)},
-            "throughput_ratio": round(span_q / span_s, 3),
-        }
-        log(f"[bench] serving ({mode}): {out[mode]}")
+        res = {}
+        for name, sto, act, half in arms + ([("batched_f32", trec.STORE_F32, 16, False)] if mode == "saturated"
+                                            else []):
+            log(f"[bench] serving ({mode}): {name}")
+            lat, span = serve(trec, reqs, arr, sto, act, half)
+            res[name] = stat(lat, span)
+            res[name]["span_s"] = round(span, 3)
+        base = res["plain_f32"]["span_s"]
+        for name in res:
+            if name != "plain_f32":
+                res[name]["throughput_vs_plain_f32"] = round(base / res[name]["span_s"], 3)
+        out[mode] = res
+        log(f"[bench] serving ({mode}): {res}")
     return out
 
 
@@ -575,7 +590,9 @@ def main():
     ap.add_argument("--no-north", action="store_true", help="skip the 61 x 2048^2 -> 2048^2 x 512 form")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end CLI run with MRC files")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16/BF16 and serving forms")
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving form (its own JSON line), e.g. with --serve-requests 120")
     ap.add_argument("--counters", action="store_true", help="only the run the counter profiles measure")
     ap.add_argument("--no-xf", action="store_true", help="skip the .xf application stage")
     ap.add_argument("--no-geopar", action="store_true", help="skip the GeoParGen stage")
@@ -612,6 +629,11 @@ def main():
         return
     if args.xf_only:
         print(json.dumps({"xf_apply": xf_stage(trec, ctx, torch, stack_d, _xf, cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.serving_only:
+        del stack_d
+        torch.cuda.empty_cache()
+        print(json.dumps({"serving": serving_form(trec, torch, args.serve_requests, 2.0)}), flush=True)
         return
     if args.methods_only:
         print(json.dumps({"methods": methods_form(trec, ctx, torch, geom, ang, stack_d)}), flush=True)

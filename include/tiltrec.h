@@ -367,9 +367,17 @@ typedef struct {
     trec_geom geom;
     trec_method method;
     const float *angles_deg; /* host, copied at submit */
-    const float *stack;      /* host [nproj][ny][nx]; must stay valid until
-                                the request is done */
-    float *volume_out;       /* host, written when the request completes */
+    const void *stack;       /* host [nproj][ny][nx] float (IEEE half with
+                                stack_half); must stay valid until the
+                                request is done */
+    void *volume_out;        /* host, float (half with volume_half), written
+                                when the request completes */
+    int stack_half;          /* 1: the stack is IEEE half (MRC mode 12); it
+                                crosses PCIe at half the bytes and is widened
+                                to float on the device */
+    int volume_half;         /* 1: the volume comes back as IEEE half, rounded
+                                to nearest even on the device (MRC mode 12
+                                output), at half the bytes */
 } trec_request;
 
 typedef struct {
@@ -378,6 +386,9 @@ typedef struct {
     double admit_s;    /* when admitted to the device */
     double done_s;     /* when its volume was back on the host */
     size_t footprint;  /* device bytes it held */
+    double staged_s;   /* when its stack was on the device */
+    double computed_s; /* when its volume was final on the device (the
+                          first chunk of the copy out had arrived) */
 } trec_result;
 
 /* hbm_budget = 0 uses 90 % of the device's free memory; max_active is at
@@ -393,7 +404,9 @@ trec_status trec_server_poll(trec_server *srv, uint64_t id, int *done);
 /* Blocks until the request finished and returns its record and status;
  * the id is released afterwards. */
 trec_status trec_server_wait(trec_server *srv, uint64_t id, trec_result *res);
-/* Device bytes a request would hold (the admission footprint). */
+/* Device bytes a request with float input and output would hold (the
+ * admission footprint, an upper bound over the row and the general path);
+ * half input or output adds 2 bytes per stack or volume element. */
 trec_status trec_request_footprint(const trec_geom *geom,
                                    const trec_method *method, size_t *bytes);
 void trec_server_destroy(trec_server *srv);

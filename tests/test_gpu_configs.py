@@ -224,8 +224,12 @@ def test_config_slab_volume_4096(torch_cuda, trec, orc):
 def test_config_serving_fp16(torch_cuda, trec, orc):
     """The serving configuration: requests of the batching mix's sizes with
     WBP, SIRT x10 or SIRT x20, FP16 storage, arriving as a Poisson stream
-    (2 per second) at the continuous-batching server; one row of every
-    served volume against the FP64 oracle at 1e-3."""
+    (2 per second) at the continuous-batching server; every other request
+    returns a half volume (MRC mode 12) and, for SIRT, also takes a half
+    stack (WBP keeps a float stack: its ramp filter raises the stack's half
+    rounding to about 3e-3 of the volume).  One row of every served volume
+    against the FP64 oracle on the float stack, at 1e-3: half input and
+    output are part of the form's error."""
     import time
     torch = torch_cuda
     mix = batch_mix(16, 1004, ["WBP", "SIRT,10,1.0", "SIRT,20,1.0"])
@@ -237,9 +241,12 @@ def test_config_serving_fp16(torch_cuda, trec, orc):
     t0 = time.perf_counter()
     for k, ((X, P, T, m, ang), st) in enumerate(zip(mix, stacks)):
         time.sleep(float(gaps[k]))
-        out = np.empty((X, T, X), np.float32)
+        half = k % 2 == 1
+        out = np.empty((X, T, X), np.float16 if half else np.float32)
         outs.append(out)
-        ids.append(srv.submit(trec.make_geom(X, X, P, T), trec.make_method(m, storage="f16"), ang, st, out))
+        half_in = half and not m.startswith("WBP")
+        ids.append(srv.submit(trec.make_geom(X, X, P, T), trec.make_method(m, storage="f16"), ang,
+                              st.astype(np.float16) if half_in else st, out))
     recs = [srv.wait(i) for i in ids]
     srv.close()
     print(f"served {len(mix)} requests in {time.perf_counter() - t0:.1f} s")
@@ -253,6 +260,7 @@ def test_config_serving_fp16(torch_cuda, trec, orc):
         refs = list(ex.map(ref, range(len(mix))))
     for k, ((X, P, T, m, _), out, rec, r) in enumerate(zip(mix, outs, recs, refs)):
         assert rec["status"] == 0
-        e = orc.rel_l2(out[rows[k]], r)
-        print(f"request {k}: {P} x {X}^2 -> thickness {T}, {m}, FP16: rel L2 vs FP64 oracle {e:.3e}")
+        e = orc.rel_l2(out[rows[k]].astype(np.float32), r)
+        io = ("half out" if m.startswith("WBP") else "half in and out") if k % 2 else "float in and out"
+        print(f"request {k}: {P} x {X}^2 -> thickness {T}, {m}, FP16, {io}: rel L2 vs FP64 oracle {e:.3e}")
         assert e < 1e-3, k

@@ -230,3 +230,33 @@ def test_server_steep_tilts_and_thin_volume(torch_cuda, trec, orc):
         fn = orc.sart_f32 if meth.startswith("SART") else orc.sirt_f32
         ref = fn(orc.coeffs_mpi(ang), stack, T, orc.origin(X, T, T / 2), it, gam)
         assert orc.rel_l2(out, ref) < 1e-5, meth
+
+
+def test_server_admm_equals_alone(torch_cuda, trec, orc):
+    """ADMM requests run through the server as one stage each (the whole CG
+    recursion), beside SIRT requests; each volume equals trec_reconstruct's
+    alone, bit for bit, on the row path and with a pitch (general path)."""
+    ctx = trec.Context(0)
+    cases = [(9, 20, 12, 40, "ADMM,2,3,0.5,0.001", 0.0), (7, 18, 10, 36, "SIRT,3,1.0", 0.0),
+             (9, 16, 10, 32, "ADMM,2,2,0.8,0.0", 4.0)]
+    refs, data = [], []
+    for k, (P, Y, T, X, meth, pitch) in enumerate(cases):
+        ang, stack = make_series(orc, P, Y, T, X, 130 + k)
+        g = trec.make_geom(X, Y, P, T, 0.0, pitch, 0.0)
+        out = torch_cuda.empty((Y, T, X), device="cuda")
+        ctx.reconstruct(g, ang, torch_cuda.from_numpy(stack).cuda(), trec.make_method(meth), out)
+        torch_cuda.cuda.synchronize()
+        refs.append(out.cpu().numpy())
+        data.append((g, ang, stack))
+    ctx.close()
+    srv = trec.Server(0, budget=512 << 20, max_active=4)
+    ids, outs = [], []
+    for (P, Y, T, X, meth, pitch), (g, ang, stack) in zip(cases, data):
+        out = np.zeros((Y, T, X), np.float32)
+        outs.append(out)
+        ids.append(srv.submit(g, trec.make_method(meth), ang, stack, out))
+    recs = [srv.wait(i) for i in ids]
+    srv.close()
+    for c, out, ref, rec in zip(cases, outs, refs, recs):
+        assert rec["status"] == 0
+        np.testing.assert_array_equal(out, ref, err_msg=c[4])
