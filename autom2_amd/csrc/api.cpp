@@ -21,6 +21,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "trec_internal.h"
@@ -35,8 +36,15 @@ constexpr size_t ALIGN = 256;
 
 size_t align_up(size_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
 
+// What trec_autotune chooses for a geometry and storage: the SIRT forward's
+// z chunks and the SIRT back-projection's shape.
+struct Tuned {
+    int nzc = 0;            // 0: not tuned
+    int bp_g = 0, bp_zt = 0;  // 0: the default
+};
+
 std::mutex g_tune_mu;
-std::map<std::string, int> g_tuned_nzc;  // process-wide autotune results
+std::map<std::string, Tuned> g_tuned;  // process-wide autotune results
 
 std::string tune_key(const trec_geom &g, int storage)
 {
@@ -136,7 +144,7 @@ struct trec_context {
     size_t ws_limit = 0;        // workspace budget of one call (0: the free device memory)
     hipGraphExec_t gexec = nullptr;
     std::string gkey;
-    std::map<std::string, int> tuned;  // nzc per geometry
+    std::map<std::string, Tuned> tuned;  // per geometry and storage
     trec::Prof prof;
 };
 
@@ -319,22 +327,28 @@ trec_status validate(trec_context *ctx, const trec_geom *g, const trec_method *m
     return TREC_OK;
 }
 
+// The tuning record for a geometry (the context's, else the process's), or
+// an empty one.
+Tuned find_tuned(trec_context *ctx, const trec_geom &g, int storage)
+{
+    const std::string key = tune_key(g, storage);
+    if (ctx) {
+        auto it = ctx->tuned.find(key);
+        if (it != ctx->tuned.end()) return it->second;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    return it != g_tuned.end() ? it->second : Tuned{};
+}
+
 int pick_nzc(trec_context *ctx, const trec_geom &g, int storage)
 {
     if (const char *e = getenv("TREC_NZC")) {
         int v = atoi(e);
         if (v >= 1) return std::min(v, g.thickness);
     }
-    const std::string key = tune_key(g, storage);
-    if (ctx) {
-        auto it = ctx->tuned.find(key);
-        if (it != ctx->tuned.end()) return std::min(it->second, g.thickness);
-    }
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tuned_nzc.find(key);
-        if (it != g_tuned_nzc.end()) return std::min(it->second, g.thickness);
-    }
+    const Tuned t = find_tuned(ctx, g, storage);
+    if (t.nzc > 0) return std::min(t.nzc, g.thickness);
     // One z chunk (the residual written by the forward projection itself, no
     // combine pass) unless the grid would then have fewer than about eight
     // work-groups per CU; more chunks split the layers between work-groups.
@@ -439,7 +453,22 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
         p.slab_rb = h.nyb;
         p.nslab = 1;
         h.nzc = 1;
-        if (m.method == TREC_SIRT) h.nzc = force_nzc > 0 ? std::min(force_nzc, g.thickness) : pick_nzc(ctx, g, p.m.storage);
+        if (m.method == TREC_SIRT) {
+            h.nzc = force_nzc > 0 ? std::min(force_nzc, g.thickness) : pick_nzc(ctx, g, p.m.storage);
+            const Tuned t = find_tuned(ctx, g, p.m.storage);
+            h.bp_g = t.bp_g;
+            h.bp_zt = t.bp_zt;
+            // TREC_BACK_SHAPE=g,zt forces one of back_shapes (tests and tuning runs)
+            if (const char *e = getenv("TREC_BACK_SHAPE")) {
+                int bg = 0, bz = 0;
+                if (sscanf(e, "%d,%d", &bg, &bz) == 2)
+                    for (const auto &sh : back_shapes(p.m.storage))
+                        if (sh.first == bg && sh.second == bz) {
+                            h.bp_g = bg;
+                            h.bp_zt = bz;
+                        }
+            }
+        }
         h.zc_len = (g.thickness + h.nzc - 1) / h.nzc;
         h.nzc = (g.thickness + h.zc_len - 1) / h.zc_len;
         h.storage = p.m.storage;
@@ -865,6 +894,8 @@ struct Batcher {
 hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &stage, hipStream_t st)
 {
     Batcher fp_fused[3], fp_part[3], comb[3], back[3][3], fp_small[3];
+    // SIRT back-projections by shape: (storage, tilts per pass, voxels per thread)
+    std::map<std::tuple<int, int, int>, Batcher> back_sirt;
     for (size_t j = 0; j < plans.size(); ++j) {
         JobPlan &p = *plans[j];
         const int s = stage[j];
@@ -884,7 +915,7 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
                 fp_part[sto].add(p.d, fwd_blocks(h, h.P), 0, 0);
                 comb[sto].add(p.d, combine_blocks(h, h.P), 0, 0);
             }
-            back[BACK_SIRT][sto].add(p.d, back_blocks(h), 0, 0);
+            back_sirt[std::make_tuple(sto, h.bp_g, h.bp_zt)].add(p.d, back_blocks(h), 0, 0);
             break;
         case TREC_SART: {
             const int th = s % h.P;
@@ -919,6 +950,11 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
     }
     {
         ProfScope ps(PROF_BACK, st);
+        for (auto &kb : back_sirt) {
+            const int sto = std::get<0>(kb.first), g = std::get<1>(kb.first), zt = std::get<2>(kb.first);
+            LCHK(kb.second.flush(
+                [&](const LaunchList &L, unsigned n) { return launch_back(L, n, sto, BACK_SIRT, st, g, zt); }));
+        }
         for (int mode = 0; mode < 3; ++mode)
             for (int sto = 0; sto < 3; ++sto)
                 LCHK(back[mode][sto].flush(
@@ -1092,6 +1128,8 @@ std::string graph_key(trec_context *ctx, const std::vector<JobPlan *> &plans)
                  p.path == PATH_ROWS ? p.h.nzc : 0, (int)p.want_rwt, p.m.cg_iterations, (double)p.m.soft,
                  (double)p.g.offset, (double)p.g.pitch, (double)p.g.zshift, (int)p.fg_small,
                  (unsigned long long)hc);
+        k += b;
+        snprintf(b, sizeof b, "%d,%d|", p.path == PATH_ROWS ? p.h.bp_g : 0, p.path == PATH_ROWS ? p.h.bp_zt : 0);
         k += b;
     }
     return k;
@@ -1821,7 +1859,8 @@ trec_status trec_plan(trec_context *ctx, const trec_geom *geom, const float *ang
         info->zchunks = p.h.nzc;
         if (uses_forward(p.m.method))
             info->band_cells = (p.m.method == TREC_SIRT && p.h.nzc == 1 && p.fg_small) ? FGS_SMALL : FGS;
-        info->back_tilts = p.m.storage == TREC_STORE_F32 ? BP_G32 : BP_G;
+        info->back_tilts = p.h.bp_g ? p.h.bp_g : back_shapes(p.m.storage)[0].first;
+        info->back_voxels_z = p.h.bp_zt ? p.h.bp_zt : back_shapes(p.m.storage)[0].second;
     }
     return TREC_OK;
 }
@@ -1886,7 +1925,7 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
     for (int i = 0; i < geom->nproj; ++i) ang[i] = geom->nproj > 1 ? -60.f + 120.f * i / (geom->nproj - 1) : 0.f;
     trec_status s = validate(ctx, geom, method, ang.data());
     if (s != TREC_OK) return s;
-    if (method->method != TREC_SIRT) return TREC_OK;  // only SIRT has a tunable z split
+    if (method->method != TREC_SIRT) return TREC_OK;  // only SIRT has tunable shapes
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t X = geom->nx, Y = geom->ny, T = geom->thickness, P = geom->nproj;
     float *stack = nullptr, *vol = nullptr;
@@ -1905,48 +1944,72 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    int best = 1;
-    float best_ms = 1e30f;
-    constexpr int REPS = 4;
-    const int cands[] = {1, 2, 4, 8, 16, 32};
+    constexpr int REPS = 6;
     trec_status res = TREC_OK;
-    for (int nzc : cands) {
-        if (nzc > (int)T) break;
+    bool rows_path = true;
+    // the best of REPS whole SIRT iterations (after one warm-up) with z
+    // chunks nzc and back-projection shape (g, zt); 0 for the shape is the
+    // default
+    auto measure = [&](int nzc, int g, int zt, float &ms) -> bool {
         JobPlan p;
         trec_method m = *method;
-        // one warm-up step and REPS timed steps, each a whole SIRT iteration
         m.iterations = 1 + REPS;
-        if ((res = plan_job(ctx, p, *geom, m, ang.data(), stack, nullptr, vol)) != TREC_OK) break;
-        if (p.path != PATH_ROWS) break;
-        p.h.zc_len = (int)((T + nzc - 1) / nzc);
-        p.h.nzc = (int)((T + p.h.zc_len - 1) / p.h.zc_len);
-        if ((res = ensure_arena(ctx, carve(p, nullptr), st)) != TREC_OK) break;
+        if ((res = plan_job(ctx, p, *geom, m, ang.data(), stack, nullptr, vol, nzc)) != TREC_OK) return false;
+        if (p.path != PATH_ROWS) {
+            rows_path = false;
+            return false;
+        }
+        p.h.bp_g = g;
+        p.h.bp_zt = zt;
+        if ((res = ensure_arena(ctx, carve(p, nullptr), st)) != TREC_OK) return false;
         carve(p, ctx->arena);
-        if ((res = upload(ctx, {&p}, st)) != TREC_OK) break;
+        if ((res = upload(ctx, {&p}, st)) != TREC_OK) return false;
         std::vector<JobPlan *> pl{&p};
         std::vector<int> stage{0};
         if (issue_prep(p, st) != hipSuccess || issue_step(pl, stage, st) != hipSuccess) {
             res = fail(ctx, TREC_ERR_DEVICE, "autotune launch failed");
-            break;
+            return false;
         }
-        float ms = 1e30f;
-        for (int rep = 0; rep < REPS && res == TREC_OK; ++rep) {
+        ms = 1e30f;
+        for (int rep = 0; rep < REPS; ++rep) {
             float t = 0.f;
             if (hipEventRecord(e0, st) != hipSuccess || issue_step(pl, stage, st) != hipSuccess ||
                 hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                 res = fail(ctx, TREC_ERR_DEVICE, "autotune run failed");
-                break;
+                return false;
             }
             (void)hipEventElapsedTime(&t, e0, e1);
             ms = std::min(ms, t);
         }
-        if (res != TREC_OK) break;
-        // more chunks must win clearly (the best of 4 steps by 3 %): one chunk
-        // needs no combine pass, fewer chunks leave more room to batch jobs,
-        // and a step differs by a percent or two from one run to the next
-        if (ms < best_ms * 0.97f) {
+        return true;
+    };
+    // Candidates are tried in order of preference (the first is the
+    // default).  A later one is kept only when it beats the best so far by
+    // more than `margin`: a step varies by a percent or two between runs, so
+    // this keeps the choice stable when candidates are close.
+    Tuned best;
+    float best_ms = 1e30f;
+    const float margin = 0.02f;
+    const int nzcs[] = {1, 2, 4, 8, 16, 32};
+    for (int nzc : nzcs) {
+        if (nzc > (int)T) break;
+        float ms;
+        if (!measure(nzc, 0, 0, ms)) break;
+        if (ms < best_ms * (1.f - margin)) {
             best_ms = ms;
-            best = p.h.nzc;
+            best.nzc = nzc;
+        }
+    }
+    if (res == TREC_OK && rows_path && best.nzc > 0) {
+        const auto &shapes = back_shapes(method->storage);
+        for (size_t k = 1; k < shapes.size(); ++k) {
+            float ms;
+            if (!measure(best.nzc, shapes[k].first, shapes[k].second, ms)) break;
+            if (ms < best_ms * (1.f - margin)) {
+                best_ms = ms;
+                best.bp_g = shapes[k].first;
+                best.bp_zt = shapes[k].second;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -1955,11 +2018,12 @@ trec_status trec_autotune(trec_context *ctx, const trec_geom *geom, const trec_m
     (void)hipFree(stack);
     (void)hipFree(vol);
     if (res != TREC_OK) return res;
+    if (best.nzc == 0) return TREC_OK;  // the general path: nothing to tune
     const std::string key = tune_key(*geom, method->storage);
     ctx->tuned[key] = best;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tuned_nzc[key] = best;
+        g_tuned[key] = best;
     }
     return TREC_OK;
 }

@@ -28,6 +28,8 @@
 #include <algorithm>
 #include <cstdlib>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "trec_internal.h"
 
@@ -775,32 +777,36 @@ __global__ __launch_bounds__(WG) void k_combine(const LaunchList L, unsigned nbl
 
 unsigned back_blocks(const JobDesc &h)
 {
+    const unsigned bz = (WG / BP_X) * (unsigned)(h.bp_zt ? h.bp_zt : BP_ZT);
     const unsigned nxb = (h.X + BP_X - 1) / BP_X;
-    const unsigned nzb = (h.T + BP_Z - 1) / BP_Z;
+    const unsigned nzb = (h.T + bz - 1) / bz;
     return nxb * nzb * (unsigned)h.nyb;
 }
 
-// Voxel-driven back-projection of one 64 x BP_Z voxel tile of one row block.
-// Per pass, the residual cells the tile can touch for G tilts are staged
-// in LDS as float planes (half pairs for half storage); every thread then
-// gathers its voxels' two taps per tilt from LDS with the exact double
-// geometry.  MODE selects the update (BackMode).  For BACK_SIRT and
-// BACK_SART the image is the residual, for BACK_BPT the (filtered) stack.
+// Voxel-driven back-projection of one 64 x BZ voxel tile of one row block,
+// BZ = 4 ZT (ZT voxels per thread, z, z + 4, ...).  Per pass, the residual
+// cells the tile can touch for G tilts are staged in LDS as float planes
+// (half pairs for half storage); every thread then gathers its voxels' two
+// taps per tilt from LDS with the exact double geometry.  MODE selects the
+// update (BackMode).  For BACK_SIRT and BACK_SART the image is the residual,
+// for BACK_BPT the (filtered) stack.  G and ZT are the shapes the autotuner
+// chooses among; every job of a launch has the same.
 
-template <typename PT, int MODE>
+template <typename PT, int MODE, int G, int ZT>
 __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
 {
+    constexpr int BZ = (WG / BP_X) * ZT;
+    constexpr int BW = BP_X + BZ + 4;  // taps of a tile span at most 64 + BZ cells
     // half storage keeps the residual in half in LDS: entry (t, q, c) holds
     // rows 4q .. 4q+3 of cells c and c+1 as (c, c+1) pairs
     constexpr bool HALF = std::is_same<PT, _Float16>::value;
-    constexpr int G = sizeof(PT) == 4 ? BP_G32 : BP_G;  // tilts per pass
-    __shared__ float4 win[G][4][BP_W];
+    __shared__ float4 win[G][4][BW];
     __shared__ int gmin_s[G];
     const unsigned b0 = xcd_remap(blockIdx.x, nblk);
     const int j = find_job(L, b0);
     KJob &J = *kconst(L.job[j]);
     unsigned lb = b0 - L.off[j];
-    const unsigned nzb = (J.T + BP_Z - 1) / BP_Z;
+    const unsigned nzb = (J.T + BZ - 1) / BZ;
     const unsigned nxb = (J.X + BP_X - 1) / BP_X;
     const int zb = lb % nzb;
     lb /= nzb;
@@ -809,21 +815,21 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
     const int X = J.X;
     const int xl = threadIdx.x % BP_X;
     const int x = xb * BP_X + xl;
-    const int zt0 = zb * BP_Z + (int)(threadIdx.x / BP_X);  // thread's voxels: zt0 + 4k
+    const int zt0 = zb * BZ + (int)(threadIdx.x / BP_X);  // thread's voxels: zt0 + 4k
     const double ox = kconst(J.tc)[0].ox, oz = kconst(J.tc)[0].oz;
     const double Xc = (double)((float)x - (float)ox);
-    double Zc[BP_ZT];
+    double Zc[ZT];
 #pragma unroll
-    for (int k = 0; k < BP_ZT; ++k) Zc[k] = (double)((float)(zt0 + 4 * k) - (float)oz);
+    for (int k = 0; k < ZT; ++k) Zc[k] = (double)((float)(zt0 + 4 * k) - (float)oz);
     const void *img = (MODE == BACK_BPT) ? J.proj : J.res;
     const PT *imgb = static_cast<const PT *>(img) + (size_t)yb * J.P * X * RB;
     const int th0 = L.theta0[j];
     const int th1 = L.ntheta[j] ? th0 + L.ntheta[j] : J.P;
 
-    float val[BP_ZT][RB];
-    float cs[BP_ZT];
+    float val[ZT][RB];
+    float cs[ZT];
 #pragma unroll
-    for (int k = 0; k < BP_ZT; ++k) {
+    for (int k = 0; k < ZT; ++k) {
         cs[k] = 0.f;
 #pragma unroll
         for (int r = 0; r < RB; ++r) val[k][r] = 0.f;
@@ -835,8 +841,8 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
         if ((int)threadIdx.x < ng) {
             // lowest tap of the tile: u is monotone in x and z, so a corner
             const double X0 = (double)((float)(xb * BP_X) - (float)ox);
-            const double Z0 = (double)((float)(zb * BP_Z) - (float)oz);
-            const double Z1 = (double)((float)(zb * BP_Z + BP_Z - 1) - (float)oz);
+            const double Z0 = (double)((float)(zb * BZ) - (float)oz);
+            const double Z1 = (double)((float)(zb * BZ + BZ - 1) - (float)oz);
             int ga, gb;
             float d;
             exact_tap(J.tc[tg + threadIdx.x], X0, Z0, ga, d);  // per-lane tilt: vector loads
@@ -847,10 +853,10 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
         if constexpr (HALF) {
             // plane p (rows 8p .. 8p+7) of cells g and g+1 -> entries q = 2p
             // and 2p + 1 as (g, g+1) pairs
-            for (int v = threadIdx.x; v < ng * 2 * BP_W; v += WG) {
-                const int t = v / (2 * BP_W);
-                const int rem = v % (2 * BP_W);
-                const int pl = rem / BP_W, c = rem % BP_W;
+            for (int v = threadIdx.x; v < ng * 2 * BW; v += WG) {
+                const int t = v / (2 * BW);
+                const int rem = v % (2 * BW);
+                const int pl = rem / BW, c = rem % BW;
                 const int g = gmin_s[t] + c;
                 const PT *rows = imgb + (size_t)(tg + t) * X * RB + (size_t)pl * X * 8;
                 uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
@@ -867,10 +873,10 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
         if constexpr (!HALF && sizeof(PT) == 2) {
             // bfloat16: plane p (rows 8p .. 8p+7) of cell g -> entries 2p and
             // 2p + 1 as floats
-            for (int v = threadIdx.x; v < ng * 2 * BP_W; v += WG) {
-                const int t = v / (2 * BP_W);
-                const int rem = v % (2 * BP_W);
-                const int pl = rem / BP_W, c = rem % BP_W;
+            for (int v = threadIdx.x; v < ng * 2 * BW; v += WG) {
+                const int t = v / (2 * BW);
+                const int rem = v % (2 * BW);
+                const int pl = rem / BW, c = rem % BW;
                 const int g = gmin_s[t] + c;
                 float e[8];
 #pragma unroll
@@ -884,10 +890,10 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
             }
         }
         // float: rows 4q .. 4q+3 of cell g
-        for (int v = threadIdx.x; sizeof(PT) == 4 && v < ng * 4 * BP_W; v += WG) {
-            const int t = v / (4 * BP_W);
-            const int rem = v % (4 * BP_W);
-            const int q = rem / BP_W, c = rem % BP_W;
+        for (int v = threadIdx.x; sizeof(PT) == 4 && v < ng * 4 * BW; v += WG) {
+            const int t = v / (4 * BW);
+            const int rem = v % (4 * BW);
+            const int q = rem / BW, c = rem % BW;
             const int g = gmin_s[t] + c;
             float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
             if ((unsigned)g < (unsigned)X)
@@ -899,11 +905,11 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
             KTilt &tc = kconst(J.tc)[tg + t];
             const int gm = gmin_s[t];
 #pragma unroll
-            for (int k = 0; k < BP_ZT; ++k) {
+            for (int k = 0; k < ZT; ++k) {
                 int g;
                 float dx;
                 exact_tap(tc, Xc, Zc[k], g, dx);
-                const int c = min(max(g - gm, 0), BP_W - 2);
+                const int c = min(max(g - gm, 0), BW - 2);
                 const float w0 = 1.f - dx, w1 = dx;
                 if constexpr (HALF) {
                     // both taps of a row in one v_dot2_f32_f16: weights and
@@ -971,7 +977,7 @@ __global__ __launch_bounds__(WG) void k_back(const LaunchList L, unsigned nblk)
     // a wave holds 64 consecutive x of one z row (xl is the lane)
     const bool live = x < X;
 #pragma unroll
-    for (int k = 0; k < BP_ZT; ++k) {
+    for (int k = 0; k < ZT; ++k) {
         const int z = zt0 + 4 * k;
         if (z >= J.T) continue;  // uniform in the wave
         float *vslice = J.vol + ((size_t)yb * J.T + z) * X * RB;
@@ -1552,26 +1558,73 @@ hipError_t launch_combine(const LaunchList &L, unsigned nblk, int storage, hipSt
     return hipGetLastError();
 }
 
-template <int MODE>
-static void launch_back_mode(const LaunchList &L, unsigned nblk, int storage, hipStream_t st)
+// The back-projection shapes (tilts per pass, voxels per thread along z)
+// instantiated for SIRT, the autotuner's candidates; the first is the
+// default.  The other modes use the default only.
+const std::vector<std::pair<int, int>> &back_shapes(int storage)
 {
-    switch (storage) {
-    case TREC_STORE_F16: hipLaunchKernelGGL((k_back<_Float16, MODE>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
-    case TREC_STORE_BF16: hipLaunchKernelGGL((k_back<__bf16, MODE>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
-    default: hipLaunchKernelGGL((k_back<float, MODE>), dim3(nblk), dim3(WG), 0, st, L, nblk); break;
-    }
+    static const std::vector<std::pair<int, int>> f32 = {{BP_G32, BP_ZT}, {4, 4}, {8, 4}, {6, 2}, {8, 2}};
+    static const std::vector<std::pair<int, int>> f16 = {{BP_G, BP_ZT}, {6, 4}, {12, 4}, {8, 2}};
+    static const std::vector<std::pair<int, int>> bf16 = {{BP_G, BP_ZT}};
+    return storage == TREC_STORE_F16 ? f16 : (storage == TREC_STORE_BF16 ? bf16 : f32);
 }
 
-hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode, hipStream_t st)
+template <typename PT, int MODE, int G, int ZT>
+static void back_one(const LaunchList &L, unsigned nblk, hipStream_t st)
 {
-    switch (mode) {
-    case BACK_SIRT: launch_back_mode<BACK_SIRT>(L, nblk, storage, st); break;
-    case BACK_BPT: launch_back_mode<BACK_BPT>(L, nblk, storage, st); break;
-    case BACK_LIN:
-        hipLaunchKernelGGL((k_back<float, BACK_LIN>), dim3(nblk), dim3(WG), 0, st, L, nblk);
-        break;
-    default: launch_back_mode<BACK_SART>(L, nblk, storage, st); break;
+    hipLaunchKernelGGL((k_back<PT, MODE, G, ZT>), dim3(nblk), dim3(WG), 0, st, L, nblk);
+}
+
+template <int MODE>
+static hipError_t launch_back_mode(const LaunchList &L, unsigned nblk, int storage, hipStream_t st)
+{
+    switch (storage) {
+    case TREC_STORE_F16: back_one<_Float16, MODE, BP_G, BP_ZT>(L, nblk, st); break;
+    case TREC_STORE_BF16: back_one<__bf16, MODE, BP_G, BP_ZT>(L, nblk, st); break;
+    default: back_one<float, MODE, BP_G32, BP_ZT>(L, nblk, st); break;
     }
+    return hipSuccess;
+}
+
+static hipError_t launch_back_sirt(const LaunchList &L, unsigned nblk, int storage, int g, int zt, hipStream_t st)
+{
+    if (storage == TREC_STORE_F16) {
+        if (g == BP_G && zt == BP_ZT) back_one<_Float16, BACK_SIRT, BP_G, BP_ZT>(L, nblk, st);
+        else if (g == 6 && zt == 4) back_one<_Float16, BACK_SIRT, 6, 4>(L, nblk, st);
+        else if (g == 12 && zt == 4) back_one<_Float16, BACK_SIRT, 12, 4>(L, nblk, st);
+        else if (g == 8 && zt == 2) back_one<_Float16, BACK_SIRT, 8, 2>(L, nblk, st);
+        else return hipErrorInvalidValue;
+    } else if (storage == TREC_STORE_BF16) {
+        if (g == BP_G && zt == BP_ZT) back_one<__bf16, BACK_SIRT, BP_G, BP_ZT>(L, nblk, st);
+        else return hipErrorInvalidValue;
+    } else {
+        if (g == BP_G32 && zt == BP_ZT) back_one<float, BACK_SIRT, BP_G32, BP_ZT>(L, nblk, st);
+        else if (g == 4 && zt == 4) back_one<float, BACK_SIRT, 4, 4>(L, nblk, st);
+        else if (g == 8 && zt == 4) back_one<float, BACK_SIRT, 8, 4>(L, nblk, st);
+        else if (g == 6 && zt == 2) back_one<float, BACK_SIRT, 6, 2>(L, nblk, st);
+        else if (g == 8 && zt == 2) back_one<float, BACK_SIRT, 8, 2>(L, nblk, st);
+        else return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode, hipStream_t st, int g, int zt)
+{
+    const auto &def = back_shapes(storage)[0];
+    if (g == 0) g = def.first;
+    if (zt == 0) zt = def.second;
+    hipError_t e = hipSuccess;
+    if (mode == BACK_SIRT) {
+        e = launch_back_sirt(L, nblk, storage, g, zt, st);
+    } else {
+        if (g != def.first || zt != def.second) return hipErrorInvalidValue;
+        switch (mode) {
+        case BACK_BPT: e = launch_back_mode<BACK_BPT>(L, nblk, storage, st); break;
+        case BACK_LIN: back_one<float, BACK_LIN, BP_G32, BP_ZT>(L, nblk, st); break;
+        default: e = launch_back_mode<BACK_SART>(L, nblk, storage, st); break;
+        }
+    }
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

@@ -6,6 +6,9 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <utility>
+#include <vector>
+
 #include "tiltrec.h"
 
 namespace trec {
@@ -86,6 +89,7 @@ struct JobDesc {
     int nzc;       // z chunks of the forward projection (1 = fused residual)
     int zc_len;    // layers per z chunk
     int storage;   // trec_storage of proj / res
+    int bp_g, bp_zt;  // SIRT back-projection shape: tilts per pass, voxels per thread along z (0: default)
     float gamma;
     float thr_upd;  // SIRT: C > thr_upd; BPT/SART: c > thr_upd
     float thr_res;  // residual: rwt > thr_res (0: rwt != 0)
@@ -130,7 +134,13 @@ hipError_t launch_shadow(const JobDesc &h, const JobDesc *d, hipStream_t st);
 hipError_t launch_forward(const LaunchList &L, unsigned nblk, int storage, bool fused, hipStream_t st,
                           bool small = false);
 hipError_t launch_combine(const LaunchList &L, unsigned nblk, int storage, hipStream_t st);
-hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode, hipStream_t st);
+// g, zt: the back-projection shape (tilts staged per pass, voxels per thread
+// along z; 0: the default).  Only SIRT has other shapes (back_shapes).
+hipError_t launch_back(const LaunchList &L, unsigned nblk, int storage, int mode, hipStream_t st, int g = 0,
+                       int zt = 0);
+// (tilts per pass, voxels per thread along z) of the SIRT back-projection
+// for a storage type; the first is the default.
+const std::vector<std::pair<int, int>> &back_shapes(int storage);
 hipError_t launch_rweight(float *data, const float *hk, int N, int X, long long nrows,
                           long long row_stride, int x_stride, int rows_per_group,
                           long long group_stride, hipStream_t st);

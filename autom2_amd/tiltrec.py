@@ -49,7 +49,8 @@ class Method(ctypes.Structure):
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [("path", ctypes.c_int), ("zchunks", ctypes.c_int), ("band_cells", ctypes.c_int),
-                ("back_tilts", ctypes.c_int), ("slabs", ctypes.c_int), ("slab_rows", ctypes.c_int),
+                ("back_tilts", ctypes.c_int), ("back_voxels_z", ctypes.c_int), ("slabs", ctypes.c_int),
+                ("slab_rows", ctypes.c_int),
                 ("workspace", ctypes.c_size_t)]
 
 

@@ -642,7 +642,15 @@ def main():
     method = trec.make_method(trec.SIRT, args.iters, 1.0)
     ctx.autotune(geom, method)
     plan = ctx.plan(geom, ang, method)
-    log(f"[bench] autotune {time.perf_counter() - t0:.2f} s; plan {plan}")
+    t_tune = time.perf_counter() - t0
+    # the tuner again on a fresh context: the same choice?
+    ctx2 = trec.Context(0)
+    ctx2.autotune(geom, method)
+    plan["autotune_repeat_same"] = ctx2.plan(geom, ang, method) == plan
+    ctx2.close()
+    ctx.autotune(geom, method)  # the process table now holds this context's choice again
+    plan = dict(ctx.plan(geom, ang, method), autotune_repeat_same=plan["autotune_repeat_same"])
+    log(f"[bench] autotune {t_tune:.2f} s; plan {plan}")
 
     updates = float(X) * Y * T * P * args.iters
     log(f"[bench] timing FP32 SIRT x{args.iters}: warmup {args.warmup}, steps {args.steps}")

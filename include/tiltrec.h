@@ -175,6 +175,8 @@ typedef struct {
     int band_cells;    /* LDS band of the tilt-group forward: 352 or 400 cells
                           (0: no forward projection) */
     int back_tilts;    /* tilts the row-path back-projection stages per pass */
+    int back_voxels_z; /* voxels per thread along z of its tile (the tile is
+                          64 x 4 back_voxels_z voxels of 16 rows) */
     int slabs;         /* y slabs the volume is reconstructed in */
     int slab_rows;     /* volume rows per slab */
     size_t workspace;  /* device workspace bytes of the call */
@@ -331,9 +333,14 @@ typedef struct {
 trec_status trec_reconstruct_batch(trec_context *ctx, int njobs,
                                    const trec_job *jobs, void *stream);
 
-/* Autotuning: tries the launch shapes of the forward projection for this
- * geometry and keeps the fastest in the context (and in the process-wide
- * table used by later contexts).  Synchronises the device. */
+/* Autotuning, replacing TiltRec-cuda's fixed slab heuristic InitYMAX
+ * (recon_y.cu:75-86): for a SIRT geometry, times whole iterations with the
+ * forward projection's z chunks 1, 2, 4, ... and then with the
+ * back-projection's tile shapes (tilts staged per pass, voxels per thread
+ * along z), and keeps the fastest in the context and in the process-wide
+ * table later contexts read (trec_plan shows the choice).  A candidate
+ * replaces an earlier, preferred one only when it is more than 2 % faster,
+ * so the choice is stable from run to run.  Synchronises the device. */
 trec_status trec_autotune(trec_context *ctx, const trec_geom *geom,
                           const trec_method *method);
 

@@ -424,3 +424,26 @@ def test_slabs_equal_whole_volume(torch_cuda, trec, orc, method, storage, layout
             ref = orc.bpt_f32(coef, stack, T, org, vol=init)
         got = np.transpose(sl, (1, 0, 2)) if layout else sl
         assert report(f"{method} in slabs", got, ref) < TOL_F32
+
+
+@pytest.mark.parametrize("storage,shapes", [("f32", [(6, 4), (4, 4), (8, 4), (6, 2), (8, 2)]),
+                                            ("f16", [(8, 4), (6, 4), (12, 4), (8, 2)])])
+def test_back_shapes_equal(torch_cuda, trec, orc, monkeypatch, storage, shapes):
+    """The autotuner's SIRT back-projection shapes (tilts staged per pass,
+    voxels per thread along z) change the staging and the tile, not the sums:
+    every shape gives the default's volume bit for bit, at a size with
+    partial tiles in x and z."""
+    P, Y, T, X = 11, 20, 37, 100
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=61)
+    ctx = trec.Context(0)
+    vols = []
+    for g, zt in shapes:
+        monkeypatch.setenv("TREC_BACK_SHAPE", f"{g},{zt}")
+        info = ctx.plan(trec.make_geom(X, Y, P, T), ang, trec.make_method("SIRT,3,1.0", storage=storage))
+        assert (info["back_tilts"], info["back_voxels_z"]) == (g, zt)
+        vols.append(run(torch_cuda, trec, ctx, stack, ang, T, "SIRT,3,1.0", storage=storage))
+    ctx.close()
+    for (g, zt), v in zip(shapes[1:], vols[1:]):
+        np.testing.assert_array_equal(v, vols[0], err_msg=f"shape {g},{zt}")
+    if storage == "f32":
+        assert report("SIRT x3, tuned shapes", vols[0], orc.sirt_f32(coef, stack, T, org, 3, 1.0)) < TOL_F32
