@@ -17,6 +17,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -428,22 +429,29 @@ def geopar_stage(trec, ctx, torch, ang, stack_d, reps=5, cpu=True):
     return res
 
 
-def counters_run(iters):
+def counters_run(iters, config="sirt1024"):
     """The run the counter profiles measure: `iters` SIRT iterations of the
-    bench workload through the library, then a 1 GiB device copy whose
-    FETCH_SIZE / WRITE_SIZE calibrate the HBM byte counters."""
+    configuration's series through the library after autotuning, as the
+    bench runs them (FP32, and FP16 storage too for sirt2048), then a 1 GiB
+    device copy whose FETCH_SIZE / WRITE_SIZE calibrate the HBM byte
+    counters."""
     import torch
     from autom2_amd import synth, tiltrec as trec
-    c = synth.CONFIGS["sirt1024"]
-    ang, stack, _, _ = synth.config_series("sirt1024")
+    c = synth.CONFIGS[config]
+    ang, stack, _, _ = synth.config_series(config)
     s = torch.from_numpy(stack).cuda()
+    del stack
     g = trec.make_geom(c["X"], c["Y"], c["P"], c["T"])
     ctx = trec.Context(0)
     vol = torch.empty((c["Y"], c["T"], c["X"]), device="cuda")
-    m = trec.make_method(trec.SIRT, iters, 1.0)
-    ctx.autotune(g, m)  # the launch shapes the bench uses
-    ctx.reconstruct(g, ang, s, m, vol)
-    torch.cuda.synchronize()
+    for sto in ([trec.STORE_F32] if config == "sirt1024" else [trec.STORE_F32, trec.STORE_F16]):
+        m = trec.make_method(trec.SIRT, iters, 1.0, storage=sto)
+        ctx.autotune(g, m)  # the launch shapes the bench uses
+        log(f"[bench] counters run: storage {sto}, plan {ctx.plan(g, ang, m)}")
+        ctx.reconstruct(g, ang, s, m, vol)
+        torch.cuda.synchronize()
+    del vol, s
+    torch.cuda.empty_cache()
     a = torch.empty(256 * 1024 * 1024, device="cuda")
     b = torch.empty_like(a)
     a.fill_(1.0)
@@ -477,7 +485,8 @@ def profiled(kernel, column, summary="sirt1024_f32_counters.md", prefix=False):
             if column in cells:
                 col = cells.index(column)
             elif col is not None:
-                name = cells[1].strip("`").split("(")[0].strip()
+                m = re.match(r"`([^`]*)`", cells[1])
+                name = (m.group(1) if m else cells[1]).split("(")[0].strip()
                 if name == kernel or (prefix and name.startswith(kernel)):
                     return float(cells[col].replace(",", ""))
     except (OSError, ValueError):
@@ -507,9 +516,10 @@ def kernel_roofline(kern, plan, X, Y, T, P, storage_bytes=4, summary=None):
            "back": 8 * V + sb * PA}
     dom = max(("forward", "back"), key=lambda k: kern[k]["ms_total"])
     st = {4: "float", 2: "_Float16"}[sb]
-    names = {"forward": (f"k_forward_g<{st}, true, false, {plan['band_cells']}>" if nzc == 1
-                         else f"k_forward_g<{st}, false, false, 400>"),
-             "back": f"k_back<{st}, 0>"}
+    hv = "true" if sb == 2 else "false"  # FP16 forwards read the half volume copy
+    names = {"forward": (f"k_forward_g<{st}, true, {hv}, {plan['band_cells']}>" if nzc == 1
+                         else f"k_forward_g<float, false, {hv}, 400>"),
+             "back": f"k_back<{st}, 0, {plan['back_tilts']}, {plan['back_voxels_z']}>"}
     name = names[dom]
     ms = kern[dom]["ms_avg"]
     achieved = alg[dom] / (ms / 1e3) / 1e9
@@ -605,7 +615,7 @@ def main():
                          "measure them")
     args = ap.parse_args()
     if args.counters:
-        counters_run(2)
+        counters_run(2, args.config)
         return
 
     if args.gpus != 1:
