@@ -39,6 +39,7 @@
 #include <thread>
 #include <vector>
 
+#include "range_alloc.h"
 #include "trec_host.h"
 
 using namespace trec;
@@ -75,7 +76,8 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// First-fit allocator over one device allocation.
+// First-fit allocator over one device allocation (range_alloc.h keeps the
+// offsets).
 class Arena {
   public:
     bool init(size_t bytes)
@@ -85,8 +87,7 @@ class Arena {
             base_ = nullptr;
             return false;
         }
-        size_ = bytes;
-        free_[0] = bytes;
+        ranges_.reset(bytes);
         return true;
     }
     ~Arena()
@@ -95,52 +96,16 @@ class Arena {
     }
     char *alloc(size_t bytes)
     {
-        bytes = align_up(bytes);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= bytes) {
-                const size_t off = it->first, len = it->second;
-                free_.erase(it);
-                if (len > bytes) free_[off + bytes] = len - bytes;
-                used_[off] = bytes;
-                return base_ + off;
-            }
-        }
-        return nullptr;
+        const size_t off = ranges_.alloc(bytes);
+        return off == RangeAllocator::NONE ? nullptr : base_ + off;
     }
-    void release(char *p)
-    {
-        const size_t off = (size_t)(p - base_);
-        auto u = used_.find(off);
-        if (u == used_.end()) return;
-        size_t len = u->second;
-        used_.erase(u);
-        auto next = free_.lower_bound(off);
-        if (next != free_.end() && next->first == off + len) {
-            len += next->second;
-            free_.erase(next);
-        }
-        auto it = free_.emplace(off, len).first;
-        if (it != free_.begin()) {
-            auto prev = std::prev(it);
-            if (prev->first + prev->second == off) {
-                prev->second += len;
-                free_.erase(it);
-            }
-        }
-    }
-    size_t size() const { return size_; }
-    bool fits(size_t bytes) const
-    {
-        bytes = align_up(bytes);
-        for (const auto &f : free_)
-            if (f.second >= bytes) return true;
-        return false;
-    }
+    void release(char *p) { ranges_.release((size_t)(p - base_)); }
+    size_t size() const { return ranges_.size(); }
+    bool fits(size_t bytes) const { return ranges_.fits(bytes); }
 
   private:
     char *base_ = nullptr;
-    size_t size_ = 0;
-    std::map<size_t, size_t> free_, used_;
+    RangeAllocator ranges_;
 };
 
 struct Request {

@@ -127,6 +127,7 @@ int main(int argc, char **argv)
     const int X = hin.nx, Y = hin.ny, P = hin.nz, T = o.thickness;
     printf("TiltRec-hip: %d projections of %d x %d -> %d x %d x %d (%s), method %s\n", P, X, Y, X, Y, T,
            layout == TREC_LAYOUT_Z ? "TiltRecZ orientation" : "TiltRec orientation", o.method.c_str());
+    fflush(stdout);  // a caller watching the pipe sees the run start
 
     MrcHeader hi{};
     std::vector<unsigned char> init;

@@ -96,3 +96,40 @@ def test_cli_geometry(tmp_path, orc, name, geometry):
     e = orc.rel_l2(vol, ref)
     print(f"{name} --geometry {off},{pitch},{zs},{T}: rel L2 {e:.3e}")
     assert e < 1e-5
+
+
+def test_cli_terminated_mid_reconstruction(tmp_path):
+    """SIGTERM while the CLI's SIRT is on the GPU: the process ends without
+    writing a volume (the reference writes per slab, so a killed run can
+    leave a partial file), and the GPU serves the next reconstruction."""
+    import signal
+    import time
+
+    import torch
+    from autom2_amd import tiltrec as trec
+    P, Y, X, T = 61, 512, 512, 256
+    rng = np.random.default_rng(9)
+    mrc.write(str(tmp_path / "stack.mrc"), rng.random((P, Y, X), dtype=np.float32))
+    with open(tmp_path / "tilt.tlt", "w") as f:
+        f.write("\n".join(f"{a:.2f}" for a in tilts(P)) + "\n")
+    out = tmp_path / "vol.mrc"
+    # several seconds of GPU work: 512^2 x 256 x 61 x 2500 voxel-projection updates
+    p = subprocess.Popen([os.path.join(BIN_DIR, "TiltRec-mpi"), "--input", str(tmp_path / "stack.mrc"),
+                          "--output", str(out), "--tiltfile", str(tmp_path / "tilt.tlt"), "--geometry",
+                          f"0,0,0,{T}", "--method", "SIRT,2500,1.0"], stdout=subprocess.PIPE, text=True)
+    line = p.stdout.readline()  # printed once the stack is read, before the upload
+    assert line.startswith("TiltRec-hip:"), line
+    time.sleep(1.0)
+    assert p.poll() is None, "the reconstruction ended before the signal"
+    p.send_signal(signal.SIGTERM)
+    rc = p.wait(timeout=30)
+    print(f"CLI ended with {rc}")
+    assert rc == -signal.SIGTERM
+    assert not out.exists()
+    ctx = trec.Context(0)
+    s = torch.ones((5, 16, 32), device="cuda")
+    v = torch.empty((16, 8, 32), device="cuda")
+    ctx.reconstruct(trec.make_geom(32, 16, 5, 8), tilts(5), s, trec.make_method("BPT"), v)
+    torch.cuda.synchronize()
+    assert float(v[:, :, 12:20].min()) == 5.0  # interior voxels see every tilt of a stack of ones
+    ctx.close()

@@ -260,3 +260,38 @@ def test_server_admm_equals_alone(torch_cuda, trec, orc):
     for c, out, ref, rec in zip(cases, outs, refs, recs):
         assert rec["status"] == 0
         np.testing.assert_array_equal(out, ref, err_msg=c[4])
+
+
+def test_device_allocation_failures_are_reported(torch_cuda, trec):
+    """Allocation failures come back as TREC_ERR_NOMEM and leave the context
+    and the device usable: a workspace larger than the device (ADMM keeps
+    seven volume-sized vectors and does not run in slabs), and a server arena
+    of 1 PiB."""
+    torch = torch_cuda
+    ctx = trec.Context(0)
+    X, Y, T, P = 4096, 4096, 1024, 61
+    stack = torch.zeros((P, Y, X), device="cuda")
+    vol = torch.empty((Y, T, X), device="cuda")
+    g = trec.make_geom(X, Y, P, T)
+    m = trec.make_method("ADMM,1,1,0.5,0.0")
+    print(f"workspace {trec.workspace_size(g, m) / 2**30:.0f} GiB")
+    with pytest.raises(trec.TrecError) as ei:   # refused by the budget (free device memory)
+        ctx.reconstruct(g, tilts(P), stack, m, vol)
+    assert ei.value.status == trec.ERR_NOMEM
+    ctx.set_workspace_limit(1 << 62)            # no budget: hipMalloc itself fails
+    with pytest.raises(trec.TrecError) as ei:
+        ctx.reconstruct(g, tilts(P), stack, m, vol)
+    assert ei.value.status == trec.ERR_NOMEM
+    print(ei.value)
+    ctx.set_workspace_limit(0)
+    del stack, vol
+    torch.cuda.empty_cache()
+    s = torch.ones((5, 16, 32), device="cuda")
+    v = torch.empty((16, 8, 32), device="cuda")
+    ctx.reconstruct(trec.make_geom(32, 16, 5, 8), tilts(5), s, trec.make_method("BPT"), v)
+    torch.cuda.synchronize()
+    assert float(v[:, :, 12:20].min()) == 5.0
+    ctx.close()
+    with pytest.raises(trec.TrecError) as ei:
+        trec.Server(0, budget=1 << 50, max_active=2)
+    assert ei.value.status == trec.ERR_NOMEM
