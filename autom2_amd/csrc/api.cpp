@@ -106,9 +106,9 @@ struct ProfScope {
     int kind;
     hipStream_t st;
     hipEvent_t a = nullptr;
-    ProfScope(int k, hipStream_t s) : kind(k), st(s)
+    ProfScope(int k, hipStream_t s, bool when = true) : kind(k), st(s)
     {
-        if (tl_prof && tl_prof->on) {
+        if (when && tl_prof && tl_prof->on) {
             a = tl_prof->get();
             if (a && hipEventRecord(a, st) != hipSuccess) a = nullptr;
         }
@@ -520,6 +520,7 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             t.oy = oy;
             t.oz = oz;
         }
+        general_tiles(p.tg.data(), g.nproj, g.nx);
         filter_taps(p, m.method, g.nx);
     }
     return TREC_OK;
@@ -833,23 +834,34 @@ hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
     GeneralJob &q = p.gj;
     const size_t X = p.g.nx, Y = p.g.ny, P = p.g.nproj;
     switch (p.m.method) {
-    case TREC_SIRT:
-        LCHK(launch_zero(q.rval, P * Y * X, st));
-        LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, false, st));
-        LCHK(launch_general_residual(q, 0, (int)P, false, st));
+    case TREC_SIRT: {
+        {
+            ProfScope ps(PROF_FORWARD, st);
+            LCHK(launch_zero(q.rval, P * Y * X, st));
+            LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, false, st));
+            LCHK(launch_general_residual(q, 0, (int)P, false, st));
+        }
+        ProfScope ps(PROF_BACK, st);
         return launch_general_back(q, p.tg.data(), BACK_SIRT, 0, (int)P, q.rval, st);
+    }
     case TREC_SART: {
         const int th = s % (int)P;
-        LCHK(launch_zero(q.rval + (size_t)th * Y * X, Y * X, st));
-        int nzc = 1;
-        LCHK(launch_general_forward(q, p.tg.data(), th, 1, false, st, &nzc));
-        LCHK(launch_general_residual(q, th, 1, false, st, nzc));
+        {
+            ProfScope ps(PROF_FORWARD, st);
+            LCHK(launch_zero(q.rval + (size_t)th * Y * X, Y * X, st));
+            int nzc = 1;
+            LCHK(launch_general_forward(q, p.tg.data(), th, 1, false, st, &nzc));
+            LCHK(launch_general_residual(q, th, 1, false, st, nzc));
+        }
+        ProfScope ps(PROF_BACK, st);
         return launch_general_back(q, p.tg.data(), BACK_SART, th, 1, q.rval, st);
     }
     case TREC_ADMM:
         return issue_admm_general(p, st);
-    default:
+    default: {
+        ProfScope ps(PROF_BACK, st);
         return launch_general_back(q, p.tg.data(), BACK_BPT, 0, (int)P, is_filtered(p.m.method) ? p.stack_copy : p.stack, st);
+    }
     }
 }
 
@@ -931,8 +943,13 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
             break;
         }
     }
+    auto any = [](const Batcher *b, int n) {
+        for (int i = 0; i < n; ++i)
+            if (!b[i].items.empty()) return true;
+        return false;
+    };
     {
-        ProfScope ps(PROF_FORWARD, st);
+        ProfScope ps(PROF_FORWARD, st, any(fp_fused, 3) || any(fp_small, 3) || any(fp_part, 3));
         for (int sto = 0; sto < 3; ++sto)
             LCHK(fp_fused[sto].flush(
                 [&](const LaunchList &L, unsigned n) { return launch_forward(L, n, sto, true, st); }));
@@ -949,7 +966,7 @@ hipError_t issue_step(const std::vector<JobPlan *> &plans, std::vector<int> &sta
             LCHK(comb[sto].flush([&](const LaunchList &L, unsigned n) { return launch_combine(L, n, sto, st); }));
     }
     {
-        ProfScope ps(PROF_BACK, st);
+        ProfScope ps(PROF_BACK, st, !back_sirt.empty() || any(&back[0][0], 9));
         for (auto &kb : back_sirt) {
             const int sto = std::get<0>(kb.first), g = std::get<1>(kb.first), zt = std::get<2>(kb.first);
             LCHK(kb.second.flush(

@@ -13,11 +13,15 @@
 //   k_general_rows        forward projection when v does not depend on x
 //                         (offset 0 or pitch 0): the splat factors into an
 //                         x pass per volume row and a y pass over rows.
+//   k_general_splat       forward projection of any pitch and offset (v may
+//                         depend on x): the reference's splat into a tile of
+//                         detector cells in LDS, owned by one wave.
 //   k_general_back_rows   back-projection: v and its two detector rows are
 //                         evaluated once per voxel, u once per voxel and tilt.
-//   k_general_back_lds    the same when v does not depend on x: a wave's row
-//                         of voxels shares its two detector rows, which the
-//                         wave stages in LDS per tilt.
+//   k_general_back_lds    the same with the taps staged in LDS: a wave's row
+//                         of voxels reaches two detector rows (v independent
+//                         of x) or a few more (a pitch with an offset), which
+//                         the wave stages per tilt.
 //
 // Geometries outside their conditions use the reference-order kernels
 // (k_general_gather, k_general_back).  Every kernel is deterministic.
@@ -27,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "trec_internal.h"
@@ -370,6 +375,146 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
         if (j0 + r < g.Y) out[(size_t)(j0 + r) * g.X + i] = acc[r];
 }
 
+// Forward projection as the reference computes it, a bilinear splat of
+// every voxel (Reproject, main-mpi.cpp:286-344), for tilts with a1 >= 1/3
+// and 0.55 <= b2 <= 1: any pitch with or without an offset.  A wave owns a
+// tile of SP_R detector rows by t.si columns of one tilt and sums it in LDS
+// over its z layers.  Per layer it walks the box of voxels whose taps can
+// reach the tile, one volume row of up to 64 voxels per pass (one per lane,
+// coalesced loads), and every voxel adds its four weighted taps to the
+// tile.  A tap outside the tile lands in padding that is never stored (the
+// tap position is clamped into the padded tile, which moves no voxel onto a
+// stored cell), so the box needs no exact edge.  The voxel's position is
+// ValCoef's u, v (main-mpi.cpp:204-226) in double at the box's first row of
+// each lane's column and advanced in float over the rows; its weights are
+// the reference's (1 - xd, xd) x (1 - yd, yd).
+//
+// No atomics: the lanes of a pass add their taps in t.ncp phases, lane %
+// ncp, and the voxels of one phase lie ncp columns apart, at least 1 + 1e-4
+// pixels in u, so within one tap's read-add-write no two lanes of a phase
+// touch the same cell.  Every cell therefore receives its adds in program
+// order (layers, rows, phases, taps) on every run.  (LDS float atomics would
+// need no phases, but gfx950 runs them lane by lane, some 30 times slower.)
+constexpr int SP_R = 16;      // detector rows per tile (SR): 8, 24 and 32 were slower
+constexpr int SP_CP = 6;      // padding columns on each side of a tile row
+constexpr int SP_RP = 2;      // padding rows above and below
+constexpr int SP_PITCH = 72;  // LDS floats per tile row
+constexpr int SP_SIMAX = SP_PITCH - 2 * SP_CP;
+static_assert(SP_SIMAX <= 64, "a tile row is stored by one wave");
+template <int SR, bool ONES>
+__global__ __launch_bounds__(64) void k_general_splat(const GeneralJob g, int theta0, int ntheta, int nzc,
+                                                      unsigned ntile)
+{
+    constexpr int SP_ROWS = SR + 2 * SP_RP;
+    __shared__ float acc[SP_ROWS * SP_PITCH];
+    const unsigned tw = blockIdx.x % ntile, rest = blockIdx.x / ntile;
+    const unsigned nrb = (g.Y + SR - 1) / SR;
+    const int rb = (int)(rest % nrb), zc = (int)(rest / nrb);
+    // the tilt of tile tw: the last whose first tile is at most tw
+    const int base = g.tg[theta0].tile0;
+    int lo = theta0, hi = theta0 + ntheta - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((unsigned)(g.tg[mid].tile0 - base) <= tw) lo = mid;
+        else hi = mid - 1;
+    }
+    const int th = lo;
+    const TiltGeneral &t = g.tg[th];
+    const int si = t.si, ncp = t.ncp;
+    const int i0 = (int)(tw - (unsigned)(t.tile0 - base)) * si, i1 = min(g.X, i0 + si);
+    const int j0 = rb * SR, j1 = min(g.Y, j0 + SR);
+    const int zlen = (g.T + nzc - 1) / nzc;
+    const int z0 = zc * zlen, z1 = min(g.T, z0 + zlen);
+    const int lane = (int)threadIdx.x;
+    for (int k = lane; k < SP_ROWS * SP_PITCH; k += 64) acc[k] = 0.f;
+    __syncthreads();
+
+    const float oxf = (float)t.ox, oyf = (float)t.oy, ozf = (float)t.oz;
+    const double oxd = oxf, oyd = oyf;
+    const double a1 = t.a[1], a2 = t.a[2], b1 = t.b[1], b2 = t.b[2];
+    const float a2f = (float)a2, c2f = (float)(1.0 - b2);  // v advances by 1 - c2 per row
+    // b1 X over the detector's columns
+    const double bxa = b1 * (double)(0.f - oxf), bxb = b1 * (double)((float)(g.X - 1) - oxf);
+    const double bl = fmin(bxa, bxb), bh = fmax(bxa, bxb);
+    const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
+    const size_t sz = g.layout == TREC_LAYOUT_Y ? (size_t)g.X : (size_t)g.Y * g.X;
+    const int ph = lane % ncp;
+    for (int z = z0; z < z1; ++z) {
+        const double Zc = (double)((float)z - ozf);
+        const double cu = t.a[0] + t.a[3] * Zc + oxd;  // u = cu + a1 X + a2 Y
+        const double cv = t.b[0] + t.b[3] * Zc + oyd;  // v = cv + b1 X + b2 Y
+        // rows whose v can fall in [j0 - 1, j1), columns whose u can fall in
+        // [i0 - 1, i1) on those rows.  A voxel the rounding of these bounds
+        // leaves out lies within rounding of the tile's edge, where its
+        // weight on the tile is of that order.
+        const int ya = max((int)floor(((double)(j0 - 1) - cv - bh) / b2 + oyd), 0);
+        const int yb = min((int)floor(((double)j1 - cv - bl) / b2 + oyd) + 1, g.Y);
+        if (ya >= yb) continue;
+        const double ya2 = a2 * (double)((float)ya - oyf), yb2 = a2 * (double)((float)(yb - 1) - oyf);
+        const double al = fmin(ya2, yb2), ah = fmax(ya2, yb2);
+        const int xa = max((int)floor(((double)(i0 - 1) - cu - ah) / a1 + oxd), 0);
+        const int xb = min((int)floor(((double)i1 - cu - al) / a1 + oxd) + 1, g.X);
+        const int nrow = yb - ya;
+        const double Ya = (double)((float)ya - oyf);
+        const float *layer = g.vol + (size_t)z * sz + (size_t)ya * sy;
+        for (int xs = xa; xs < xb; xs += 64) {
+            const int x = xs + lane;
+            const bool in = x < xb;
+            const double Xc = (double)((float)x - oxf);
+            const double U0 = cu + a1 * Xc + a2 * Ya, V0 = cv + b1 * Xc + b2 * Ya;
+            const double fU = floor(U0), fV = floor(V0);
+            const float fu0 = (float)(U0 - fU), fv0 = (float)(V0 - fV);
+            // LDS column and row of the first row's upper-left tap
+            const int cu0 = (int)fU - i0 + SP_CP, rv0 = (int)fV - j0 + SP_RP;
+            const float *colp = layer + min(x, g.X - 1);
+            for (int k0 = 0; k0 < nrow; k0 += 4) {
+                float vals[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) vals[q] = (ONES || k0 + q >= nrow) ? 1.f : colp[(size_t)(k0 + q) * sy];
+                if (!in) continue;  // lanes past the box (the whole row group)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int k = k0 + q;
+                    if (k >= nrow) break;
+                    const float val = vals[q];
+                    const float fu = fmaf(a2f, (float)k, fu0), fv = fmaf(-c2f, (float)k, fv0);
+                    const float flu = floorf(fu), flv = floorf(fv);
+                    const float fx = fu - flu, fy = fv - flv;
+                    const int cc = min(max(cu0 + (int)flu, 0), SP_PITCH - 2);
+                    const int rr = min(max(rv0 + k + (int)flv, 0), SP_ROWS - 2);
+                    float *cell = acc + rr * SP_PITCH + cc;
+                    const float t1 = val * fy, t0 = val - t1;
+                    const float w01 = t0 * fx, w00 = t0 - w01, w11 = t1 * fx, w10 = t1 - w11;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) {
+                        if (p < ncp && ph == p) {
+                            // the left taps of both rows, then the right
+                            // ones: within each group the lanes of a phase
+                            // hit distinct cells, and each group's reads see
+                            // the writes before them (a wave's LDS accesses
+                            // run in order; the wave barriers keep the
+                            // compiler from merging the groups, as lane x's
+                            // right tap may be lane x + ncp's left one)
+                            cell[0] += w00;
+                            cell[SP_PITCH] += w10;
+                            __builtin_amdgcn_wave_barrier();
+                            cell[1] += w01;
+                            cell[SP_PITCH + 1] += w11;
+                            __builtin_amdgcn_wave_barrier();
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float *out = nzc > 1 ? g.part + ((size_t)zc * ntheta + (th - theta0)) * g.Y * g.X
+                         : (ONES ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+    if (lane < i1 - i0)
+        for (int r = 0; r < j1 - j0; ++r)
+            out[(size_t)(j0 + r) * g.X + i0 + lane] = acc[(r + SP_RP) * SP_PITCH + SP_CP + lane];
+}
+
 // Back-projection for tilts that share b and the origins.  Each thread takes
 // GB_Q voxels 64 apart along x in one (y, z) row of the volume (one row per
 // wave).  v is evaluated once per voxel with ValCoef's operation order, u
@@ -451,25 +596,29 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
     }
 }
 
-// Back-projection for tilts that also have b1 = 0.  Then v, and with it the
-// two detector rows r0, r0 + 1 and their weights, is the same for every
-// voxel of a wave's (y, z) row.  Per tilt the wave stages the columns of
-// those two rows that its GL_Q * 64 voxels reach into its own LDS, with
-// coalesced loads: from c = min(max(trunc(u), 0), X - 2) at its least u, at
-// most (64 GL_Q - 1) |a1| + 3 <= GL_W columns.  It takes its taps from
-// there: half the bytes of reading them from memory, where neighbouring
-// lanes read the same columns again.  The next tilt's window is loaded while
-// the current one is used.  The weights are k_general_back_rows'; SIRT and
-// ADMM sum each detector row's column-weighted taps over the tilts and apply
-// the row weights, the same in every tilt, once per voxel.
+// Back-projection for tilts that share b, with the taps taken from LDS.
+// With b1 = 0 (NR = 2) v, and with it the two detector rows r0, r0 + 1 and
+// their weights, is the same for every voxel of a wave's (y, z) row.  With
+// b1 != 0 v moves by |b1| per voxel, less than NR - 2 rows over the wave's
+// GL_Q * 64 voxels, so their rows lie among the NR rows from the least r0;
+// each voxel keeps its own rows and weights (v does not depend on the tilt).
+// Per tilt the wave stages the columns of those rows that its voxels reach
+// into its own LDS, with coalesced loads: from c = min(max(trunc(u), 0),
+// X - 2) at its least u, at most (64 GL_Q - 1) |a1| + 3 <= GL_W columns.  It
+// takes its taps from there: half the bytes of reading them from memory,
+// where neighbouring lanes read the same columns again.  The next tilt's
+// window is loaded while the current one is used.  The weights are
+// k_general_back_rows'; SIRT and ADMM sum each voxel's two detector rows'
+// column-weighted taps over the tilts and apply the row weights, the same in
+// every tilt, once per voxel.
 constexpr int GL_Q = 8;        // voxels per lane, 64 apart
 constexpr int GL_W = 516;      // staged columns per row
-constexpr int GL_PITCH = 520;  // floats between the two staged rows
-template <int MODE>
+constexpr int GL_PITCH = 520;  // floats between two staged rows
+template <int MODE, int NR>
 __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int theta0, int ntheta,
                                                          const float *__restrict__ img)
 {
-    __shared__ float stage[WG / 64][2 * GL_PITCH];
+    __shared__ float stage[WG / 64][NR * GL_PITCH];
     const unsigned nxt = (g.X + GL_Q * 64 - 1) / (GL_Q * 64);
     const unsigned wave = threadIdx.x / 64;
     const unsigned row = __builtin_amdgcn_readfirstlane((blockIdx.x / nxt) * (WG / 64) + wave);
@@ -482,24 +631,37 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
     const double Yc = (double)((float)y - (float)t0.oy);
     const double Zc = (double)((float)z - (float)t0.oz);
     const int X = g.X, Y = g.Y;
-    // ValCoef's v with b1 X = 0
-    const double v = __dadd_rn(__dadd_rn(__dadd_rn(t0.b[0], __dmul_rn(t0.b[2], Yc)), __dmul_rn(t0.b[3], Zc)), t0.oy);
-    const int r0 = min(max((int)v, 0), Y - 2);
-    const float dy = (float)(v - (double)r0);
-    const float wy0 = hat(dy), wy1 = hat(dy - 1.f), wys = wy0 + wy1;
-    const unsigned o0 = (unsigned)r0 * (unsigned)X, o1 = o0 + (unsigned)X;
+    // ValCoef's v, n1 = ((b0 + b1 X) + b2 Y) + b3 Z and v = n1 + oy; its
+    // double rounding is monotone in X, so the wave's least row comes from
+    // its first or its last voxel
+    auto vof = [&](double Xc) {
+        return __dadd_rn(
+            __dadd_rn(__dadd_rn(__dadd_rn(t0.b[0], __dmul_rn(t0.b[1], Xc)), __dmul_rn(t0.b[2], Yc)),
+                      __dmul_rn(t0.b[3], Zc)),
+            t0.oy);
+    };
     const double X0 = (double)((float)xb0 - (float)t0.ox);                  // X of the wave's first voxel
     const double X1 = (double)((float)(xb0 + GL_Q * 64 - 1) - (float)t0.ox);  // and of its last
-    // SIRT sums, per voxel, the column-weighted taps of each detector row and
-    // the column weights over the tilts; the row weights, the same in every
-    // tilt, are applied once at the end
+    const int rlo = min(max((int)fmin(vof(X0), vof(X1)), 0), Y - 2);
+    unsigned orow[NR];  // staged rows rlo .. rlo + NR - 1 (clamped into the projection)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) orow[k] = (unsigned)min(rlo + k, Y - 1) * (unsigned)X;
+    // per voxel: the staged offset of row r0 and the weights of r0, r0 + 1
+    // (NR = 2: the same for every voxel)
     double Xc[GL_Q];
-    float h0[GL_Q], h1[GL_Q], hw[GL_Q], out[GL_Q];
+    float h0[GL_Q], h1[GL_Q], hw[GL_Q], out[GL_Q], wy0[GL_Q], wy1[GL_Q];
+    int ro[GL_Q];
     float *vout = g.vol + (size_t)row * g.X;
 #pragma unroll
     for (int q = 0; q < GL_Q; ++q) {
         const int x = xb0 + lane + 64 * q;
         Xc[q] = (double)((float)x - (float)t0.ox);
+        const double v = NR == 2 ? vof(X0) : vof(Xc[q]);
+        const int r0 = min(max((int)v, 0), Y - 2);
+        const float dy = (float)(v - (double)r0);
+        wy0[q] = hat(dy);
+        wy1[q] = hat(dy - 1.f);
+        ro[q] = NR == 2 ? 0 : min(max(r0 - rlo, 0), NR - 2) * GL_PITCH;
         h0[q] = 0.f;
         h1[q] = 0.f;
         hw[q] = 0.f;
@@ -512,7 +674,7 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
     // they lie between them.
     constexpr int GM = (GL_W + 63) / 64;
     static_assert(64 * (GM - 1) <= GL_W && GL_W <= GL_PITCH, "staged columns fit a row, full chunks inside");
-    float s0[GM], s1[GM];
+    float sv[NR][GM];
     int cnext = 0;
     double ubnext = 0.0, a1next = 0.0;
     auto load = [&](int th) {
@@ -524,18 +686,18 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
         const float *im = img + (size_t)th * Y * X;
         if (cnext + 64 * GM <= X) {
             // the window lies inside the row: one lane offset from the row bases
-            const float *p0 = im + o0 + cnext + lane, *p1 = im + o1 + cnext + lane;
 #pragma unroll
-            for (int m = 0; m < GM; ++m) {
-                s0[m] = p0[64 * m];
-                s1[m] = p1[64 * m];
+            for (int k = 0; k < NR; ++k) {
+                const float *pk = im + orow[k] + cnext + lane;
+#pragma unroll
+                for (int m = 0; m < GM; ++m) sv[k][m] = pk[64 * m];
             }
         } else {
 #pragma unroll
             for (int m = 0; m < GM; ++m) {
                 const unsigned col = (unsigned)min(cnext + lane + 64 * m, X - 1);
-                s0[m] = im[o0 + col];
-                s1[m] = im[o1 + col];
+#pragma unroll
+                for (int k = 0; k < NR; ++k) sv[k][m] = im[orow[k] + col];
             }
         }
     };
@@ -547,8 +709,8 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
 #pragma unroll
         for (int m = 0; m < GM; ++m) {
             if (m < GM - 1 || lane + 64 * m < GL_W) {
-                lds[lane + 64 * m] = s0[m];
-                lds[GL_PITCH + lane + 64 * m] = s1[m];
+#pragma unroll
+                for (int k = 0; k < NR; ++k) lds[k * GL_PITCH + lane + 64 * m] = sv[k][m];
             }
         }
         __builtin_amdgcn_wave_barrier();
@@ -564,14 +726,15 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
             const int li = min(max((int)ur, 0), lhi);
             const float dx = (float)(ur - (double)li);
             const float w0 = hat(dx), w1 = hat(dx - 1.f);
+            const float *l0 = lds + ro[q] + li;
             if (MODE == BACK_SIRT || MODE == BACK_LIN) {
-                h0[q] = fmaf(w1, lds[li + 1], fmaf(w0, lds[li], h0[q]));
-                h1[q] = fmaf(w1, lds[GL_PITCH + li + 1], fmaf(w0, lds[GL_PITCH + li], h1[q]));
+                h0[q] = fmaf(w1, l0[1], fmaf(w0, l0[0], h0[q]));
+                h1[q] = fmaf(w1, l0[GL_PITCH + 1], fmaf(w0, l0[GL_PITCH], h1[q]));
                 hw[q] += w0 + w1;
             } else {
-                const float s = wy0 * fmaf(w1, lds[li + 1], w0 * lds[li]) +
-                                wy1 * fmaf(w1, lds[GL_PITCH + li + 1], w0 * lds[GL_PITCH + li]);
-                const float c = (w0 + w1) * wys;
+                const float s = wy0[q] * fmaf(w1, l0[1], w0 * l0[0]) +
+                                wy1[q] * fmaf(w1, l0[GL_PITCH + 1], w0 * l0[GL_PITCH]);
+                const float c = (w0 + w1) * (wy0[q] + wy1[q]);
                 if (c > g.thr_upd) out[q] += (MODE == BACK_SART) ? (s / c) * g.gamma : s / c;
             }
         }
@@ -581,11 +744,11 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
     for (int q = 0; q < GL_Q; ++q) {
         const int x = xb0 + lane + 64 * q;
         if (MODE == BACK_LIN) {
-            if (x < X) g.out[(size_t)row * g.X + x] = g.lin_a * (wy0 * h0[q] + wy1 * h1[q]) + g.lin_b * out[q];
+            if (x < X) g.out[(size_t)row * g.X + x] = g.lin_a * (wy0[q] * h0[q] + wy1[q] * h1[q]) + g.lin_b * out[q];
             continue;
         }
         if (MODE == BACK_SIRT) {
-            const float val = wy0 * h0[q] + wy1 * h1[q], wt = wys * hw[q];
+            const float val = wy0[q] * h0[q] + wy1[q] * h1[q], wt = (wy0[q] + wy1[q]) * hw[q];
             if (wt > g.thr_upd) out[q] += val / wt * g.gamma;
         }
         if (x < X) vout[x] = out[q];
@@ -622,15 +785,70 @@ void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, int 
     }
 }
 
+template <int NR>
+void launch_back_lds(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img, dim3 grid,
+                     hipStream_t st)
+{
+    if (mode == BACK_SIRT)
+        hipLaunchKernelGGL((k_general_back_lds<BACK_SIRT, NR>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+    else if (mode == BACK_LIN)
+        hipLaunchKernelGGL((k_general_back_lds<BACK_LIN, NR>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+    else if (mode == BACK_SART)
+        hipLaunchKernelGGL((k_general_back_lds<BACK_SART, NR>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+    else
+        hipLaunchKernelGGL((k_general_back_lds<BACK_BPT, NR>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+}
+
 }  // namespace
+
+bool general_tiles(TiltGeneral *tg, int P, int X)
+{
+    bool all = X >= 2;
+    int tile = 0;
+    for (int i = 0; i < P; ++i) {
+        TiltGeneral &t = tg[i];
+        const double a1 = t.a[1], b2 = t.b[2];
+        t.si = t.ncp = t.ntx = 0;
+        t.tile0 = tile;
+        if (!(a1 >= 1.0 / 3.0 && b2 >= 0.55 && b2 <= 1.0) || X < 2) {
+            all = false;
+            continue;
+        }
+        // a tile's box: at most (SR + 1 + |b1| X) / b2 + 2 rows, and
+        // (si + 1 + |a2| rows) / a1 + 2 columns, which should fit 64 lanes
+        // (a wider box takes a second pass)
+        const double rows = (SP_R + 1 + std::fabs(t.b[1]) * X) / b2 + 4.0;
+        const int si = std::min(SP_SIMAX, (int)std::floor(a1 * 62.0 - 1.0 - std::fabs(t.a[2]) * rows));
+        if (si < 8) {
+            all = false;
+            continue;
+        }
+        int ncp = 1;
+        while (a1 * ncp < 1.0 + 1e-4) ++ncp;
+        t.si = si;
+        t.ncp = ncp;
+        t.ntx = (X + si - 1) / si;
+        tile += t.ntx;
+    }
+    return all;
+}
 
 hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
                                   hipStream_t st, int *nzc_used)
 {
     if (nzc_used) *nzc_used = 1;
+    // TREC_GENERAL_FORWARD=rows|splat|gather puts that kernel first where it
+    // applies (tests and timing).  By default: the row windows when v does
+    // not depend on x, else the splat, else the per-pixel gather.
+    const char *force = getenv("TREC_GENERAL_FORWARD");
+    const bool no_rows = force && !strcmp(force, "gather");
+    const bool no_splat = force && (!strcmp(force, "gather") || !strcmp(force, "rows"));
+    // a single-tilt value projection whose caller takes partials splits the
+    // layers into chunks, for GZ_CHUNKS times the waves
+    const int nzc = nzc_used && g.part && ntheta == 1 && !ones ? std::min(GZ_CHUNKS, g.T) : 1;
     // row windows: v independent of x and the same in every tilt launched,
     // 7/8 <= b2 <= 1, at most 6 candidate columns (|a1| >= 1/3)
-    bool rows = ntheta > 0;
+    bool rows = ntheta > 0 && !no_rows;
     int nc = 2;
     double b2min = 1.0;
     for (int th = theta0; th < theta0 + ntheta && rows; ++th) {
@@ -649,10 +867,25 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     if (ntheta <= 4 && b2min >= 3.0 / 4.0) rj = 4;
     rows = rows && rj > 0 && g.X >= nc;
     const size_t nib = cdiv((size_t)g.X, WG);
+    // the splat: every tilt launched has its tiles
+    bool splat = ntheta > 0 && !no_splat && (!rows || (force && !strcmp(force, "splat")));
+    for (int th = theta0; th < theta0 + ntheta && splat; ++th) splat = host_tg[th].si > 0;
+    if (splat) {
+        const TiltGeneral &tl = host_tg[theta0 + ntheta - 1];
+        const unsigned ntile = (unsigned)(tl.tile0 + tl.ntx - host_tg[theta0].tile0);
+        const size_t nblk = (size_t)ntile * cdiv((size_t)g.Y, SP_R) * (size_t)nzc;
+        if (nblk <= 0x7fffffffu) {
+            if (ones)
+                hipLaunchKernelGGL((k_general_splat<SP_R, true>), dim3((unsigned)nblk), dim3(64), 0, st, g, theta0,
+                                   ntheta, nzc, ntile);
+            else
+                hipLaunchKernelGGL((k_general_splat<SP_R, false>), dim3((unsigned)nblk), dim3(64), 0, st, g, theta0,
+                                   ntheta, nzc, ntile);
+            if (nzc_used) *nzc_used = nzc;
+            return hipGetLastError();
+        }
+    }
     if (rows) {
-        // a single-tilt value projection whose caller takes partials splits
-        // the layers into chunks, for GZ_CHUNKS times the waves
-        const int nzc = nzc_used && g.part && ntheta == 1 && !ones ? std::min(GZ_CHUNKS, g.T) : 1;
         const size_t nblk = nib * cdiv((size_t)g.Y, rj) * (size_t)ntheta * (size_t)nzc;
         if (nblk <= 0x7fffffffu) {
             if (rj == 16) launch_rows_nc<16>(g, theta0, ntheta, ones, nc, nzc, (unsigned)nblk, st);
@@ -686,19 +919,21 @@ hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, 
     for (int th = theta0; th < theta0 + ntheta && rows; ++th) rows = same_rows(host_tg[th], host_tg[theta0]);
     const size_t nrows_wg = cdiv((size_t)g.Y * g.T, WG / 64);
     const size_t nblk = cdiv((size_t)g.X, GB_Q * 64) * nrows_wg;
-    bool flat = rows;  // v independent of x: the staged kernel
-    for (int th = theta0; th < theta0 + ntheta && flat; ++th) flat = host_tg[th].b[1] == 0.0;
+    // staged rows: 2 when v does not depend on x, else enough for the rows
+    // a wave's voxels reach (|b1| per voxel over 64 GL_Q - 1 voxels)
+    int nr = 0;
+    if (rows) {
+        const double span = std::fabs(host_tg[theta0].b[1]) * (64.0 * GL_Q - 1.0);
+        nr = host_tg[theta0].b[1] == 0.0 ? 2 : (span < 0.999 ? 3 : (span < 1.999 ? 4 : 0));
+    }
     const size_t nblk_lds = cdiv((size_t)g.X, GL_Q * 64) * nrows_wg;
-    if (flat && nblk_lds <= 0x7fffffffu) {
+    if (nr && nblk_lds <= 0x7fffffffu) {
         const dim3 grid((unsigned)nblk_lds);
-        if (mode == BACK_SIRT)
-            hipLaunchKernelGGL(k_general_back_lds<BACK_SIRT>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
-        else if (mode == BACK_LIN)
-            hipLaunchKernelGGL(k_general_back_lds<BACK_LIN>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
-        else if (mode == BACK_SART)
-            hipLaunchKernelGGL(k_general_back_lds<BACK_SART>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
-        else
-            hipLaunchKernelGGL(k_general_back_lds<BACK_BPT>, grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        switch (nr) {
+        case 2: launch_back_lds<2>(g, mode, theta0, ntheta, img, grid, st); break;
+        case 3: launch_back_lds<3>(g, mode, theta0, ntheta, img, grid, st); break;
+        default: launch_back_lds<4>(g, mode, theta0, ntheta, img, grid, st); break;
+        }
         return hipGetLastError();
     }
     if (rows && nblk <= 0x7fffffffu) {

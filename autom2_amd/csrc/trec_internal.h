@@ -77,7 +77,16 @@ struct TiltGeneral {
     double a[4];
     double b[4];
     double ox, oy, oz;
+    // tiles of the splat forward projection (k_general_splat), set by
+    // general_tiles: si detector columns per tile, ncp lane phases whose
+    // voxels are at least one pixel apart, ntx tiles across the detector,
+    // tile0 the tiles of the tilts before this one (0 for every tilt the
+    // splat cannot take)
+    int si, ncp, ntx, tile0;
 };
+// Fills the splat tiles of P tilts (host); false when some tilt needs
+// another forward kernel (|a1| < 1/3, b2 < 0.55 or a narrow tile).
+bool general_tiles(TiltGeneral *tg, int P, int X);
 
 enum PathKind { PATH_ROWS = 0, PATH_GENERAL = 1 };
 

@@ -447,3 +447,56 @@ def test_back_shapes_equal(torch_cuda, trec, orc, monkeypatch, storage, shapes):
         np.testing.assert_array_equal(v, vols[0], err_msg=f"shape {g},{zt}")
     if storage == "f32":
         assert report("SIRT x3, tuned shapes", vols[0], orc.sirt_f32(coef, stack, T, org, 3, 1.0)) < TOL_F32
+
+
+# The general forward kernels one by one (TREC_GENERAL_FORWARD forces one
+# where it applies): the splat (any pitch and offset with a1 >= 1/3), the
+# row windows (offset 0) and the per-pixel gather (everything else), values
+# and ray weights against Reproject at 1e-6, each twice with equal bits.
+@pytest.mark.parametrize("kind", ["splat", "rows", "gather"])
+@pytest.mark.parametrize("pitch,offset,span,shape", [
+    (0.5, 0.0, 60.0, (7, 37, 10, 300)), (12.0, 0.0, 66.0, (7, 37, 10, 300)), (3.0, 2.0, 60.0, (7, 37, 12, 300)),
+    (6.0, -4.0, 70.0, (5, 40, 9, 130)), (35.0, 5.0, 60.0, (5, 21, 9, 70)), (4.0, 1.0, 60.0, (5, 9, 40, 20))])
+def test_general_forward_kernels(torch_cuda, trec, ctx, orc, monkeypatch, kind, pitch, offset, span, shape):
+    monkeypatch.setenv("TREC_GENERAL_FORWARD", kind)
+    P, Y, T, X = shape
+    ang = tilts(P, span)
+    coef = orc.coeffs_mpi(ang, offset, pitch, 0.0)
+    org = orc.origin(X, T, T / 2)
+    vol = smooth_volume(Y, T, X, 17)
+    g = trec.make_geom(X, Y, P, T, offset, pitch, 0.0)
+    ref = [orc.reproject_f32(coef[k], vol, org) for k in range(P)]
+    outs = []
+    for _ in range(2):
+        rv = torch_cuda.empty((P, Y, X), dtype=torch_cuda.float32, device="cuda")
+        rw = torch_cuda.empty_like(rv)
+        ctx.forward_project(g, ang, dev(torch_cuda, vol), rv, rw)
+        torch_cuda.cuda.synchronize()
+        outs.append((rv.cpu().numpy(), rw.cpu().numpy()))
+    tag = f"{kind} pitch {pitch} offset {offset} span {span} {shape}"
+    assert report(f"forward value, {tag}", outs[0][0], np.stack([r[0] for r in ref])) < 1e-6
+    assert report(f"forward weight, {tag}", outs[0][1], np.stack([r[1] for r in ref])) < 1e-6
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+# A pitch with an offset (b1 != 0): v moves along x, so the back-projection
+# stages 3 or 4 detector rows per wave (|b1| up to 2/511), or reads its taps
+# from memory beyond.  X = 600 spans two waves of 512 voxels per row.
+@pytest.mark.parametrize("method", ["SIRT,3,1.0", "SART,1,0.8", "BPT"])
+@pytest.mark.parametrize("pitch,offset", [(3.0, 2.0), (4.0, 3.0), (6.0, 3.0)])
+def test_pitch_offset_paths(torch_cuda, trec, ctx, orc, method, pitch, offset):
+    P, Y, T, X = 7, 21, 10, 600
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=41, pitch=pitch, offset=offset)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset)
+    name = method.split(",")[0]
+    if name == "BPT":
+        ref = orc.bpt_f32(coef, stack, T, org)
+    elif name == "SIRT":
+        ref = orc.sirt_f32(coef, stack, T, org, 3, 1.0)
+    else:
+        ref = orc.sart_f32(coef, stack, T, org, 1, 0.8)
+    b1 = abs(coef[0, 5])
+    assert report(f"pitch {pitch} offset {offset} (b1 span {511 * b1:.2f} rows) {method}", got, ref) < TOL_F32
+    again = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset)
+    np.testing.assert_array_equal(again, got)
