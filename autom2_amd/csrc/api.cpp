@@ -520,7 +520,17 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             t.oy = oy;
             t.oz = oz;
         }
-        general_tiles(p.tg.data(), g.nproj, g.nx);
+        // SIRT's splat forward (a pitch with an offset) sums chunks of about
+        // 32 layers into partials, so that the waves in flight on one XCD
+        // share their volume rows in its L2; the partials take at most 4 GB
+        q.nzc_all = 1;
+        bool flat = true;
+        for (const TiltGeneral &t : p.tg) flat = flat && t.b[1] == 0.0;
+        if (general_tiles(p.tg.data(), g.nproj, g.nx) && !flat && m.method == TREC_SIRT) {
+            const size_t one = 4 * (size_t)g.nproj * g.ny * g.nx;
+            q.nzc_all = (int)std::min<size_t>({8, (size_t)(g.thickness + 31) / 32, ((size_t)4 << 30) / one});
+            if (q.nzc_all < 2) q.nzc_all = 1;
+        }
         filter_taps(p, m.method, g.nx);
     }
     return TREC_OK;
@@ -580,7 +590,9 @@ size_t carve(JobPlan &p, char *base)
         q.rval = uses_forward(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
         q.rwt = uses_forward(k) || p.want_rwt ? (float *)take(4 * P * Y * X) : nullptr;
         p.stack_copy = is_filtered(k) ? (float *)take(4 * P * Y * X) : nullptr;
-        q.part = k == TREC_SART ? (float *)take(4 * (size_t)GZ_CHUNKS * Y * X) : nullptr;
+        const size_t npart = std::max(k == TREC_SART ? (size_t)GZ_CHUNKS * Y * X : 0,
+                                      q.nzc_all > 1 ? (size_t)q.nzc_all * P * Y * X : 0);
+        q.part = npart ? (float *)take(4 * npart) : nullptr;
         if (k == TREC_ADMM) {
             // the CG and ADMM vectors, each a volume in the job's layout
             AdmmVecs &a = p.av;
@@ -838,8 +850,9 @@ hipError_t issue_general_stage(JobPlan &p, int s, hipStream_t st)
         {
             ProfScope ps(PROF_FORWARD, st);
             LCHK(launch_zero(q.rval, P * Y * X, st));
-            LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, false, st));
-            LCHK(launch_general_residual(q, 0, (int)P, false, st));
+            int nzc = 1;
+            LCHK(launch_general_forward(q, p.tg.data(), 0, (int)P, false, st, &nzc));
+            LCHK(launch_general_residual(q, 0, (int)P, false, st, nzc));
         }
         ProfScope ps(PROF_BACK, st);
         return launch_general_back(q, p.tg.data(), BACK_SIRT, 0, (int)P, q.rval, st);

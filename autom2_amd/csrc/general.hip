@@ -42,6 +42,16 @@ static inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) 
 
 __device__ __forceinline__ float hat(float d) { return fmaxf(0.f, 1.f - fabsf(d)); }
 
+// Work-group b of n as dispatched (round robin over the 8 XCDs) -> its
+// place in the work list, so that each XCD walks one contiguous range of it
+// and the work-groups in flight on one XCD share their volume rows in its L2.
+__device__ __forceinline__ unsigned xcd_order(unsigned b, unsigned n)
+{
+    const unsigned q = n >> 3, r = n & 7u;
+    const unsigned xcd = b & 7u, k = b >> 3;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
 // ------------------------------------------------------------------ reference order
 
 // Exact ValCoef for any pitch (main-mpi.cpp:204-226).
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
     constexpr int NH = (NE + 63) / 64;     // lane registers holding them
     constexpr int SB = 5;                  // slots whose loads may be in flight together
     const unsigned nib = (g.X + WG - 1) / WG;
-    const unsigned lin = blockIdx.x;
+    const unsigned lin = xcd_order(blockIdx.x, gridDim.x);
     const int th = theta0 + (int)(lin % (unsigned)ntheta);
     const unsigned rest = lin / (unsigned)ntheta;
     // every lane stays to the end: the wave's lanes hold the y weights of all
@@ -407,7 +417,8 @@ __global__ __launch_bounds__(64) void k_general_splat(const GeneralJob g, int th
 {
     constexpr int SP_ROWS = SR + 2 * SP_RP;
     __shared__ float acc[SP_ROWS * SP_PITCH];
-    const unsigned tw = blockIdx.x % ntile, rest = blockIdx.x / ntile;
+    const unsigned w = xcd_order(blockIdx.x, gridDim.x);
+    const unsigned tw = w % ntile, rest = w / ntile;
     const unsigned nrb = (g.Y + SR - 1) / SR;
     const int rb = (int)(rest % nrb), zc = (int)(rest / nrb);
     // the tilt of tile tw: the last whose first tile is at most tw
@@ -844,8 +855,10 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     const bool no_rows = force && !strcmp(force, "gather");
     const bool no_splat = force && (!strcmp(force, "gather") || !strcmp(force, "rows"));
     // a single-tilt value projection whose caller takes partials splits the
-    // layers into chunks, for GZ_CHUNKS times the waves
+    // layers into chunks, for GZ_CHUNKS times the waves; the splat of all
+    // tilts takes g.nzc_all chunks, for L2 locality
     const int nzc = nzc_used && g.part && ntheta == 1 && !ones ? std::min(GZ_CHUNKS, g.T) : 1;
+    const int nzc_splat = nzc_used && g.part && !ones && ntheta == g.P && g.nzc_all > 1 ? g.nzc_all : nzc;
     // row windows: v independent of x and the same in every tilt launched,
     // 7/8 <= b2 <= 1, at most 6 candidate columns (|a1| >= 1/3)
     bool rows = ntheta > 0 && !no_rows;
@@ -873,15 +886,15 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     if (splat) {
         const TiltGeneral &tl = host_tg[theta0 + ntheta - 1];
         const unsigned ntile = (unsigned)(tl.tile0 + tl.ntx - host_tg[theta0].tile0);
-        const size_t nblk = (size_t)ntile * cdiv((size_t)g.Y, SP_R) * (size_t)nzc;
+        const size_t nblk = (size_t)ntile * cdiv((size_t)g.Y, SP_R) * (size_t)nzc_splat;
         if (nblk <= 0x7fffffffu) {
             if (ones)
                 hipLaunchKernelGGL((k_general_splat<SP_R, true>), dim3((unsigned)nblk), dim3(64), 0, st, g, theta0,
-                                   ntheta, nzc, ntile);
+                                   ntheta, nzc_splat, ntile);
             else
                 hipLaunchKernelGGL((k_general_splat<SP_R, false>), dim3((unsigned)nblk), dim3(64), 0, st, g, theta0,
-                                   ntheta, nzc, ntile);
-            if (nzc_used) *nzc_used = nzc;
+                                   ntheta, nzc_splat, ntile);
+            if (nzc_used) *nzc_used = nzc_splat;
             return hipGetLastError();
         }
     }

@@ -197,7 +197,9 @@ struct GeneralJob {
     // BACK_LIN (ADMM): out = lin_a * sum_theta s_theta(img) + lin_b * vol
     float *out;
     float lin_a, lin_b;
-    float *part;  // [GZ_CHUNKS][Y][X] layer-chunk partials of a single-tilt forward (SART), or null
+    float *part;  // layer-chunk partials: [GZ_CHUNKS][Y][X] of a single-tilt forward (SART), or
+                  // [nzc_all][P][Y][X] of an all-tilt splat forward (SIRT); or null
+    int nzc_all;  // layer chunks of an all-tilt splat forward into part (1: none)
 };
 constexpr int GZ_CHUNKS = 4;
 // host_tg: the same tilt table on the host, for the choice of kernel
