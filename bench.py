@@ -498,8 +498,10 @@ def profiled(kernel, column, summary="sirt1024_f32_counters.md", prefix=False):
     return None
 
 
-# MI355X: 256 CUs, each issuing one 64-lane VALU instruction per cycle, at
-# 2.4 GHz.
+# MI355X: 256 CUs at 2.4 GHz, each issuing one 64-lane FP32 multiply-add per
+# cycle without packed instructions (half the published 157.3 TFLOP/s, which
+# counts packed FP32; this build emits none).  Mixed instruction streams can
+# issue somewhat faster, so the fraction is a rate, not a hard ceiling.
 VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
 
 
@@ -539,7 +541,7 @@ def kernel_roofline(kern, plan, X, Y, T, P, storage_bytes=4, summary=None):
         if valu is not None:
             out["valu"] = {"insts_per_launch": valu,
                            "frac": round(valu / (ms / 1e3) / VALU_PEAK_WAVE_INSTS, 4),
-                           "peak": "256 CUs x 1 wave64 instruction per cycle x 2.4 GHz"}
+                           "peak": "256 CUs x 1 wave64 FP32 multiply-add per cycle x 2.4 GHz (non-packed FP32 rate)"}
     return out
 
 
