@@ -278,11 +278,12 @@ __global__ __launch_bounds__(WG) void k_general_back(const GeneralJob g, int mod
 // with |u - i| < 1 (clamped into the row, so every load is in bounds; a
 // column that is no candidate gets hat = 0).  The y weights hat(v - j0 - r)
 // are the same in every lane: per layer the wave evaluates the 5 (RJ + 4)
-// slot/row pairs once, one per lane, and each slot reads its five from the
-// lanes that hold them (zero for rows it does not reach and for slots
-// outside the volume).  Sums run over z, then slots in y order, then columns
-// in x order.  Work-groups are ordered tilt first, so those in flight
-// together read the same rows of the volume.
+// slot/row pairs once, one per lane, stores them in its LDS, and each slot
+// reads its five from there as uniform operands (zero for rows it does not
+// reach and for slots outside the volume).  Sums run over z, then slots in y
+// order, then columns in x order.  Work-groups are ordered tilt first, so
+// those in flight together read the same rows of the volume, and each XCD
+// walks a contiguous range of that order.
 template <int RJ, int NC, bool ONES>
 __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int theta0, int ntheta, int nzc)
 {
@@ -290,6 +291,8 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
     constexpr int NE = 5 * NS;             // slot/row pairs: slot q / 5, row q / 5 - 4 + q % 5
     constexpr int NH = (NE + 63) / 64;     // lane registers holding them
     constexpr int SB = 5;                  // slots whose loads may be in flight together
+    __shared__ float ywl[WG / 64][64 * NH];
+    float *yw = ywl[threadIdx.x / 64];
     const unsigned nib = (g.X + WG - 1) / WG;
     const unsigned lin = xcd_order(blockIdx.x, gridDim.x);
     const int th = theta0 + (int)(lin % (unsigned)ntheta);
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
     // u - i = W + a1 x with W = cu - i + a2 Y - a1 ox; the first candidate is
     // the least x with (W + a1 x - s1) / a1 > 0
     const double s1ra1 = (a1 > 0.0 ? -1.0 : 1.0) * ra1;
-    const float a1f = (float)a1;
+    const float a1f = (float)a1, a2f = (float)a2, rkf = (float)(a2 * ra1);
     const int xmax = g.X - NC;
     // volume strides of a row and a layer
     const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
@@ -347,14 +350,30 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
             const float e = (float)((b2 * (Yc0 + pkd[h]) + cv) - pj[h]);
             pw[h] = prow[h] && (unsigned)(ylo + pk[h]) < (unsigned)g.Y ? hat(e - pm[h]) : 0.f;
         }
+        // the slot/row weights go to the wave's LDS, from where every slot
+        // reads its five as uniform operands (broadcast reads, no VALU)
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int h = 0; h < NH; ++h) yw[64 * h + lane] = pw[h];
+        __builtin_amdgcn_wave_barrier();
         const double W0 = cu - (double)ic + a2 * Yc0 - a1 * ox;
+        // slot k's first candidate is floor(F0 - rk k) + 1 with F0 = (s1 -
+        // W0) / a1, and u - i at column x is E0 + a1 (x - xf0) + a2 k with
+        // E0 = a1 xf0 + W0 (|E0| <= 1 + |a1|): both in float from the layer's
+        // double F0 and E0.  Where float rounding moves a slot's start by a
+        // column, the column gained or lost lies within rounding of
+        // |u - i| = 1, where its weight is of that order.
+        const double F0 = s1ra1 - W0 * ra1;
+        const double xf0d = floor(F0);
+        const int xf0 = (int)xf0d;
+        const float f0 = (float)(F0 - xf0d);
+        const float E0 = (float)(a1 * xf0d + W0);
         const float *vlayer = g.vol + (size_t)z * sz;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const double W = a2 * (double)k + W0;
-            const int xf = (int)floor(s1ra1 - W * ra1) + 1;
+            const int xf = xf0 + (int)floorf(fmaf(-rkf, (float)k, f0)) + 1;
             const int xs = min(max(xf, 0), xmax);
-            const float d0 = (float)(a1 * (double)xs + W);  // u - i at column xs
+            const float d0 = fmaf(a1f, (float)(xs - xf0), fmaf(a2f, (float)k, E0));  // u - i at column xs
             float s = 0.f;
             if (ONES) {
 #pragma unroll
@@ -371,8 +390,7 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
 #pragma unroll
             for (int r = (k > 4 ? k - 4 : 0); r <= (k < RJ - 1 ? k : RJ - 1); ++r) {
                 const int q = 5 * k + r - k + 4;
-                const float w = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pw[q / 64]), q % 64));
-                acc[r] = fmaf(w, s, acc[r]);
+                acc[r] = fmaf(yw[q], s, acc[r]);
             }
             if ((k + 1) % SB == 0) __builtin_amdgcn_sched_barrier(0);
         }
