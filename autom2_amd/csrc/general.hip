@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "trec_internal.h"
 
@@ -641,6 +642,7 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
 // column-weighted taps over the tilts and apply the row weights, the same in
 // every tilt, once per voxel.
 constexpr int GL_Q = 8;        // voxels per lane, 64 apart
+constexpr int GL_BUF_DWORD3 = 0x00020000;  // raw buffer resource, gfx9 family
 constexpr int GL_W = 516;      // staged columns per row
 constexpr int GL_PITCH = 520;  // floats between two staged rows
 template <int MODE, int NR>
@@ -698,9 +700,7 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
     }
     float *lds = stage[wave];
     // window of tilt th: u = ub + a1 X, its first column, and the staged
-    // values of columns cmin + lane + 64 m.  The voxels' u below come from
-    // the same ub and the same fused multiply-add as the window's ends, so
-    // they lie between them.
+    // values of columns cmin + lane + 64 m.
     constexpr int GM = (GL_W + 63) / 64;
     static_assert(64 * (GM - 1) <= GL_W && GL_W <= GL_PITCH, "staged columns fit a row, full chunks inside");
     float sv[NR][GM];
@@ -712,22 +712,17 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
         a1next = t.a[1];
         const double u0 = __fma_rn(a1next, X0, ubnext), u1 = __fma_rn(a1next, X1, ubnext);
         cnext = min(max((int)fmin(u0, u1), 0), X - 2);
-        const float *im = img + (size_t)th * Y * X;
-        if (cnext + 64 * GM <= X) {
-            // the window lies inside the row: one lane offset from the row bases
+        // buffer loads: the row base as the scalar offset, the lane and chunk
+        // in the instruction; columns past the row (never used: taps stop at
+        // X - 1) read the next row, or 0 past the projection
+        const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)(img + (size_t)th * Y * X), (short)0,
+                                                          (int)((unsigned)Y * (unsigned)X * 4u), GL_BUF_DWORD3);
 #pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                const float *pk = im + orow[k] + cnext + lane;
+        for (int k = 0; k < NR; ++k) {
+            const int soff = (int)((orow[k] + (unsigned)cnext) * 4u);
 #pragma unroll
-                for (int m = 0; m < GM; ++m) sv[k][m] = pk[64 * m];
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < GM; ++m) {
-                const unsigned col = (unsigned)min(cnext + lane + 64 * m, X - 1);
-#pragma unroll
-                for (int k = 0; k < NR; ++k) sv[k][m] = im[orow[k] + col];
-            }
+            for (int m = 0; m < GM; ++m)
+                sv[k][m] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4 + 256 * m, soff, 0));
         }
     };
     load(theta0);
@@ -749,24 +744,42 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
         // (the bound GL_W - 2 only guards it)
         const double ubr = ub - (double)cmin;
         const int lhi = min(X - 2 - cmin, GL_W - 2);
+        // u - cmin of voxel q is U0 + q A, A = 64 a1: in float from the
+        // integer and fractional parts of U0 and A (|f0 + q fA| < 8, so the
+        // fraction is good to about 5e-7).  Inside the detector (every voxel
+        // of the wave at 1 <= u <= X - 3) the tap is floor(u); at its edges
+        // it is clamped into [0, X - 2] and the fraction measured from there.
+        const double U0 = __fma_rn(a1, Xc[0], ubr), A = 64.0 * a1;
+        const double fU0 = floor(U0), fA_ = floor(A);
+        const int I0 = (int)fU0, IA = (int)fA_;
+        const float f0 = (float)(U0 - fU0), fA = (float)(A - fA_);
+        const double uext = __fma_rn(a1, X1, ub), uext0 = __fma_rn(a1, X0, ub);
+        const bool inside = fmin(uext0, uext) >= 1.0 && fmax(uext0, uext) <= (double)(X - 3);
+        auto voxels = [&](auto edge_tag) {
+            constexpr bool EDGE = decltype(edge_tag)::value;
 #pragma unroll
-        for (int q = 0; q < GL_Q; ++q) {
-            const double ur = __fma_rn(a1, Xc[q], ubr);
-            const int li = min(max((int)ur, 0), lhi);
-            const float dx = (float)(ur - (double)li);
-            const float w0 = hat(dx), w1 = hat(dx - 1.f);
-            const float *l0 = lds + ro[q] + li;
-            if (MODE == BACK_SIRT || MODE == BACK_LIN) {
-                h0[q] = fmaf(w1, l0[1], fmaf(w0, l0[0], h0[q]));
-                h1[q] = fmaf(w1, l0[GL_PITCH + 1], fmaf(w0, l0[GL_PITCH], h1[q]));
-                hw[q] += w0 + w1;
-            } else {
-                const float s = wy0[q] * fmaf(w1, l0[1], w0 * l0[0]) +
-                                wy1[q] * fmaf(w1, l0[GL_PITCH + 1], w0 * l0[GL_PITCH]);
-                const float c = (w0 + w1) * (wy0[q] + wy1[q]);
-                if (c > g.thr_upd) out[q] += (MODE == BACK_SART) ? (s / c) * g.gamma : s / c;
+            for (int q = 0; q < GL_Q; ++q) {
+                const float tq = fmaf((float)q, fA, f0);
+                const float fl = floorf(tq);
+                const int base = I0 + q * IA;
+                const int li = min(max(base + (int)fl, 0), lhi);
+                const float dx = EDGE ? tq - (float)(li - base) : tq - fl;
+                const float w0 = hat(dx), w1 = hat(dx - 1.f);
+                const float *l0 = lds + ro[q] + li;
+                if (MODE == BACK_SIRT || MODE == BACK_LIN) {
+                    h0[q] = fmaf(w1, l0[1], fmaf(w0, l0[0], h0[q]));
+                    h1[q] = fmaf(w1, l0[GL_PITCH + 1], fmaf(w0, l0[GL_PITCH], h1[q]));
+                    hw[q] += w0 + w1;
+                } else {
+                    const float s = wy0[q] * fmaf(w1, l0[1], w0 * l0[0]) +
+                                    wy1[q] * fmaf(w1, l0[GL_PITCH + 1], w0 * l0[GL_PITCH]);
+                    const float c = (w0 + w1) * (wy0[q] + wy1[q]);
+                    if (c > g.thr_upd) out[q] += (MODE == BACK_SART) ? (s / c) * g.gamma : s / c;
+                }
             }
-        }
+        };
+        if (inside) voxels(std::false_type{});
+        else voxels(std::true_type{});
         __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
