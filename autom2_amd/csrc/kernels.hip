@@ -669,10 +669,11 @@ __global__ __launch_bounds__(FW_PIX *FG) __attribute__((amdgpu_waves_per_eu(S <=
                     if ((unsigned)(x0 + m * (FG * FW_G) + E0) >= (unsigned)(X + E0))
                         pre[m] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
+            // one lane address, the chunks at constant offsets from it
+            float4 *bp = &band[lq][wv * FW_G + lc];
 #pragma unroll
             for (int m = 0; m < NLW; ++m) {
-                const int c = wv + FG * m;
-                if (c < nch) band[lq][c * FW_G + lc] = pre[m];
+                if (wv + FG * m < nch) bp[m * FG * FW_G] = pre[m];
             }
             // Every path now holds all of pre[] (a load skipped above was
             // skipped when fetching too): saying so keeps the next fetch from
