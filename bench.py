@@ -262,7 +262,7 @@ def serve(trec, reqs, arrivals, storage, max_active, half_io=False):
     return lat, span
 
 
-def serving_form(trec, torch, n, rate, seed=1004):
+def serving_form(trec, torch, n, rate, seed=1004, active=4, saturated_only=False):
     """The serving configuration: n requests of the batching mix, Poisson
     arrivals at `rate` per second and all at once (saturated).  Every arm
     runs through the same server and copy machinery (pinned double-buffered
@@ -272,10 +272,8 @@ def serving_form(trec, torch, n, rate, seed=1004):
       batched_f16_half_io  the same with half volumes out (MRC mode 12) and,
                       for SIRT, half stacks in: half the bytes over PCIe;
       batched_f32     continuous batching, FP32 storage (saturated only).
-    Up to 4 active requests: most requests of the mix fill the GPU alone, so
-    more at once only dilute the L2 (120 saturated requests: FP16 with half
-    transfers gave 5.50, 5.37 and 5.20 series/s at 2, 4 and 16 active, FP32
-    4.70 and 4.86 at 2 and 4, plain FP32 4.22).
+    Up to `active` (4) requests at once: most requests of the mix fill the
+    GPU alone, so more at once only dilute the L2.
     Latency is from submission to the volume in host memory."""
     import numpy as np
     reqs = serving_mix(n, seed)
@@ -285,12 +283,13 @@ def serving_form(trec, torch, n, rate, seed=1004):
            "copies": "every arm through the server's pinned staging; latency includes host copies"}
     stat = lambda lat, span: {"series_per_s": round(n / span, 3), "p50_ms": round(percentile(lat, 0.5), 1),
                               "p99_ms": round(percentile(lat, 0.99), 1)}
-    arms = [("plain_f32", trec.STORE_F32, 1, False), ("batched_f16", trec.STORE_F16, 4, False),
-            ("batched_f16_half_io", trec.STORE_F16, 4, True)]
-    for mode in ("poisson", "saturated"):
+    arms = [("plain_f32", trec.STORE_F32, 1, False), ("batched_f16", trec.STORE_F16, active, False),
+            ("batched_f16_half_io", trec.STORE_F16, active, True)]
+    out["max_active"] = active
+    for mode in (("saturated",) if saturated_only else ("poisson", "saturated")):
         arr = arrivals if mode == "poisson" else [0.0] * n
         res = {}
-        for name, sto, act, half in arms + ([("batched_f32", trec.STORE_F32, 4, False)] if mode == "saturated"
+        for name, sto, act, half in arms + ([("batched_f32", trec.STORE_F32, active, False)] if mode == "saturated"
                                             else []):
             log(f"[bench] serving ({mode}): {name}")
             lat, span = serve(trec, reqs, arr, sto, act, half)
@@ -611,6 +610,8 @@ def main():
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end CLI run with MRC files")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16/BF16 and serving forms")
     ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-active", type=int, default=4, help="most active requests of the batched serving arms")
+    ap.add_argument("--saturated-only", action="store_true", help="serving form: only the saturated load")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving form (its own JSON line), e.g. with --serve-requests 120")
     ap.add_argument("--counters", action="store_true", help="only the run the counter profiles measure")
@@ -653,7 +654,8 @@ def main():
     if args.serving_only:
         del stack_d
         torch.cuda.empty_cache()
-        print(json.dumps({"serving": serving_form(trec, torch, args.serve_requests, 2.0)}), flush=True)
+        print(json.dumps({"serving": serving_form(trec, torch, args.serve_requests, 2.0, active=args.serve_active,
+                                                  saturated_only=args.saturated_only)}), flush=True)
         return
     if args.methods_only:
         print(json.dumps({"methods": methods_form(trec, ctx, torch, geom, ang, stack_d)}), flush=True)
