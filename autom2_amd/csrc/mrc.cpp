@@ -333,6 +333,77 @@ void mrc_set_2014(MrcHeader &h, float pixel)
     h.alpha = h.beta = h.gamma = 90;
 }
 
+size_t mrc_mid_section(const MrcHeader &h) { return (size_t)h.nx * h.ny * (size_t)(int)(h.nz * .5); }
+
+void update_header_mid(MrcHeader &h, const void *section, int mode)
+{
+    // UpdateHeader (mrcstack.cpp:104-142): section nz / 2, amax starting at
+    // FLT_MIN, the mean accumulated in long double
+    const size_t sec = (size_t)h.nx * h.ny;
+    float amin = FLT_MAX, amax = FLT_MIN;
+    long double amean = 0;
+    for (size_t i = sec; i--;) {
+        const float v = elem(section, mode, i);
+        if (v > amax) amax = v;
+        if (v < amin) amin = v;
+        amean += v;
+    }
+    amean /= sec;
+    h.amin = amin;
+    h.amax = amax;
+    h.amean = (float)amean;
+    h.nlabl = 0;
+}
+
+bool MrcStreamWriter::open(const std::string &path, MrcHeader h, int mode, const void *mid_section,
+                           std::string &err)
+{
+    if (mode != MRC_FLOAT && mode != MRC_HALF) {
+        err = "MRC output mode must be 2 or 12";
+        return false;
+    }
+    update_header_mid(h, mid_section, mode);
+    h.mode = mode;
+    h.next = 0;
+    path_ = path;
+    f_ = fopen(path.c_str(), "wb");
+    if (!f_) {
+        err = "cannot create " + path + ": " + strerror(errno);
+        return false;
+    }
+    if (fwrite(&h, sizeof h, 1, f_) != 1) {
+        err = "write failed for " + path;
+        return false;
+    }
+    return true;
+}
+
+bool MrcStreamWriter::write(const void *data, size_t bytes, std::string &err)
+{
+    if (!f_ || fwrite(data, 1, bytes, f_) != bytes) {
+        err = "write failed for " + path_;
+        return false;
+    }
+    return true;
+}
+
+bool MrcStreamWriter::close(std::string &err)
+{
+    if (!f_) return true;
+    const int r = fclose(f_);
+    f_ = nullptr;
+    if (r != 0) {
+        err = "close failed for " + path_;
+        return false;
+    }
+    return true;
+}
+
+MrcStreamWriter::~MrcStreamWriter()
+{
+    if (f_) fclose(f_);
+}
+
 bool mrc_write_volume(const std::string &path, MrcHeader h, const void *data, int mode, bool mrc2014,
                       std::string &err)
 {
@@ -349,22 +420,8 @@ bool mrc_write_volume(const std::string &path, MrcHeader h, const void *data, in
         memset(h.labels, 0, sizeof h.labels);
         snprintf(h.labels[0], sizeof h.labels[0], "%-79s", "autom2_amd: TiltRec reconstruction on MI355X");
     } else {
-        // UpdateHeader (mrcstack.cpp:104-142): section nz / 2, amax starting
-        // at FLT_MIN, the mean accumulated in long double
-        const size_t mid = sec * (size_t)(int)(h.nz * .5);
-        float amin = FLT_MAX, amax = FLT_MIN;
-        long double amean = 0;
-        for (size_t i = sec; i--;) {
-            const float v = elem(data, mode, mid + i);
-            if (v > amax) amax = v;
-            if (v < amin) amin = v;
-            amean += v;
-        }
-        amean /= sec;
-        h.amin = amin;
-        h.amax = amax;
-        h.amean = (float)amean;
-        h.nlabl = 0;
+        const size_t es = mode == MRC_HALF ? 2 : 4;
+        update_header_mid(h, (const unsigned char *)data + es * mrc_mid_section(h), mode);
     }
     h.mode = mode;
     h.next = 0;

@@ -15,6 +15,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -94,5 +95,26 @@ void mrc_set_2014(MrcHeader &h, float pixel);
 // UpdateHeader's, as mrc_write_float.
 bool mrc_write_volume(const std::string &path, MrcHeader h, const void *data, int mode, bool mrc2014,
                       std::string &err);
+
+// Element offset of section nz / 2, whose statistics UpdateHeader puts in
+// the header (mrcstack.cpp:104-142), and that update from the section alone.
+size_t mrc_mid_section(const MrcHeader &h);
+void update_header_mid(MrcHeader &h, const void *section, int mode);
+
+// The same file as mrc_write_volume without mrc2014, written in pieces: the
+// header from section nz / 2 (which the caller fetches first), then the
+// elements in order, so a volume can go out while the rest of it is still
+// being copied from the device.
+class MrcStreamWriter {
+  public:
+    bool open(const std::string &path, MrcHeader h, int mode, const void *mid_section, std::string &err);
+    bool write(const void *data, size_t bytes, std::string &err);
+    bool close(std::string &err);
+    ~MrcStreamWriter();
+
+  private:
+    FILE *f_ = nullptr;
+    std::string path_;
+};
 
 }  // namespace trec_cli

@@ -22,7 +22,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <memory>
+#include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mrc.h"
@@ -79,6 +82,45 @@ struct DevBuf {
     }
 };
 
+// Copies `bytes` of device memory into the file in 64 MiB pieces through two
+// pinned buffers: piece k + 1 comes down while piece k is written.  false
+// with err empty: no pinned memory, the caller copies the volume whole.
+bool stream_out(MrcStreamWriter &w, const unsigned char *src, size_t bytes, std::string &err)
+{
+    constexpr size_t CH = (size_t)64 << 20;
+    unsigned char *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t st = nullptr;
+    bool ok = hipStreamCreate(&st) == hipSuccess;
+    for (int b = 0; b < 2 && ok; ++b)
+        ok = hipHostMalloc((void **)&buf[b], CH, hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&ev[b], hipEventDisableTiming) == hipSuccess;
+    auto issue = [&](size_t off, int b) {
+        return hipMemcpyAsync(buf[b], src + off, std::min(CH, bytes - off), hipMemcpyDeviceToHost, st) == hipSuccess &&
+               hipEventRecord(ev[b], st) == hipSuccess;
+    };
+    bool wrote = false;
+    if (ok && bytes) ok = issue(0, 0);
+    for (size_t off = 0; ok && off < bytes; off += CH) {
+        const int b = (int)((off / CH) & 1);
+        ok = hipEventSynchronize(ev[b]) == hipSuccess;
+        if (ok && off + CH < bytes) ok = issue(off + CH, b ^ 1);
+        if (ok) {
+            wrote = true;
+            if (!w.write(buf[b], std::min(CH, bytes - off), err)) return false;
+        }
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    for (int b = 0; b < 2; ++b) {
+        if (buf[b]) (void)hipHostFree(buf[b]);
+        if (ev[b]) (void)hipEventDestroy(ev[b]);
+    }
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipGetLastError();
+    if (!ok && wrote) err = "copy from device failed";
+    return ok;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -118,6 +160,42 @@ int main(int argc, char **argv)
     MrcHeader hin;
     std::vector<unsigned char> stack;
     std::string err;
+    // The device comes up (runtime, buffers, context) on a thread of its
+    // own while the stack is read from the file.
+    if (!mrc_read_header(o.input, hin, err, nullptr)) return die(4, "%s", err.c_str());
+    const size_t nstack = (size_t)hin.nz * hin.ny * hin.nx, nvol = (size_t)hin.nx * hin.ny * o.thickness;
+    DevBuf dstack, dvol, draw;
+    trec_context *ctx = nullptr;
+    struct CtxGuard {
+        trec_context *&c;
+        ~CtxGuard()
+        {
+            if (c) trec_context_destroy(c);
+        }
+    } ctx_guard{ctx};  // destroyed after the thread below has joined
+    auto drop_ctx = [&] {
+        if (ctx) trec_context_destroy(ctx);
+        ctx = nullptr;
+    };
+    int dev_fail = 0;  // 1: no device, 2: allocation, 3: context
+    std::thread dev_up([&] {
+        if (hipSetDevice(o.device) != hipSuccess) {
+            dev_fail = 1;
+            return;
+        }
+        if (hipMalloc(&dstack.p, 4 * nstack) != hipSuccess || hipMalloc(&dvol.p, 4 * nvol) != hipSuccess) {
+            dev_fail = 2;
+            return;
+        }
+        if (trec_context_create(o.device, &ctx) != TREC_OK) dev_fail = 3;
+    });
+    struct Joiner {
+        std::thread &t;
+        ~Joiner()
+        {
+            if (t.joinable()) t.join();
+        }
+    } joiner{dev_up};
     if (!mrc_read_raw(o.input, hin, stack, err)) return die(4, "%s", err.c_str());
     std::vector<float> angles;
     if (!read_angles(o.angle, angles)) return die(4, "cannot read tilt file %s", o.angle.c_str());
@@ -142,14 +220,12 @@ int main(int argc, char **argv)
     trec_geom g{X, Y, P, T, o.offset, o.pitch, o.zshift, rules};
     const int storage = o.storage == "f16" ? TREC_STORE_F16 : (o.storage == "bf16" ? TREC_STORE_BF16 : TREC_STORE_F32);
     trec_method m{kind, o.iteration, o.gamma, storage, layout, o.cgiter, o.soft};
-    if (hipSetDevice(o.device) != hipSuccess) return die(5, "HIP device %s is not available", std::to_string(o.device).c_str());
-    const size_t nstack = (size_t)P * Y * X, nvol = (size_t)X * Y * T;
-    DevBuf dstack, dvol, draw;
-    if (hipMalloc(&dstack.p, 4 * nstack) != hipSuccess || hipMalloc(&dvol.p, 4 * nvol) != hipSuccess)
-        return die(5, "device allocation failed");
-    trec_context *ctx = nullptr;
-    trec_status s = trec_context_create(o.device, &ctx);
-    if (s != TREC_OK) return die(5, "context: %s", trec_status_string(s));
+    dev_up.join();
+    if (dev_fail == 1 || hipSetDevice(o.device) != hipSuccess)
+        return die(5, "HIP device %s is not available", std::to_string(o.device).c_str());
+    if (dev_fail == 2) return die(5, "device allocation failed");
+    if (dev_fail == 3) return die(5, "context: cannot be created");
+    trec_status s = TREC_OK;
     // the file's own elements go up; the GPU widens them to float
     // (MRC mode 0 as signed bytes, mrcstack.cpp:255-287)
     auto upload = [&](const std::vector<unsigned char> &raw, int mode, void *dst, size_t n) -> bool {
@@ -164,7 +240,7 @@ int main(int argc, char **argv)
                hipDeviceSynchronize() == hipSuccess;
     };
     if (!upload(stack, hin.mode, dstack.p, nstack) || (!init.empty() && !upload(init, hi.mode, dvol.p, nvol))) {
-        trec_context_destroy(ctx);
+        drop_ctx();
         return die(5, "copy to device failed");
     }
     stack.clear();
@@ -175,28 +251,52 @@ int main(int argc, char **argv)
     if (s == TREC_OK && hipDeviceSynchronize() != hipSuccess) s = TREC_ERR_DEVICE;
     if (s != TREC_OK) {
         std::string msg = std::string(trec_status_string(s)) + ": " + trec_last_error(ctx);
-        trec_context_destroy(ctx);
+        drop_ctx();
         return die(6, "reconstruction failed: %s", msg.c_str());
     }
     const double t_rec = now_s();
     MrcHeader hout = layout == TREC_LAYOUT_Y ? mrc_new_header(X, T, Y) : mrc_new_header(X, Y, T);
     if (o.mrc2014) mrc_set_2014(hout, hin.mx > 0 && hin.xlen > 0 ? hin.xlen / hin.mx : 1.f);
+    // The volume (float or half on the device) to the file: without MRC2014
+    // statistics the header needs only section nz / 2, so the volume streams
+    // out through pinned memory; otherwise, or without pinned memory, it is
+    // copied whole first.
+    auto write_out = [&](const void *dsrc, int mode) -> int {
+        const size_t es = mode == MRC_HALF ? 2 : 4;
+        const unsigned char *src = static_cast<const unsigned char *>(dsrc);
+        if (!o.mrc2014) {
+            std::vector<unsigned char> mid((size_t)hout.nx * hout.ny * es);
+            if (hipMemcpy(mid.data(), src + es * mrc_mid_section(hout), mid.size(), hipMemcpyDeviceToHost) !=
+                hipSuccess)
+                return die(5, "copy from device failed");
+            MrcStreamWriter w;
+            if (!w.open(o.output, hout, mode, mid.data(), err)) return die(4, "%s", err.c_str());
+            if (stream_out(w, src, es * nvol, err)) return w.close(err) ? 0 : die(4, "%s", err.c_str());
+            if (!err.empty()) return die(err == "copy from device failed" ? 5 : 4, "%s", err.c_str());
+        }
+        std::unique_ptr<unsigned char[]> vol(new (std::nothrow) unsigned char[es * nvol]);
+        if (!vol) return die(5, "host allocation failed");
+        if (hipMemcpy(vol.get(), src, es * nvol, hipMemcpyDeviceToHost) != hipSuccess)
+            return die(5, "copy from device failed");
+        if (!mrc_write_volume(o.output, hout, vol.get(), mode, o.mrc2014, err)) return die(4, "%s", err.c_str());
+        return 0;
+    };
     if (o.f2b) {
         // byte output over the whole volume's mean and deviation, as
         // TiltRecZ-cuda writes it (recon_z.cu:253-262)
         DevBuf dbytes;
         float stats[2] = {0, 0};
         if (hipMalloc(&dbytes.p, nvol) != hipSuccess) {
-            trec_context_destroy(ctx);
+            drop_ctx();
             return die(5, "device allocation failed");
         }
         s = trec_float_to_byte(ctx, (const float *)dvol.p, nvol, (unsigned char *)dbytes.p, stats, nullptr);
         if (s != TREC_OK) {
             std::string msg = std::string(trec_status_string(s)) + ": " + trec_last_error(ctx);
-            trec_context_destroy(ctx);
+            drop_ctx();
             return die(6, "byte conversion failed: %s", msg.c_str());
         }
-        trec_context_destroy(ctx);
+        drop_ctx();
         std::vector<unsigned char> bytes(nvol);
         if (hipMemcpy(bytes.data(), dbytes.p, nvol, hipMemcpyDeviceToHost) != hipSuccess)
             return die(5, "copy from device failed");
@@ -206,22 +306,16 @@ int main(int argc, char **argv)
         // half output: the GPU rounds to nearest even, half the bytes come down
         DevBuf dhalf;
         if (hipMalloc(&dhalf.p, 2 * nvol) != hipSuccess) {
-            trec_context_destroy(ctx);
+            drop_ctx();
             return die(5, "device allocation failed");
         }
         s = trec_convert(ctx, dvol.p, TREC_DT_F32, dhalf.p, TREC_DT_F16, nvol, nullptr);
-        trec_context_destroy(ctx);
+        drop_ctx();
         if (s != TREC_OK) return die(6, "half conversion failed: %s", trec_status_string(s));
-        std::vector<uint16_t> vol(nvol);
-        if (hipMemcpy(vol.data(), dhalf.p, 2 * nvol, hipMemcpyDeviceToHost) != hipSuccess)
-            return die(5, "copy from device failed");
-        if (!mrc_write_volume(o.output, hout, vol.data(), MRC_HALF, o.mrc2014, err)) return die(4, "%s", err.c_str());
+        if (const int rc = write_out(dhalf.p, MRC_HALF)) return rc;
     } else {
-        trec_context_destroy(ctx);
-        std::vector<float> vol(nvol);
-        if (hipMemcpy(vol.data(), dvol.p, 4 * nvol, hipMemcpyDeviceToHost) != hipSuccess)
-            return die(5, "copy from device failed");
-        if (!mrc_write_volume(o.output, hout, vol.data(), MRC_FLOAT, o.mrc2014, err)) return die(4, "%s", err.c_str());
+        drop_ctx();
+        if (const int rc = write_out(dvol.p, MRC_FLOAT)) return rc;
     }
     const double t_end = now_s();
     printf("read %.3f s, upload %.3f s, reconstruct %.3f s, write %.3f s, total %.3f s\n", t_read - t0,
