@@ -646,7 +646,7 @@ constexpr int GL_BUF_DWORD3 = 0x00020000;  // raw buffer resource, gfx9 family
 constexpr int GL_W = 516;      // staged columns per row
 constexpr int GL_PITCH = 520;  // floats between two staged rows
 template <int MODE, int NR>
-__global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int theta0, int ntheta,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 : 3))) void k_general_back_lds(const GeneralJob g, int theta0, int ntheta,
                                                          const float *__restrict__ img)
 {
     __shared__ float stage[WG / 64][NR * GL_PITCH];
@@ -677,26 +677,37 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
     unsigned orow[NR];  // staged rows rlo .. rlo + NR - 1 (clamped into the projection)
 #pragma unroll
     for (int k = 0; k < NR; ++k) orow[k] = (unsigned)min(rlo + k, Y - 1) * (unsigned)X;
-    // per voxel: the staged offset of row r0 and the weights of r0, r0 + 1
-    // (NR = 2: the same for every voxel)
-    double Xc[GL_Q];
-    float h0[GL_Q], h1[GL_Q], hw[GL_Q], out[GL_Q], wy0[GL_Q], wy1[GL_Q];
+    // per voxel: the staged offset of row r0 (0 with NR = 2) and the weights
+    // of r0, r0 + 1, which the tilt loop does not need: they are evaluated
+    // again at the end.  BPT and SART update the volume per tilt and keep it
+    // in registers; SIRT and ADMM read it at the end.
+    constexpr bool PER_TILT = MODE == BACK_BPT || MODE == BACK_SART;
+    auto row_weights = [&](int q, float &w0, float &w1) {
+        const double v = NR == 2 ? vof(X0) : vof((double)((float)(xb0 + lane + 64 * q) - (float)t0.ox));
+        const int r0 = min(max((int)v, 0), Y - 2);
+        const float dy = (float)(v - (double)r0);
+        w0 = hat(dy);
+        w1 = hat(dy - 1.f);
+        return r0;
+    };
+    const double Xc0 = (double)((float)(xb0 + lane) - (float)t0.ox);  // X of the lane's first voxel
+    float h0[GL_Q], h1[GL_Q], hw[GL_Q], out[PER_TILT ? GL_Q : 1], wy0[PER_TILT ? GL_Q : 1], wy1[PER_TILT ? GL_Q : 1];
     int ro[GL_Q];
     float *vout = g.vol + (size_t)row * g.X;
 #pragma unroll
     for (int q = 0; q < GL_Q; ++q) {
-        const int x = xb0 + lane + 64 * q;
-        Xc[q] = (double)((float)x - (float)t0.ox);
-        const double v = NR == 2 ? vof(X0) : vof(Xc[q]);
-        const int r0 = min(max((int)v, 0), Y - 2);
-        const float dy = (float)(v - (double)r0);
-        wy0[q] = hat(dy);
-        wy1[q] = hat(dy - 1.f);
+        float w0, w1;
+        const int r0 = row_weights(q, w0, w1);
         ro[q] = NR == 2 ? 0 : min(max(r0 - rlo, 0), NR - 2) * GL_PITCH;
+        if constexpr (PER_TILT) {
+            const int x = xb0 + lane + 64 * q;
+            wy0[q] = w0;
+            wy1[q] = w1;
+            out[q] = x < X ? vout[x] : 0.f;
+        }
         h0[q] = 0.f;
         h1[q] = 0.f;
         hw[q] = 0.f;
-        out[q] = x < X ? vout[x] : 0.f;
     }
     float *lds = stage[wave];
     // window of tilt th: u = ub + a1 X, its first column, and the staged
@@ -749,7 +760,7 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
         // fraction is good to about 5e-7).  Inside the detector (every voxel
         // of the wave at 1 <= u <= X - 3) the tap is floor(u); at its edges
         // it is clamped into [0, X - 2] and the fraction measured from there.
-        const double U0 = __fma_rn(a1, Xc[0], ubr), A = 64.0 * a1;
+        const double U0 = __fma_rn(a1, Xc0, ubr), A = 64.0 * a1;
         const double fU0 = floor(U0), fA_ = floor(A);
         const int I0 = (int)fU0, IA = (int)fA_;
         const float f0 = (float)(U0 - fU0), fA = (float)(A - fA_);
@@ -764,9 +775,11 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
                 const int base = I0 + q * IA;
                 const int li = min(max(base + (int)fl, 0), lhi);
                 const float dx = EDGE ? tq - (float)(li - base) : tq - fl;
-                const float w0 = hat(dx), w1 = hat(dx - 1.f);
+                // inside the detector dx is the fraction in [0, 1), whose
+                // hat weights are 1 - dx and dx
+                const float w0 = EDGE ? hat(dx) : 1.f - dx, w1 = EDGE ? hat(dx - 1.f) : dx;
                 const float *l0 = lds + ro[q] + li;
-                if (MODE == BACK_SIRT || MODE == BACK_LIN) {
+                if constexpr (!PER_TILT) {
                     h0[q] = fmaf(w1, l0[1], fmaf(w0, l0[0], h0[q]));
                     h1[q] = fmaf(w1, l0[GL_PITCH + 1], fmaf(w0, l0[GL_PITCH], h1[q]));
                     hw[q] += w0 + w1;
@@ -785,15 +798,19 @@ __global__ __launch_bounds__(WG) void k_general_back_lds(const GeneralJob g, int
 #pragma unroll
     for (int q = 0; q < GL_Q; ++q) {
         const int x = xb0 + lane + 64 * q;
-        if (MODE == BACK_LIN) {
-            if (x < X) g.out[(size_t)row * g.X + x] = g.lin_a * (wy0[q] * h0[q] + wy1[q] * h1[q]) + g.lin_b * out[q];
-            continue;
+        if constexpr (PER_TILT) {
+            if (x < X) vout[x] = out[q];
+        } else {
+            float w0, w1;
+            row_weights(q, w0, w1);
+            const float vq = x < X ? vout[x] : 0.f;
+            if (MODE == BACK_LIN) {
+                if (x < X) g.out[(size_t)row * g.X + x] = g.lin_a * (w0 * h0[q] + w1 * h1[q]) + g.lin_b * vq;
+            } else {
+                const float val = w0 * h0[q] + w1 * h1[q], wt = (w0 + w1) * hw[q];
+                if (x < X) vout[x] = wt > g.thr_upd ? vq + val / wt * g.gamma : vq;
+            }
         }
-        if (MODE == BACK_SIRT) {
-            const float val = wy0[q] * h0[q] + wy1[q] * h1[q], wt = (wy0[q] + wy1[q]) * hw[q];
-            if (wt > g.thr_upd) out[q] += val / wt * g.gamma;
-        }
-        if (x < X) vout[x] = out[q];
     }
 }
 
