@@ -342,7 +342,9 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
         const double Zc = (double)((float)z - (float)t.oz);
         const double cu = t.a[0] + t.a[3] * Zc + t.ox;  // u = cu + a1 X + a2 Y
         const double cv = t.b[0] + t.b[3] * Zc + t.oy;  // v = cv + b2 Y
-        const int ylo = (int)floor(((double)j0 - cv) * rb2 + oy) - 1;
+        // uniform (the layer's first slot row): the slots' row bases then live
+        // in scalar registers and a load adds only the lane's column
+        const int ylo = __builtin_amdgcn_readfirstlane((int)floor(((double)j0 - cv) * rb2 + oy) - 1);
         const double Yc0 = (double)((float)ylo - (float)t.oy);
         // y weights: v - (j0 + k - 2) of slot k, then hat of its distance to row r
         float pw[NH];
@@ -381,7 +383,8 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
                 for (int c = 0; c < NC; ++c) s += hat(fmaf(a1f, (float)c, d0));
             } else {
                 const int yc = min(max(ylo + k, 0), g.Y - 1);
-                const float *p = vlayer + (size_t)yc * sy + xs;
+                const float *p = reinterpret_cast<const float *>(
+                    reinterpret_cast<const char *>(vlayer + (size_t)yc * sy) + (unsigned)xs * 4u);
                 float v[NC];
 #pragma unroll
                 for (int c = 0; c < NC; ++c) v[c] = p[c];
