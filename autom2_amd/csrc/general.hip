@@ -480,13 +480,13 @@ __global__ __launch_bounds__(64) void k_general_splat(const GeneralJob g, int th
         // [i0 - 1, i1) on those rows.  A voxel the rounding of these bounds
         // leaves out lies within rounding of the tile's edge, where its
         // weight on the tile is of that order.
-        const int ya = max((int)floor(((double)(j0 - 1) - cv - bh) / b2 + oyd), 0);
-        const int yb = min((int)floor(((double)j1 - cv - bl) / b2 + oyd) + 1, g.Y);
+        const int ya = __builtin_amdgcn_readfirstlane(max((int)floor(((double)(j0 - 1) - cv - bh) / b2 + oyd), 0));
+        const int yb = __builtin_amdgcn_readfirstlane(min((int)floor(((double)j1 - cv - bl) / b2 + oyd) + 1, g.Y));
         if (ya >= yb) continue;
         const double ya2 = a2 * (double)((float)ya - oyf), yb2 = a2 * (double)((float)(yb - 1) - oyf);
         const double al = fmin(ya2, yb2), ah = fmax(ya2, yb2);
-        const int xa = max((int)floor(((double)(i0 - 1) - cu - ah) / a1 + oxd), 0);
-        const int xb = min((int)floor(((double)i1 - cu - al) / a1 + oxd) + 1, g.X);
+        const int xa = __builtin_amdgcn_readfirstlane(max((int)floor(((double)(i0 - 1) - cu - ah) / a1 + oxd), 0));
+        const int xb = __builtin_amdgcn_readfirstlane(min((int)floor(((double)i1 - cu - al) / a1 + oxd) + 1, g.X));
         const int nrow = yb - ya;
         const double Ya = (double)((float)ya - oyf);
         const float *layer = g.vol + (size_t)z * sz + (size_t)ya * sy;
@@ -499,11 +499,16 @@ __global__ __launch_bounds__(64) void k_general_splat(const GeneralJob g, int th
             const float fu0 = (float)(U0 - fU), fv0 = (float)(V0 - fV);
             // LDS column and row of the first row's upper-left tap
             const int cu0 = (int)fU - i0 + SP_CP, rv0 = (int)fV - j0 + SP_RP;
-            const float *colp = layer + min(x, g.X - 1);
+            // row bases are uniform (scalar registers); a load adds the lane's column
+            const unsigned xoff = (unsigned)min(x, g.X - 1) * 4u;
             for (int k0 = 0; k0 < nrow; k0 += 4) {
                 float vals[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) vals[q] = (ONES || k0 + q >= nrow) ? 1.f : colp[(size_t)(k0 + q) * sy];
+                for (int q = 0; q < 4; ++q)
+                    vals[q] = (ONES || k0 + q >= nrow)
+                                  ? 1.f
+                                  : *reinterpret_cast<const float *>(
+                                        reinterpret_cast<const char *>(layer + (size_t)(k0 + q) * sy) + xoff);
                 if (!in) continue;  // lanes past the box (the whole row group)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
