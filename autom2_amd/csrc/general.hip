@@ -407,6 +407,181 @@ __global__ __launch_bounds__(WG) void k_general_rows(const GeneralJob g, int the
         if (j0 + r < g.Y) out[(size_t)(j0 + r) * g.X + i] = acc[r];
 }
 
+// The row-window sums of k_general_rows for a1 > 0, arranged so that a slot
+// costs about half the instructions:
+//
+//  - With e = 1 - b2, ylo is the least volume row with v(ylo) >= j0 - e.
+//    Then g_k = v(ylo + k) - j0 - k = g0 - e k with g0 in [-e, 1 - 2e), so
+//    g_k lies in [-1, 1] for k = -2 .. RJ + 1 while e (RJ + 2) <= 1.  Slot k
+//    then reaches rows k - 1, k, k + 1 only, with weights max(0, -g_k),
+//    1 - |g_k| and max(0, g_k), and only slots k = -1 .. RJ reach rows 0 ..
+//    RJ - 1 (RJ + 2 slots, three weights each, instead of RJ + 4 and five).
+//  - The first candidate column of slot k is floor(F0 - (a2 / a1) k) + 1;
+//    with fr the fraction of F0 - (a2 / a1) k, candidate c lies at
+//    a1 (1 + c - fr) - 1 from the pixel.  Its weight is a1 (1 - fr) for c =
+//    0 and 1 - |a1 (2 - fr) - 1| for c = 1, both known to lie in [0, 1], and
+//    for the later ones whose distance is known to be positive (2 c >= NC)
+//    2 - a1 (1 + c - fr) clamped into [0, 1] (the clamp is an instruction
+//    modifier): one or two instructions each instead of three.
+//  - The per-layer constants (ylo, g0, the u offset, whether the wave's
+//    windows reach the row at all, and whether they all lie inside it) are
+//    evaluated for 64 layers at once, one per lane, and read per layer with
+//    v_readlane.  A layer whose windows all miss the row is skipped.
+//  - A wave whose windows all lie inside the row (decided from its first and
+//    last pixel, with a column of margin) loads them as they are; the others
+//    clamp each window into the row and weigh every column by the hat of its
+//    distance, as k_general_rows does.
+//
+// Sums run over z, then slots in y order, then columns in x order.
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.f, 1.f); }
+
+template <int RJ, int NC, bool ONES>
+__global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int theta0, int ntheta, int nzc)
+{
+    constexpr int NS = RJ + 2;  // slots k = -1 .. RJ at s = k + 1
+    constexpr int SB = 6;       // slots whose loads may be in flight together
+    static_assert(NS <= 64, "a lane per slot");
+    __shared__ float4 ywl[WG / 64][NS];
+    float4 *yw = ywl[threadIdx.x / 64];
+    const unsigned nib = (g.X + WG - 1) / WG;
+    const unsigned lin = xcd_order(blockIdx.x, gridDim.x);
+    const int th = theta0 + (int)(lin % (unsigned)ntheta);
+    const unsigned rest = lin / (unsigned)ntheta;
+    // pixels beyond the row repeat its last one and are not stored
+    const int i = (int)(rest % nib) * WG + threadIdx.x;
+    const int ic = min(i, g.X - 1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int iw = __builtin_amdgcn_readfirstlane(i - lane);  // the wave's first pixel
+    const int ia = min(iw, g.X - 1), ib = min(iw + 63, g.X - 1);
+    const unsigned njb = (g.Y + RJ - 1) / RJ;
+    const int j0 = (int)((rest / nib) % njb) * RJ;
+    const int zc = (int)(rest / nib / njb);
+    const int zlen = (g.T + nzc - 1) / nzc;
+    const int z0 = zc * zlen, z1 = min(g.T, z0 + zlen);
+    const TiltGeneral &t = g.tg[th];
+    const double a1 = t.a[1], a2 = t.a[2], b2 = t.b[2];
+    const double ra1 = 1.0 / a1, rb2 = 1.0 / b2;
+    const double ox = (float)t.ox, oy = (float)t.oy;
+    const float a1f = (float)a1, rkf = (float)(a2 * ra1), ef = (float)(1.0 - b2);
+    float kc[NC];  // c = 1: 2 a1 - 1; 2 c >= NC: 2 - a1 (1 + c); else a1 (1 + c) - 1
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        kc[c] = c == 1 ? (float)(2.0 * a1 - 1.0) : (2 * c >= NC ? (float)(2.0 - a1 * (1.0 + c)) : (float)(a1 * (1.0 + c) - 1.0));
+    // how far the first candidate moves over the slots, plus rounding slack
+    const double drift = fabs(a2 * ra1) * (double)RJ + 0.01;
+    const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
+    const size_t sz = g.layout == TREC_LAYOUT_Y ? (size_t)g.X : (size_t)g.Y * g.X;
+    // constants of layers zb .. zb + 63, lane l holding layer zb + l: u - i
+    // at column x of slot k is W - i + a1 x + a2 k; mode 0 skips the layer,
+    // 1 takes the clamped windows, 2 the inner ones
+    int ylo_v = 0, mode_v = 0;
+    float g0_v = 0.f;
+    double w_v = 0.0;
+    auto layer_block = [&](int zb) {
+        const int z = min(zb + lane, z1 - 1);
+        const double Zc = (double)((float)z - (float)t.oz);
+        const double cu = t.a[0] + t.a[3] * Zc + t.ox;  // u = cu + a1 X + a2 Y
+        const double cv = t.b[0] + t.b[3] * Zc + t.oy;  // v = cv + b2 Y
+        ylo_v = (int)ceil(((double)j0 - (1.0 - b2) - cv) * rb2 + oy);
+        const double Yc0 = (double)((float)ylo_v - (float)t.oy);
+        g0_v = (float)(cv + b2 * Yc0 - (double)j0);
+        w_v = cu + a2 * Yc0 - a1 * ox;
+        const double la = floor(((double)ia - 1.0 - w_v) * ra1 - drift) + 1.0;  // least first column
+        const double lb = floor(((double)ib - 1.0 - w_v) * ra1 + drift) + 1.0;  // greatest first column
+        const bool reach = lb + (double)NC >= 1.0 && la <= (double)(g.X - 1) && ylo_v + RJ >= 0 && ylo_v - 1 < g.Y;
+        const bool inner = la >= 0.0 && lb + (double)NC <= (double)g.X;
+        mode_v = reach ? (inner ? 2 : 1) : 0;
+    };
+    float acc[RJ];
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) acc[r] = 0.f;
+    int zb = z0;
+    layer_block(zb);
+    for (int z = z0; z < z1; ++z) {
+        if (z - zb == 64) {
+            zb = z;
+            layer_block(zb);
+        }
+        const int zl = z - zb;
+        const int mode = __builtin_amdgcn_readlane(mode_v, zl);
+        if (mode == 0) continue;
+        const int ylo = __builtin_amdgcn_readlane(ylo_v, zl);
+        const float g0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0_v), zl));
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(w_v);
+        const double W = __longlong_as_double(
+            (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(wb >> 32), zl) << 32) |
+                        (unsigned)__builtin_amdgcn_readlane((int)(unsigned)wb, zl)));
+        {
+            // lane s takes slot s; rows outside the volume weigh nothing
+            const int s = min(lane, NS - 1);
+            const float gk = __builtin_amdgcn_fmed3f(fmaf(-ef, (float)(s - 1), g0), -1.f, 1.f);
+            const bool in = (unsigned)(ylo - 1 + s) < (unsigned)g.Y;
+            const float4 w = in ? make_float4(fmaxf(0.f, -gk), 1.f - fabsf(gk), fmaxf(0.f, gk), 0.f)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+            __builtin_amdgcn_wave_barrier();
+            if (lane < NS) yw[s] = w;
+            __builtin_amdgcn_wave_barrier();
+        }
+        const double F0 = ((double)ic - 1.0 - W) * ra1;
+        const double X0d = floor(F0);
+        const int X1 = (int)X0d + 1;
+        const float f0 = (float)(F0 - X0d);
+        const float *vlayer = g.vol + (size_t)z * sz;
+        auto slots = [&](auto edge_tag) {
+            constexpr bool EDGE = decltype(edge_tag)::value;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int k = s - 1;
+                const float tk = fmaf(-rkf, (float)k, f0);
+                const int xf = X1 + (int)floorf(tk);
+                const float fr = __builtin_amdgcn_fractf(tk);
+                const int xs = EDGE ? min(max(xf, 0), g.X - NC) : xf;
+                float w[NC];
+                if constexpr (EDGE) {
+                    const float d0 = fmaf(a1f, (float)(xs - xf), fmaf(-a1f, fr, a1f - 1.f));
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) w[c] = clamp01(1.f - fabsf(fmaf(a1f, (float)c, d0)));
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) {
+                        if (c == 0) w[c] = fmaf(-a1f, fr, a1f);
+                        else if (c == 1) w[c] = 1.f - fabsf(fmaf(-a1f, fr, kc[1]));
+                        else if (2 * c >= NC) w[c] = clamp01(fmaf(a1f, fr, kc[c]));
+                        else w[c] = clamp01(1.f - fabsf(fmaf(-a1f, fr, kc[c])));
+                    }
+                }
+                float sx = 0.f;
+                if constexpr (ONES) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) sx += w[c];
+                } else {
+                    const int yc = min(max(ylo + k, 0), g.Y - 1);
+                    const float *p = reinterpret_cast<const float *>(
+                        reinterpret_cast<const char *>(vlayer + (size_t)yc * sy) + (unsigned)xs * 4u);
+                    float v[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) v[c] = p[c];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) sx = fmaf(w[c], v[c], sx);
+                }
+                const float4 wy = yw[s];
+                if (s >= 2 && s - 2 < RJ) acc[s - 2] = fmaf(wy.x, sx, acc[s - 2]);
+                if (s >= 1 && s - 1 < RJ) acc[s - 1] = fmaf(wy.y, sx, acc[s - 1]);
+                if (s < RJ) acc[s] = fmaf(wy.z, sx, acc[s]);
+                if ((s + 1) % SB == 0) __builtin_amdgcn_sched_barrier(0);
+            }
+        };
+        if (mode == 2) slots(std::false_type{});
+        else slots(std::true_type{});
+    }
+    if (i >= g.X) return;
+    float *out = nzc > 1 ? g.part + ((size_t)zc * ntheta + (th - theta0)) * g.Y * g.X
+                         : (ONES ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+#pragma unroll
+    for (int r = 0; r < RJ; ++r)
+        if (j0 + r < g.Y) out[(size_t)(j0 + r) * g.X + i] = acc[r];
+}
+
 // Forward projection as the reference computes it, a bilinear splat of
 // every voxel (Reproject, main-mpi.cpp:286-344), for tilts with a1 >= 1/3
 // and 0.55 <= b2 <= 1: any pitch with or without an offset.  A wave owns a
@@ -657,7 +832,7 @@ template <int MODE, int NR>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 : 3))) void k_general_back_lds(const GeneralJob g, int theta0, int ntheta,
                                                          const float *__restrict__ img)
 {
-    __shared__ float stage[WG / 64][NR * GL_PITCH];
+    __shared__ __attribute__((aligned(16))) float stage[WG / 64][NR * GL_PITCH];
     const unsigned nxt = (g.X + GL_Q * 64 - 1) / (GL_Q * 64);
     const unsigned wave = threadIdx.x / 64;
     const unsigned row = __builtin_amdgcn_readfirstlane((blockIdx.x / nxt) * (WG / 64) + wave);
@@ -720,44 +895,86 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
     float *lds = stage[wave];
     // window of tilt th: u = ub + a1 X, its first column, and the staged
     // values of columns cmin + lane + 64 m.
-    constexpr int GM = (GL_W + 63) / 64;
-    static_assert(64 * (GM - 1) <= GL_W && GL_W <= GL_PITCH, "staged columns fit a row, full chunks inside");
-    float sv[NR][GM];
-    int cnext = 0;
-    double ubnext = 0.0, a1next = 0.0;
-    auto load = [&](int th) {
-        const TiltGeneral &t = g.tg[th];
-        ubnext = t.a[0] + t.a[2] * Yc + t.a[3] * Zc + t.ox;
-        a1next = t.a[1];
-        const double u0 = __fma_rn(a1next, X0, ubnext), u1 = __fma_rn(a1next, X1, ubnext);
-        cnext = min(max((int)fmin(u0, u1), 0), X - 2);
-        // buffer loads: the row base as the scalar offset, the lane and chunk
-        // in the instruction; columns past the row (never used: taps stop at
-        // X - 1) read the next row, or 0 past the projection
+    static_assert(GL_W <= GL_PITCH, "staged columns fit a row");
+    // the window is staged in chunks of 4 columns, at most GC per row, of
+    // which lane l loads chunks l, l + 64, ...
+    constexpr int GC = GL_W / 4, GK = (GC + 63) / 64;
+    static_assert(GL_W % 4 == 0 && GL_PITCH % 4 == 0, "16-byte chunks");
+    float4 sv[NR][GK];
+    // Constants of tilts tb .. tb + 63, lane l holding tilt tb + l: u = ub +
+    // a1 X, the window's first column cmin, whether every voxel of the wave
+    // lies inside the detector (1 <= u <= X - 3), 64 a1 split into its
+    // integer and fractional parts, and the chunks the taps can reach
+    // (columns up to (64 GL_Q - 1) |a1| + 3 past cmin, with a chunk of
+    // margin).  Each tilt reads them with v_readlane.
+    double ub_v = 0.0, a1_v = 0.0;
+    int cmin_v = 0, in_v = 0, ia_v = 0, nch_v = 0;
+    float fa_v = 0.f;
+    auto tilt_block = [&](int tb) {
+        const TiltGeneral &t = g.tg[min(tb + lane, theta0 + ntheta - 1)];
+        ub_v = t.a[0] + t.a[2] * Yc + t.a[3] * Zc + t.ox;
+        a1_v = t.a[1];
+        const double u0 = __fma_rn(a1_v, X0, ub_v), u1 = __fma_rn(a1_v, X1, ub_v);
+        cmin_v = min(max((int)fmin(u0, u1), 0), X - 2);
+        in_v = fmin(u0, u1) >= 1.0 && fmax(u0, u1) <= (double)(X - 3);
+        const double A = 64.0 * a1_v, fA_ = floor(A);
+        ia_v = (int)fA_;
+        fa_v = (float)(A - fA_);
+        nch_v = min(GC, (int)(((64.0 * GL_Q - 1.0) * fabs(a1_v) + 4.0) * 0.25) + 2);
+    };
+    auto lane_d = [](double v, int l) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    };
+    // buffer loads: the row base as the scalar offset, the lane and chunk in
+    // the instruction; columns past the row (never used: taps stop at X - 1)
+    // read the next row, or 0 past the projection
+    auto load = [&](int th, int cnext, int nch) {
         const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)(img + (size_t)th * Y * X), (short)0,
                                                           (int)((unsigned)Y * (unsigned)X * 4u), GL_BUF_DWORD3);
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const int soff = (int)((orow[k] + (unsigned)cnext) * 4u);
 #pragma unroll
-            for (int m = 0; m < GM; ++m)
-                sv[k][m] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, lane * 4 + 256 * m, soff, 0));
+            for (int m = 0; m < GK; ++m) {
+                if (lane + 64 * m < nch) {
+                    const auto r = __builtin_amdgcn_raw_buffer_load_b128(rs, lane * 16 + 1024 * m, soff, 0);
+                    sv[k][m] = make_float4(__uint_as_float(r[0]), __uint_as_float(r[1]), __uint_as_float(r[2]),
+                                           __uint_as_float(r[3]));
+                }
+            }
         }
     };
-    load(theta0);
+    int tb = theta0;
+    tilt_block(tb);
+    load(theta0, __builtin_amdgcn_readlane(cmin_v, 0), __builtin_amdgcn_readlane(nch_v, 0));
+    int nin = 0;  // SIRT and ADMM: tilts in which every voxel of the wave lies inside
     for (int th = theta0; th < theta0 + ntheta; ++th) {
-        const int cmin = cnext;
-        const double ub = ubnext, a1 = a1next;
+        const int tl = th - tb;
+        const int cmin = __builtin_amdgcn_readlane(cmin_v, tl);
+        const bool inside = __builtin_amdgcn_readlane(in_v, tl) != 0;
+        const int IA = __builtin_amdgcn_readlane(ia_v, tl);
+        const float fA = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fa_v), tl));
+        const int nch = __builtin_amdgcn_readlane(nch_v, tl);
+        const double ub = lane_d(ub_v, tl), a1 = lane_d(a1_v, tl);
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int m = 0; m < GM; ++m) {
-            if (m < GM - 1 || lane + 64 * m < GL_W) {
+        for (int m = 0; m < GK; ++m) {
+            if (lane + 64 * m < nch) {
 #pragma unroll
-                for (int k = 0; k < NR; ++k) lds[k * GL_PITCH + lane + 64 * m] = sv[k][m];
+                for (int k = 0; k < NR; ++k) *reinterpret_cast<float4 *>(lds + k * GL_PITCH + 4 * (lane + 64 * m)) = sv[k][m];
             }
         }
         __builtin_amdgcn_wave_barrier();
-        if (th + 1 < theta0 + ntheta) load(th + 1);
+        if (th + 1 < theta0 + ntheta) {
+            if (th + 1 - tb == 64) {
+                tb = th + 1;
+                tilt_block(tb);
+            }
+            load(th + 1, __builtin_amdgcn_readlane(cmin_v, th + 1 - tb), __builtin_amdgcn_readlane(nch_v, th + 1 - tb));
+        }
         // u - cmin = ubr + a1 X; the staged column li = c0 - cmin of tap c0 =
         // min(max(trunc u, 0), X - 2), which is at most (64 GL_Q - 1) |a1| + 1
         // (the bound GL_W - 2 only guards it)
@@ -766,14 +983,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
         // u - cmin of voxel q is U0 + q A, A = 64 a1: in float from the
         // integer and fractional parts of U0 and A (|f0 + q fA| < 8, so the
         // fraction is good to about 5e-7).  Inside the detector (every voxel
-        // of the wave at 1 <= u <= X - 3) the tap is floor(u); at its edges
-        // it is clamped into [0, X - 2] and the fraction measured from there.
-        const double U0 = __fma_rn(a1, Xc0, ubr), A = 64.0 * a1;
-        const double fU0 = floor(U0), fA_ = floor(A);
-        const int I0 = (int)fU0, IA = (int)fA_;
-        const float f0 = (float)(U0 - fU0), fA = (float)(A - fA_);
-        const double uext = __fma_rn(a1, X1, ub), uext0 = __fma_rn(a1, X0, ub);
-        const bool inside = fmin(uext0, uext) >= 1.0 && fmax(uext0, uext) <= (double)(X - 3);
+        // of the wave at 1 <= u <= X - 3) the tap is floor(u), at most
+        // (64 GL_Q - 1) |a1| + 1 columns past cmin, with the weights 1 - dx and
+        // dx, which sum to 1 (SIRT and ADMM count those tilts instead of
+        // adding them); at its edges the tap is clamped into [0, X - 2] and
+        // the fraction measured from there.
+        const double U0 = __fma_rn(a1, Xc0, ubr);
+        const double fU0 = floor(U0);
+        const int I0 = (int)fU0;
+        const float f0 = (float)(U0 - fU0);
         auto voxels = [&](auto edge_tag) {
             constexpr bool EDGE = decltype(edge_tag)::value;
 #pragma unroll
@@ -781,16 +999,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
                 const float tq = fmaf((float)q, fA, f0);
                 const float fl = floorf(tq);
                 const int base = I0 + q * IA;
-                const int li = min(max(base + (int)fl, 0), lhi);
+                const int li = EDGE ? min(max(base + (int)fl, 0), lhi) : base + (int)fl;
                 const float dx = EDGE ? tq - (float)(li - base) : tq - fl;
-                // inside the detector dx is the fraction in [0, 1), whose
-                // hat weights are 1 - dx and dx
                 const float w0 = EDGE ? hat(dx) : 1.f - dx, w1 = EDGE ? hat(dx - 1.f) : dx;
                 const float *l0 = lds + ro[q] + li;
                 if constexpr (!PER_TILT) {
                     h0[q] = fmaf(w1, l0[1], fmaf(w0, l0[0], h0[q]));
                     h1[q] = fmaf(w1, l0[GL_PITCH + 1], fmaf(w0, l0[GL_PITCH], h1[q]));
-                    hw[q] += w0 + w1;
+                    if (EDGE) hw[q] += w0 + w1;
                 } else {
                     const float s = wy0[q] * fmaf(w1, l0[1], w0 * l0[0]) +
                                     wy1[q] * fmaf(w1, l0[GL_PITCH + 1], w0 * l0[GL_PITCH]);
@@ -799,10 +1015,16 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
                 }
             }
         };
-        if (inside) voxels(std::false_type{});
-        else voxels(std::true_type{});
+        if (inside) {
+            voxels(std::false_type{});
+            ++nin;
+        } else {
+            voxels(std::true_type{});
+        }
         __builtin_amdgcn_wave_barrier();
     }
+#pragma unroll
+    for (int q = 0; q < GL_Q; ++q) hw[q] += (float)nin;
 #pragma unroll
     for (int q = 0; q < GL_Q; ++q) {
         const int x = xb0 + lane + 64 * q;
@@ -831,24 +1053,30 @@ bool same_rows(const TiltGeneral &a, const TiltGeneral &b)
     return std::memcmp(a.b, b.b, sizeof a.b) == 0 && a.ox == b.ox && a.oy == b.oy && a.oz == b.oz;
 }
 
+// three: k_general_rows3 (a1 > 0 in every tilt), else k_general_rows
 template <int RJ, int NC>
-void launch_rows(const GeneralJob &g, int theta0, int ntheta, bool ones, int nzc, unsigned nblk, hipStream_t st)
+void launch_rows(const GeneralJob &g, int theta0, int ntheta, bool ones, bool three, int nzc, unsigned nblk,
+                 hipStream_t st)
 {
-    if (ones)
+    if (three && ones)
+        hipLaunchKernelGGL((k_general_rows3<RJ, NC, true>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc);
+    else if (three)
+        hipLaunchKernelGGL((k_general_rows3<RJ, NC, false>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc);
+    else if (ones)
         hipLaunchKernelGGL((k_general_rows<RJ, NC, true>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc);
     else
         hipLaunchKernelGGL((k_general_rows<RJ, NC, false>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc);
 }
 
 template <int RJ>
-void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, int nc, int nzc, unsigned nblk,
-                    hipStream_t st)
+void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, bool three, int nc, int nzc,
+                    unsigned nblk, hipStream_t st)
 {
     switch (nc) {
-    case 2: launch_rows<RJ, 2>(g, theta0, ntheta, ones, nzc, nblk, st); break;
-    case 3: launch_rows<RJ, 3>(g, theta0, ntheta, ones, nzc, nblk, st); break;
-    case 4: launch_rows<RJ, 4>(g, theta0, ntheta, ones, nzc, nblk, st); break;
-    default: launch_rows<RJ, 6>(g, theta0, ntheta, ones, nzc, nblk, st); break;
+    case 2: launch_rows<RJ, 2>(g, theta0, ntheta, ones, three, nzc, nblk, st); break;
+    case 3: launch_rows<RJ, 3>(g, theta0, ntheta, ones, three, nzc, nblk, st); break;
+    case 4: launch_rows<RJ, 4>(g, theta0, ntheta, ones, three, nzc, nblk, st); break;
+    default: launch_rows<RJ, 6>(g, theta0, ntheta, ones, three, nzc, nblk, st); break;
     }
 }
 
@@ -904,12 +1132,12 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
                                   hipStream_t st, int *nzc_used)
 {
     if (nzc_used) *nzc_used = 1;
-    // TREC_GENERAL_FORWARD=rows|splat|gather puts that kernel first where it
+    // TREC_GENERAL_FORWARD=rows|rows5|splat|gather puts that kernel first where it
     // applies (tests and timing).  By default: the row windows when v does
     // not depend on x, else the splat, else the per-pixel gather.
     const char *force = getenv("TREC_GENERAL_FORWARD");
     const bool no_rows = force && !strcmp(force, "gather");
-    const bool no_splat = force && (!strcmp(force, "gather") || !strcmp(force, "rows"));
+    const bool no_splat = force && (!strcmp(force, "gather") || !strcmp(force, "rows") || !strcmp(force, "rows5"));
     // a single-tilt value projection whose caller takes partials splits the
     // layers into chunks, for GZ_CHUNKS times the waves; the splat of all
     // tilts takes g.nzc_all chunks, for L2 locality
@@ -920,11 +1148,14 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     bool rows = ntheta > 0 && !no_rows;
     int nc = 2;
     double b2min = 1.0;
+    // the three-row slots take a1 > 0 (TREC_GENERAL_FORWARD=rows5: the five-row ones)
+    bool three = !(force && !strcmp(force, "rows5"));
     for (int th = theta0; th < theta0 + ntheta && rows; ++th) {
         const TiltGeneral &t = host_tg[th];
         const double aa1 = fabs(t.a[1]);
         rows = t.b[1] == 0.0 && t.b[2] <= 1.0 && same_rows(t, host_tg[theta0]) && aa1 >= 1.0 / 3.0;
         if (!rows) break;
+        three = three && t.a[1] > 0.0;
         nc = std::max(nc, (int)std::ceil(2.0 / aa1 - 1e-6));
         b2min = std::min(b2min, t.b[2]);
     }
@@ -934,6 +1165,14 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     // times the waves of 16 at twice the slots per row
     int rj = b2min >= 15.0 / 16.0 ? 16 : (b2min >= 7.0 / 8.0 ? 8 : 0);
     if (ntheta <= 4 && b2min >= 3.0 / 4.0) rj = 4;
+    if (three) {
+        // three rows per slot need (1 - b2)(RJ + 2) <= 1: 16 rows up to about
+        // 19 degrees, 8 up to about 25, 4 up to about 33
+        int r3 = ntheta <= 4 ? 4 : 16;
+        while (r3 >= 4 && (1.0 - b2min) * (r3 + 2) > 1.0 - 1e-9) r3 /= 2;
+        if (r3 >= 4) rj = r3;
+        else three = false;
+    }
     rows = rows && rj > 0 && g.X >= nc;
     const size_t nib = cdiv((size_t)g.X, WG);
     // the splat: every tilt launched has its tiles
@@ -957,9 +1196,9 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     if (rows) {
         const size_t nblk = nib * cdiv((size_t)g.Y, rj) * (size_t)ntheta * (size_t)nzc;
         if (nblk <= 0x7fffffffu) {
-            if (rj == 16) launch_rows_nc<16>(g, theta0, ntheta, ones, nc, nzc, (unsigned)nblk, st);
-            else if (rj == 8) launch_rows_nc<8>(g, theta0, ntheta, ones, nc, nzc, (unsigned)nblk, st);
-            else launch_rows_nc<4>(g, theta0, ntheta, ones, nc, nzc, (unsigned)nblk, st);
+            if (rj == 16) launch_rows_nc<16>(g, theta0, ntheta, ones, three, nc, nzc, (unsigned)nblk, st);
+            else if (rj == 8) launch_rows_nc<8>(g, theta0, ntheta, ones, three, nc, nzc, (unsigned)nblk, st);
+            else launch_rows_nc<4>(g, theta0, ntheta, ones, three, nc, nzc, (unsigned)nblk, st);
             if (nzc_used) *nzc_used = nzc;
             return hipGetLastError();
         }
