@@ -824,19 +824,36 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
 // k_general_back_rows'; SIRT and ADMM sum each voxel's two detector rows'
 // column-weighted taps over the tilts and apply the row weights, the same in
 // every tilt, once per voxel.
+//
+// SH: the four waves of a work-group take four neighbouring layers of one
+// volume row (layout Y with T a multiple of 4), whose detector rows and
+// columns nearly coincide: v moves by b3 and u by a3 per layer.  The group
+// then stages the union of their windows once per tilt, NR + 1 rows (3 |b3|
+// < 1) of at most GL_WS columns, double-buffered with one barrier per tilt,
+// and each wave takes its taps from its own part of it.  Every wave
+// evaluates the union itself, with the same expressions, so they agree.
 constexpr int GL_Q = 8;        // voxels per lane, 64 apart
 constexpr int GL_BUF_DWORD3 = 0x00020000;  // raw buffer resource, gfx9 family
 constexpr int GL_W = 516;      // staged columns per row
 constexpr int GL_PITCH = 520;  // floats between two staged rows
-template <int MODE, int NR>
+constexpr int GL_WS = 528;     // columns of a shared window: GL_W plus the shift of 3 layers, 3 |a3| <= 3
+template <int MODE, int NR, bool SH>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 : 3))) void k_general_back_lds(const GeneralJob g, int theta0, int ntheta,
                                                          const float *__restrict__ img)
 {
-    __shared__ __attribute__((aligned(16))) float stage[WG / 64][NR * GL_PITCH];
+    constexpr int NRS = SH ? NR + 1 : NR;        // staged rows
+    constexpr int PITCH = SH ? GL_WS : GL_PITCH;  // floats between two staged rows
+    constexpr int NBUF = SH ? 2 : WG / 64;        // shared: two buffers; else one per wave
+    __shared__ __attribute__((aligned(16))) float stage[NBUF][NRS * PITCH];
     const unsigned nxt = (g.X + GL_Q * 64 - 1) / (GL_Q * 64);
     const unsigned wave = threadIdx.x / 64;
-    const unsigned row = __builtin_amdgcn_readfirstlane((blockIdx.x / nxt) * (WG / 64) + wave);
-    if (row >= (unsigned)g.Y * (unsigned)g.T) return;  // the whole wave: it shares nothing with the others
+    const unsigned rows = (unsigned)g.Y * (unsigned)g.T;
+    const unsigned row0 = __builtin_amdgcn_readfirstlane((blockIdx.x / nxt) * (WG / 64));  // the group's first row
+    const unsigned row_w = __builtin_amdgcn_readfirstlane(row0 + wave);
+    // a wave past the last row shares the group's barriers but computes nothing
+    if (!SH && row_w >= rows) return;  // the whole wave: it shares nothing with the others
+    const bool active = row_w < rows;
+    const unsigned row = min(row_w, rows - 1);
     const int lane = (int)(threadIdx.x & 63);
     const int xb0 = (int)(blockIdx.x % nxt) * GL_Q * 64;
     const int y = g.layout == TREC_LAYOUT_Y ? (int)(row / g.T) : (int)(row % g.Y);
@@ -848,18 +865,31 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
     // ValCoef's v, n1 = ((b0 + b1 X) + b2 Y) + b3 Z and v = n1 + oy; its
     // double rounding is monotone in X, so the wave's least row comes from
     // its first or its last voxel
-    auto vof = [&](double Xc) {
+    auto vofr = [&](double Xc, double Ycr, double Zcr) {
         return __dadd_rn(
-            __dadd_rn(__dadd_rn(__dadd_rn(t0.b[0], __dmul_rn(t0.b[1], Xc)), __dmul_rn(t0.b[2], Yc)),
-                      __dmul_rn(t0.b[3], Zc)),
+            __dadd_rn(__dadd_rn(__dadd_rn(t0.b[0], __dmul_rn(t0.b[1], Xc)), __dmul_rn(t0.b[2], Ycr)),
+                      __dmul_rn(t0.b[3], Zcr)),
             t0.oy);
     };
+    auto vof = [&](double Xc) { return vofr(Xc, Yc, Zc); };
     const double X0 = (double)((float)xb0 - (float)t0.ox);                  // X of the wave's first voxel
     const double X1 = (double)((float)(xb0 + GL_Q * 64 - 1) - (float)t0.ox);  // and of its last
     const int rlo = min(max((int)fmin(vof(X0), vof(X1)), 0), Y - 2);
-    unsigned orow[NR];  // staged rows rlo .. rlo + NR - 1 (clamped into the projection)
+    // the group's rows (SH): Yc and Zc of its wave k, and the least row of all
+    double Ycg[SH ? 4 : 1], Zcg[SH ? 4 : 1];
+    int rlo_g = rlo;
+    if constexpr (SH) {
 #pragma unroll
-    for (int k = 0; k < NR; ++k) orow[k] = (unsigned)min(rlo + k, Y - 1) * (unsigned)X;
+        for (int k = 0; k < 4; ++k) {
+            const unsigned rk = min(row0 + k, rows - 1);
+            Ycg[k] = (double)((float)(int)(rk / g.T) - (float)t0.oy);
+            Zcg[k] = (double)((float)(int)(rk % g.T) - (float)t0.oz);
+            rlo_g = min(rlo_g, min(max((int)fmin(vofr(X0, Ycg[k], Zcg[k]), vofr(X1, Ycg[k], Zcg[k])), 0), Y - 2));
+        }
+    }
+    unsigned orow[NRS];  // staged rows rlo_g .. rlo_g + NRS - 1 (clamped into the projection)
+#pragma unroll
+    for (int k = 0; k < NRS; ++k) orow[k] = (unsigned)min(rlo_g + k, Y - 1) * (unsigned)X;
     // per voxel: the staged offset of row r0 (0 with NR = 2) and the weights
     // of r0, r0 + 1, which the tilt loop does not need: they are evaluated
     // again at the end.  BPT and SART update the volume per tilt and keep it
@@ -881,7 +911,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
     for (int q = 0; q < GL_Q; ++q) {
         float w0, w1;
         const int r0 = row_weights(q, w0, w1);
-        ro[q] = NR == 2 ? 0 : min(max(r0 - rlo, 0), NR - 2) * GL_PITCH;
+        ro[q] = (rlo - rlo_g + (NR == 2 ? 0 : min(max(r0 - rlo, 0), NR - 2))) * PITCH;
         if constexpr (PER_TILT) {
             const int x = xb0 + lane + 64 * q;
             wy0[q] = w0;
@@ -892,23 +922,26 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
         h1[q] = 0.f;
         hw[q] = 0.f;
     }
-    float *lds = stage[wave];
     // window of tilt th: u = ub + a1 X, its first column, and the staged
     // values of columns cmin + lane + 64 m.
-    static_assert(GL_W <= GL_PITCH, "staged columns fit a row");
+    static_assert(GL_W <= GL_PITCH && GL_W + 12 <= GL_WS, "staged columns fit a row");
     // the window is staged in chunks of 4 columns, at most GC per row, of
-    // which lane l loads chunks l, l + 64, ...
+    // which lane l loads chunks l, l + 64, ... (SH: the group's window, at
+    // most GCS chunks per row, of which thread t loads chunks t, t + WG, ...
+    // counted row by row)
     constexpr int GC = GL_W / 4, GK = (GC + 63) / 64;
-    static_assert(GL_W % 4 == 0 && GL_PITCH % 4 == 0, "16-byte chunks");
-    float4 sv[NR][GK];
+    constexpr int GCS = GL_WS / 4, GKS = (NRS * GCS + WG - 1) / WG;
+    static_assert(GL_W % 4 == 0 && GL_PITCH % 4 == 0 && GL_WS % 4 == 0, "16-byte chunks");
+    float4 sv[SH ? 1 : NR][SH ? GKS : GK];
     // Constants of tilts tb .. tb + 63, lane l holding tilt tb + l: u = ub +
     // a1 X, the window's first column cmin, whether every voxel of the wave
     // lies inside the detector (1 <= u <= X - 3), 64 a1 split into its
     // integer and fractional parts, and the chunks the taps can reach
     // (columns up to (64 GL_Q - 1) |a1| + 3 past cmin, with a chunk of
     // margin).  Each tilt reads them with v_readlane.
+    // SH: also the group window's first column cu and its chunks per row.
     double ub_v = 0.0, a1_v = 0.0;
-    int cmin_v = 0, in_v = 0, ia_v = 0, nch_v = 0;
+    int cmin_v = 0, in_v = 0, ia_v = 0, nch_v = 0, cu_v = 0, nchu_v = 0;
     float fa_v = 0.f;
     auto tilt_block = [&](int tb) {
         const TiltGeneral &t = g.tg[min(tb + lane, theta0 + ntheta - 1)];
@@ -921,6 +954,19 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
         ia_v = (int)fA_;
         fa_v = (float)(A - fA_);
         nch_v = min(GC, (int)(((64.0 * GL_Q - 1.0) * fabs(a1_v) + 4.0) * 0.25) + 2);
+        if constexpr (SH) {
+            int lo = X, hi = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double ubk = t.a[0] + t.a[2] * Ycg[k] + t.a[3] * Zcg[k] + t.ox;
+                const double v0 = __fma_rn(a1_v, X0, ubk), v1 = __fma_rn(a1_v, X1, ubk);
+                const int ck = min(max((int)fmin(v0, v1), 0), X - 2);
+                lo = min(lo, ck);
+                hi = max(hi, ck);
+            }
+            cu_v = lo;
+            nchu_v = min(GCS, (hi - lo + 3) / 4 + nch_v);
+        }
     };
     auto lane_d = [](double v, int l) {
         const unsigned long long b = (unsigned long long)__double_as_longlong(v);
@@ -934,6 +980,19 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
     auto load = [&](int th, int cnext, int nch) {
         const auto rs = __builtin_amdgcn_make_buffer_rsrc((void *)(img + (size_t)th * Y * X), (short)0,
                                                           (int)((unsigned)Y * (unsigned)X * 4u), GL_BUF_DWORD3);
+        if constexpr (SH) {
+#pragma unroll
+            for (int m = 0; m < GKS; ++m) {
+                const int idx = (int)threadIdx.x + WG * m, r = idx / GCS, c = idx % GCS;
+                if (r < NRS && c < nch) {
+                    const auto v = __builtin_amdgcn_raw_buffer_load_b128(
+                        rs, (int)((orow[min(r, NRS - 1)] + (unsigned)(cnext + 4 * c)) * 4u), 0, 0);
+                    sv[0][m] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                                           __uint_as_float(v[3]));
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const int soff = (int)((orow[k] + (unsigned)cnext) * 4u);
@@ -947,9 +1006,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
             }
         }
     };
+    // the window a load stages: the wave's own, or the group's (SH)
+    auto wcol = [&](int l) { return __builtin_amdgcn_readlane(SH ? cu_v : cmin_v, l); };
+    auto wch = [&](int l) { return __builtin_amdgcn_readlane(SH ? nchu_v : nch_v, l); };
     int tb = theta0;
     tilt_block(tb);
-    load(theta0, __builtin_amdgcn_readlane(cmin_v, 0), __builtin_amdgcn_readlane(nch_v, 0));
+    load(theta0, wcol(0), wch(0));
     int nin = 0;  // SIRT and ADMM: tilts in which every voxel of the wave lies inside
     for (int th = theta0; th < theta0 + ntheta; ++th) {
         const int tl = th - tb;
@@ -957,23 +1019,38 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
         const bool inside = __builtin_amdgcn_readlane(in_v, tl) != 0;
         const int IA = __builtin_amdgcn_readlane(ia_v, tl);
         const float fA = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fa_v), tl));
-        const int nch = __builtin_amdgcn_readlane(nch_v, tl);
+        const int nch = wch(tl);
         const double ub = lane_d(ub_v, tl), a1 = lane_d(a1_v, tl);
-        __builtin_amdgcn_wave_barrier();
+        float *lds;
+        if constexpr (SH) {
+            // buffer (th - theta0) & 1: the group finished reading it two
+            // tilts ago, before the barrier of the last tilt
+            float *buf = stage[(th - theta0) & 1];
 #pragma unroll
-        for (int m = 0; m < GK; ++m) {
-            if (lane + 64 * m < nch) {
-#pragma unroll
-                for (int k = 0; k < NR; ++k) *reinterpret_cast<float4 *>(lds + k * GL_PITCH + 4 * (lane + 64 * m)) = sv[k][m];
+            for (int m = 0; m < GKS; ++m) {
+                const int idx = (int)threadIdx.x + WG * m, r = idx / GCS, c = idx % GCS;
+                if (r < NRS && c < nch) *reinterpret_cast<float4 *>(buf + r * PITCH + 4 * c) = sv[0][m];
             }
+            __syncthreads();  // the group's window of tilt th is staged
+            lds = buf + (cmin - wcol(tl));
+        } else {
+            lds = stage[wave];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int m = 0; m < GK; ++m) {
+                if (lane + 64 * m < nch) {
+#pragma unroll
+                    for (int k = 0; k < NR; ++k) *reinterpret_cast<float4 *>(lds + k * PITCH + 4 * (lane + 64 * m)) = sv[k][m];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
         if (th + 1 < theta0 + ntheta) {
             if (th + 1 - tb == 64) {
                 tb = th + 1;
                 tilt_block(tb);
             }
-            load(th + 1, __builtin_amdgcn_readlane(cmin_v, th + 1 - tb), __builtin_amdgcn_readlane(nch_v, th + 1 - tb));
+            load(th + 1, wcol(th + 1 - tb), wch(th + 1 - tb));
         }
         // u - cmin = ubr + a1 X; the staged column li = c0 - cmin of tap c0 =
         // min(max(trunc u, 0), X - 2), which is at most (64 GL_Q - 1) |a1| + 1
@@ -1005,17 +1082,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
                 const float *l0 = lds + ro[q] + li;
                 if constexpr (!PER_TILT) {
                     h0[q] = fmaf(w1, l0[1], fmaf(w0, l0[0], h0[q]));
-                    h1[q] = fmaf(w1, l0[GL_PITCH + 1], fmaf(w0, l0[GL_PITCH], h1[q]));
+                    h1[q] = fmaf(w1, l0[PITCH + 1], fmaf(w0, l0[PITCH], h1[q]));
                     if (EDGE) hw[q] += w0 + w1;
                 } else {
                     const float s = wy0[q] * fmaf(w1, l0[1], w0 * l0[0]) +
-                                    wy1[q] * fmaf(w1, l0[GL_PITCH + 1], w0 * l0[GL_PITCH]);
+                                    wy1[q] * fmaf(w1, l0[PITCH + 1], w0 * l0[PITCH]);
                     const float c = (w0 + w1) * (wy0[q] + wy1[q]);
                     if (c > g.thr_upd) out[q] += (MODE == BACK_SART) ? (s / c) * g.gamma : s / c;
                 }
             }
         };
-        if (inside) {
+        if (!active) {
+        } else if (inside) {
             voxels(std::false_type{});
             ++nin;
         } else {
@@ -1023,6 +1101,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
         }
         __builtin_amdgcn_wave_barrier();
     }
+    if (!active) return;
 #pragma unroll
     for (int q = 0; q < GL_Q; ++q) hw[q] += (float)nin;
 #pragma unroll
@@ -1080,18 +1159,26 @@ void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, bool
     }
 }
 
-template <int NR>
-void launch_back_lds(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img, dim3 grid,
-                     hipStream_t st)
+template <int NR, bool SH>
+void launch_back_lds_sh(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img, dim3 grid,
+                        hipStream_t st)
 {
     if (mode == BACK_SIRT)
-        hipLaunchKernelGGL((k_general_back_lds<BACK_SIRT, NR>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        hipLaunchKernelGGL((k_general_back_lds<BACK_SIRT, NR, SH>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
     else if (mode == BACK_LIN)
-        hipLaunchKernelGGL((k_general_back_lds<BACK_LIN, NR>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        hipLaunchKernelGGL((k_general_back_lds<BACK_LIN, NR, SH>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
     else if (mode == BACK_SART)
-        hipLaunchKernelGGL((k_general_back_lds<BACK_SART, NR>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        hipLaunchKernelGGL((k_general_back_lds<BACK_SART, NR, SH>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
     else
-        hipLaunchKernelGGL((k_general_back_lds<BACK_BPT, NR>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+        hipLaunchKernelGGL((k_general_back_lds<BACK_BPT, NR, SH>), grid, dim3(WG), 0, st, g, theta0, ntheta, img);
+}
+
+template <int NR>
+void launch_back_lds(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img, dim3 grid, bool sh,
+                     hipStream_t st)
+{
+    if (sh) launch_back_lds_sh<NR, true>(g, mode, theta0, ntheta, img, grid, st);
+    else launch_back_lds_sh<NR, false>(g, mode, theta0, ntheta, img, grid, st);
 }
 
 }  // namespace
@@ -1235,12 +1322,18 @@ hipError_t launch_general_back(const GeneralJob &g, const TiltGeneral *host_tg, 
         nr = host_tg[theta0].b[1] == 0.0 ? 2 : (span < 0.999 ? 3 : (span < 1.999 ? 4 : 0));
     }
     const size_t nblk_lds = cdiv((size_t)g.X, GL_Q * 64) * nrows_wg;
+    // the group's four waves share their window when they take four layers
+    // of one volume row whose rows differ by less than one (3 |b3| < 1)
+    // (TREC_GENERAL_BACK=wave: every wave stages its own)
+    const char *force = getenv("TREC_GENERAL_BACK");
+    const bool sh = nr && g.layout == TREC_LAYOUT_Y && g.T % 4 == 0 && 3.0 * std::fabs(host_tg[theta0].b[3]) < 1.0 &&
+                    !(force && !strcmp(force, "wave"));
     if (nr && nblk_lds <= 0x7fffffffu) {
         const dim3 grid((unsigned)nblk_lds);
         switch (nr) {
-        case 2: launch_back_lds<2>(g, mode, theta0, ntheta, img, grid, st); break;
-        case 3: launch_back_lds<3>(g, mode, theta0, ntheta, img, grid, st); break;
-        default: launch_back_lds<4>(g, mode, theta0, ntheta, img, grid, st); break;
+        case 2: launch_back_lds<2>(g, mode, theta0, ntheta, img, grid, sh, st); break;
+        case 3: launch_back_lds<3>(g, mode, theta0, ntheta, img, grid, sh, st); break;
+        default: launch_back_lds<4>(g, mode, theta0, ntheta, img, grid, sh, st); break;
         }
         return hipGetLastError();
     }

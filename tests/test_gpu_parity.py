@@ -500,3 +500,28 @@ def test_pitch_offset_paths(torch_cuda, trec, ctx, orc, method, pitch, offset):
     assert report(f"pitch {pitch} offset {offset} (b1 span {511 * b1:.2f} rows) {method}", got, ref) < TOL_F32
     again = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset)
     np.testing.assert_array_equal(again, got)
+
+
+# The back-projection's group-shared window: the four waves of a work-group
+# take four layers of one volume row (layout Y, T a multiple of 4) and stage
+# the union of their windows once per tilt.  It must give the bits of every
+# wave staging its own window (TREC_GENERAL_BACK=wave), and the oracle's
+# values.  Two, three and four staged rows per wave (offset 0, |b1| spans of
+# 0.93 and 1.87 rows); X = 600 leaves a partial tile of 512 voxels per row.
+@pytest.mark.parametrize("method", ["SIRT,3,1.0", "SART,1,0.8", "BPT"])
+@pytest.mark.parametrize("pitch,offset", [(0.5, 0.0), (15.0, 0.0), (3.0, 2.0), (4.0, 3.0)])
+def test_general_back_shared_window(torch_cuda, trec, ctx, orc, monkeypatch, method, pitch, offset):
+    P, Y, T, X = 7, 21, 12, 600
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=43, pitch=pitch, offset=offset)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset)
+    name = method.split(",")[0]
+    if name == "BPT":
+        ref = orc.bpt_f32(coef, stack, T, org)
+    elif name == "SIRT":
+        ref = orc.sirt_f32(coef, stack, T, org, 3, 1.0)
+    else:
+        ref = orc.sart_f32(coef, stack, T, org, 1, 0.8)
+    assert report(f"shared window pitch {pitch} offset {offset} {method}", got, ref) < TOL_F32
+    monkeypatch.setenv("TREC_GENERAL_BACK", "wave")
+    own = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset)
+    np.testing.assert_array_equal(got, own)
