@@ -584,11 +584,12 @@ __global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int th
 
 // Forward projection as the reference computes it, a bilinear splat of
 // every voxel (Reproject, main-mpi.cpp:286-344), for tilts with a1 >= 1/3
-// and 0.55 <= b2 <= 1: any pitch with or without an offset.  A wave owns a
-// tile of SP_R detector rows by t.si columns of one tilt and sums it in LDS
-// over its z layers.  Per layer it walks the box of voxels whose taps can
-// reach the tile, one volume row of up to 64 voxels per pass (one per lane,
-// coalesced loads), and every voxel adds its four weighted taps to the
+// and 0.55 <= b2 <= 1: any pitch with or without an offset.  A wave owns two
+// neighbouring tiles of SP_R detector rows by t.si columns of one tilt,
+// lanes 0-31 the first and lanes 32-63 the second, and sums them in LDS over
+// its z layers.  Per layer each half walks the box of voxels whose taps can
+// reach its tile, one volume row of up to 32 ncp voxels per pass (loads
+// coalesced per half), and every voxel adds its four weighted taps to the
 // tile.  A tap outside the tile lands in padding that is never stored (the
 // tap position is clamped into the padded tile, which moves no voxel onto a
 // stored cell), so the box needs no exact edge.  The voxel's position is
@@ -596,12 +597,13 @@ __global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int th
 // each lane's column and advanced in float over the rows; its weights are
 // the reference's (1 - xd, xd) x (1 - yd, yd).
 //
-// No atomics: the lanes of a pass add their taps in t.ncp phases, lane %
-// ncp, and the voxels of one phase lie ncp columns apart, at least 1 + 1e-4
-// pixels in u, so within one tap's read-add-write no two lanes of a phase
-// touch the same cell.  Every cell therefore receives its adds in program
-// order (layers, rows, phases, taps) on every run.  (LDS float atomics would
-// need no phases, but gfx950 runs them lane by lane, some 30 times slower.)
+// No atomics: a pass takes its voxels in t.ncp phases, and in phase p lane l
+// of a half holds voxel ncp l + p, so the voxels of one read-add-write lie
+// ncp columns apart, at least 1 + 1e-4 pixels in u, and no two lanes of a
+// half touch the same cell; every lane is busy in every phase.  Every cell
+// therefore receives its adds in program order (layers, rows, phases, taps)
+// on every run.  (LDS float atomics would need no phases, but gfx950 runs
+// them lane by lane, some 30 times slower.)
 constexpr int SP_R = 16;      // detector rows per tile (SR): 8, 24 and 32 were slower
 constexpr int SP_CP = 6;      // padding columns on each side of a tile row
 constexpr int SP_RP = 2;      // padding rows above and below
@@ -613,12 +615,12 @@ __global__ __launch_bounds__(64) void k_general_splat(const GeneralJob g, int th
                                                       unsigned ntile)
 {
     constexpr int SP_ROWS = SR + 2 * SP_RP;
-    __shared__ float acc[SP_ROWS * SP_PITCH];
+    __shared__ float acc[2][SP_ROWS * SP_PITCH];
     const unsigned w = xcd_order(blockIdx.x, gridDim.x);
     const unsigned tw = w % ntile, rest = w / ntile;
     const unsigned nrb = (g.Y + SR - 1) / SR;
     const int rb = (int)(rest % nrb), zc = (int)(rest / nrb);
-    // the tilt of tile tw: the last whose first tile is at most tw
+    // the tilt of tile pair tw: the last whose first pair is at most tw
     const int base = g.tg[theta0].tile0;
     int lo = theta0, hi = theta0 + ntheta - 1;
     while (lo < hi) {
@@ -629,13 +631,16 @@ __global__ __launch_bounds__(64) void k_general_splat(const GeneralJob g, int th
     const int th = lo;
     const TiltGeneral &t = g.tg[th];
     const int si = t.si, ncp = t.ncp;
-    const int i0 = (int)(tw - (unsigned)(t.tile0 - base)) * si, i1 = min(g.X, i0 + si);
+    // lanes 0-31 take the pair's first tile, lanes 32-63 its second
+    const int lane = (int)threadIdx.x, half = lane >> 5, hl = lane & 31;
+    const int ia = (int)(2 * (tw - (unsigned)(t.tile0 - base))) * si;  // the pair's first column
+    const int i0 = ia + half * si, i1 = min(g.X, i0 + si);              // this lane's tile
     const int j0 = rb * SR, j1 = min(g.Y, j0 + SR);
     const int zlen = (g.T + nzc - 1) / nzc;
     const int z0 = zc * zlen, z1 = min(g.T, z0 + zlen);
-    const int lane = (int)threadIdx.x;
-    for (int k = lane; k < SP_ROWS * SP_PITCH; k += 64) acc[k] = 0.f;
+    for (int k = lane; k < 2 * SP_ROWS * SP_PITCH; k += 64) (&acc[0][0])[k] = 0.f;
     __syncthreads();
+    float *tile = acc[half];
 
     const float oxf = (float)t.ox, oyf = (float)t.oy, ozf = (float)t.oz;
     const double oxd = oxf, oyd = oyf;
@@ -646,75 +651,76 @@ __global__ __launch_bounds__(64) void k_general_splat(const GeneralJob g, int th
     const double bl = fmin(bxa, bxb), bh = fmax(bxa, bxb);
     const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
     const size_t sz = g.layout == TREC_LAYOUT_Y ? (size_t)g.X : (size_t)g.Y * g.X;
-    const int ph = lane % ncp;
     for (int z = z0; z < z1; ++z) {
         const double Zc = (double)((float)z - ozf);
         const double cu = t.a[0] + t.a[3] * Zc + oxd;  // u = cu + a1 X + a2 Y
         const double cv = t.b[0] + t.b[3] * Zc + oyd;  // v = cv + b1 X + b2 Y
         // rows whose v can fall in [j0 - 1, j1), columns whose u can fall in
-        // [i0 - 1, i1) on those rows.  A voxel the rounding of these bounds
-        // leaves out lies within rounding of the tile's edge, where its
-        // weight on the tile is of that order.
+        // [i0 - 1, i1) on those rows (per half).  A voxel the rounding of
+        // these bounds leaves out lies within rounding of the tile's edge,
+        // where its weight on the tile is of that order.
         const int ya = __builtin_amdgcn_readfirstlane(max((int)floor(((double)(j0 - 1) - cv - bh) / b2 + oyd), 0));
         const int yb = __builtin_amdgcn_readfirstlane(min((int)floor(((double)j1 - cv - bl) / b2 + oyd) + 1, g.Y));
         if (ya >= yb) continue;
         const double ya2 = a2 * (double)((float)ya - oyf), yb2 = a2 * (double)((float)(yb - 1) - oyf);
         const double al = fmin(ya2, yb2), ah = fmax(ya2, yb2);
-        const int xa = __builtin_amdgcn_readfirstlane(max((int)floor(((double)(i0 - 1) - cu - ah) / a1 + oxd), 0));
-        const int xb = __builtin_amdgcn_readfirstlane(min((int)floor(((double)i1 - cu - al) / a1 + oxd) + 1, g.X));
+        const int xa = max((int)floor(((double)(i0 - 1) - cu - ah) / a1 + oxd), 0);
+        const int xb = i0 < g.X ? min((int)floor(((double)i1 - cu - al) / a1 + oxd) + 1, g.X) : xa;
+        // passes of 32 ncp voxels per half, as many as the wider box needs
+        const int need = (max(xb - xa, 0) + 32 * ncp - 1) / (32 * ncp);
+        const int npass = max(__builtin_amdgcn_readlane(need, 0), __builtin_amdgcn_readlane(need, 32));
         const int nrow = yb - ya;
         const double Ya = (double)((float)ya - oyf);
         const float *layer = g.vol + (size_t)z * sz + (size_t)ya * sy;
-        for (int xs = xa; xs < xb; xs += 64) {
-            const int x = xs + lane;
-            const bool in = x < xb;
-            const double Xc = (double)((float)x - oxf);
-            const double U0 = cu + a1 * Xc + a2 * Ya, V0 = cv + b1 * Xc + b2 * Ya;
-            const double fU = floor(U0), fV = floor(V0);
-            const float fu0 = (float)(U0 - fU), fv0 = (float)(V0 - fV);
-            // LDS column and row of the first row's upper-left tap
-            const int cu0 = (int)fU - i0 + SP_CP, rv0 = (int)fV - j0 + SP_RP;
-            // row bases are uniform (scalar registers); a load adds the lane's column
-            const unsigned xoff = (unsigned)min(x, g.X - 1) * 4u;
-            for (int k0 = 0; k0 < nrow; k0 += 4) {
-                float vals[4];
+        for (int ps = 0; ps < npass; ++ps) {
+            // phase p: voxel xa + 32 ncp ps + ncp hl + p of each half; the
+            // voxels of a half lie ncp apart, at least 1 + 1e-4 pixels in u
+            for (int p = 0; p < ncp; ++p) {
+                const int x = xa + 32 * ncp * ps + ncp * hl + p;
+                const bool in = x < xb;
+                const double Xc = (double)((float)x - oxf);
+                const double U0 = cu + a1 * Xc + a2 * Ya, V0 = cv + b1 * Xc + b2 * Ya;
+                const double fU = floor(U0), fV = floor(V0);
+                const float fu0 = (float)(U0 - fU), fv0 = (float)(V0 - fV);
+                // LDS column and row of the first row's upper-left tap
+                const int cu0 = (int)fU - i0 + SP_CP, rv0 = (int)fV - j0 + SP_RP;
+                // row bases are uniform (scalar registers); a load adds the lane's column
+                const unsigned xoff = (unsigned)min(x, g.X - 1) * 4u;
+                for (int k0 = 0; k0 < nrow; k0 += 4) {
+                    float vals[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    vals[q] = (ONES || k0 + q >= nrow)
-                                  ? 1.f
-                                  : *reinterpret_cast<const float *>(
-                                        reinterpret_cast<const char *>(layer + (size_t)(k0 + q) * sy) + xoff);
-                if (!in) continue;  // lanes past the box (the whole row group)
+                    for (int q = 0; q < 4; ++q)
+                        vals[q] = (ONES || k0 + q >= nrow)
+                                      ? 1.f
+                                      : *reinterpret_cast<const float *>(
+                                            reinterpret_cast<const char *>(layer + (size_t)(k0 + q) * sy) + xoff);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int k = k0 + q;
-                    if (k >= nrow) break;
-                    const float val = vals[q];
-                    const float fu = fmaf(a2f, (float)k, fu0), fv = fmaf(-c2f, (float)k, fv0);
-                    const float flu = floorf(fu), flv = floorf(fv);
-                    const float fx = fu - flu, fy = fv - flv;
-                    const int cc = min(max(cu0 + (int)flu, 0), SP_PITCH - 2);
-                    const int rr = min(max(rv0 + k + (int)flv, 0), SP_ROWS - 2);
-                    float *cell = acc + rr * SP_PITCH + cc;
-                    const float t1 = val * fy, t0 = val - t1;
-                    const float w01 = t0 * fx, w00 = t0 - w01, w11 = t1 * fx, w10 = t1 - w11;
-#pragma unroll
-                    for (int p = 0; p < 4; ++p) {
-                        if (p < ncp && ph == p) {
-                            // the left taps of both rows, then the right
-                            // ones: within each group the lanes of a phase
-                            // hit distinct cells, and each group's reads see
-                            // the writes before them (a wave's LDS accesses
-                            // run in order; the wave barriers keep the
-                            // compiler from merging the groups, as lane x's
-                            // right tap may be lane x + ncp's left one)
-                            cell[0] += w00;
-                            cell[SP_PITCH] += w10;
-                            __builtin_amdgcn_wave_barrier();
-                            cell[1] += w01;
-                            cell[SP_PITCH + 1] += w11;
-                            __builtin_amdgcn_wave_barrier();
-                        }
+                    for (int q = 0; q < 4; ++q) {
+                        const int k = k0 + q;
+                        if (k >= nrow) break;
+                        // lanes past their box add nothing
+                        const float val = in ? vals[q] : 0.f;
+                        const float fu = fmaf(a2f, (float)k, fu0), fv = fmaf(-c2f, (float)k, fv0);
+                        const float flu = floorf(fu), flv = floorf(fv);
+                        const float fx = fu - flu, fy = fv - flv;
+                        const int cc = min(max(cu0 + (int)flu, 0), SP_PITCH - 2);
+                        const int rr = min(max(rv0 + k + (int)flv, 0), SP_ROWS - 2);
+                        float *cell = tile + rr * SP_PITCH + cc;
+                        const float t1 = val * fy, t0 = val - t1;
+                        const float w01 = t0 * fx, w00 = t0 - w01, w11 = t1 * fx, w10 = t1 - w11;
+                        // the left taps of both rows, then the right ones:
+                        // within each group the lanes of a half hit distinct
+                        // cells of their own tile, and each group's reads see
+                        // the writes before them (a wave's LDS accesses run in
+                        // order; the wave barriers keep the compiler from
+                        // merging the groups, as lane x's right tap may be
+                        // lane x + 1's left one)
+                        cell[0] += w00;
+                        cell[SP_PITCH] += w10;
+                        __builtin_amdgcn_wave_barrier();
+                        cell[1] += w01;
+                        cell[SP_PITCH + 1] += w11;
+                        __builtin_amdgcn_wave_barrier();
                     }
                 }
             }
@@ -723,9 +729,9 @@ __global__ __launch_bounds__(64) void k_general_splat(const GeneralJob g, int th
     __syncthreads();
     float *out = nzc > 1 ? g.part + ((size_t)zc * ntheta + (th - theta0)) * g.Y * g.X
                          : (ONES ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
-    if (lane < i1 - i0)
-        for (int r = 0; r < j1 - j0; ++r)
-            out[(size_t)(j0 + r) * g.X + i0 + lane] = acc[(r + SP_RP) * SP_PITCH + SP_CP + lane];
+    // each half stores its tile, 32 columns per row at a time
+    for (int c = hl; c < i1 - i0; c += 32)
+        for (int r = 0; r < j1 - j0; ++r) out[(size_t)(j0 + r) * g.X + i0 + c] = tile[(r + SP_RP) * SP_PITCH + SP_CP + c];
 }
 
 // Back-projection for tilts that share b and the origins.  Each thread takes
@@ -1209,7 +1215,7 @@ bool general_tiles(TiltGeneral *tg, int P, int X)
         while (a1 * ncp < 1.0 + 1e-4) ++ncp;
         t.si = si;
         t.ncp = ncp;
-        t.ntx = (X + si - 1) / si;
+        t.ntx = (X + 2 * si - 1) / (2 * si);  // pairs of tiles: a wave takes two
         tile += t.ntx;
     }
     return all;

@@ -79,9 +79,9 @@ struct TiltGeneral {
     double ox, oy, oz;
     // tiles of the splat forward projection (k_general_splat), set by
     // general_tiles: si detector columns per tile, ncp lane phases whose
-    // voxels are at least one pixel apart, ntx tiles across the detector,
-    // tile0 the tiles of the tilts before this one (0 for every tilt the
-    // splat cannot take)
+    // voxels are at least one pixel apart, ntx pairs of tiles across the
+    // detector (a wave takes a pair), tile0 the pairs of the tilts before
+    // this one (0 for every tilt the splat cannot take)
     int si, ncp, ntx, tile0;
 };
 // Fills the splat tiles of P tilts (host); false when some tilt needs
