@@ -251,6 +251,8 @@ def serve(trec, reqs, arrivals, storage, max_active, half_io=False):
         dt = ta - (time.perf_counter() - t0)
         if dt > 0:
             time.sleep(dt)
+        if k and k % 50 == 0:
+            log(f"[bench] serving: {k} of {len(reqs)} submitted at {time.perf_counter() - t0:.1f} s")
         out = pool[k % len(pool)][:r["X"] * r["T"] * r["X"]]
         st = stacks[id(r["stack"])] if half_in(r) else r["stack"]
         ids.append(srv.submit(trec.make_geom(r["X"], r["X"], r["P"], r["T"]),
@@ -262,7 +264,7 @@ def serve(trec, reqs, arrivals, storage, max_active, half_io=False):
     return lat, span
 
 
-def serving_form(trec, torch, n, rate, seed=1004, active=4, saturated_only=False):
+def serving_form(trec, torch, n, rate, seed=1004, active=4, saturated_only=False, poisson_only=False, arms_only=None):
     """The serving configuration: n requests of the batching mix, Poisson
     arrivals at `rate` per second and all at once (saturated).  Every arm
     runs through the same server and copy machinery (pinned double-buffered
@@ -274,7 +276,8 @@ def serving_form(trec, torch, n, rate, seed=1004, active=4, saturated_only=False
       batched_f32     continuous batching, FP32 storage (saturated only).
     Up to `active` (4) requests at once: most requests of the mix fill the
     GPU alone, so more at once only dilute the L2.
-    Latency is from submission to the volume in host memory."""
+    Latency is from submission to the volume in host memory.  saturated_only
+    and poisson_only keep one load; arms_only (names) keeps those arms."""
     import numpy as np
     reqs = serving_mix(n, seed)
     gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, n)
@@ -286,18 +289,21 @@ def serving_form(trec, torch, n, rate, seed=1004, active=4, saturated_only=False
     arms = [("plain_f32", trec.STORE_F32, 1, False), ("batched_f16", trec.STORE_F16, active, False),
             ("batched_f16_half_io", trec.STORE_F16, active, True)]
     out["max_active"] = active
-    for mode in (("saturated",) if saturated_only else ("poisson", "saturated")):
+    modes = ("saturated",) if saturated_only else (("poisson",) if poisson_only else ("poisson", "saturated"))
+    for mode in modes:
         arr = arrivals if mode == "poisson" else [0.0] * n
         res = {}
         for name, sto, act, half in arms + ([("batched_f32", trec.STORE_F32, active, False)] if mode == "saturated"
                                             else []):
+            if arms_only and name not in arms_only:
+                continue
             log(f"[bench] serving ({mode}): {name}")
             lat, span = serve(trec, reqs, arr, sto, act, half)
             res[name] = stat(lat, span)
             res[name]["span_s"] = round(span, 3)
-        base = res["plain_f32"]["span_s"]
+        base = res["plain_f32"]["span_s"] if "plain_f32" in res else None
         for name in res:
-            if name != "plain_f32":
+            if name != "plain_f32" and base:
                 res[name]["throughput_vs_plain_f32"] = round(base / res[name]["span_s"], 3)
         out[mode] = res
         log(f"[bench] serving ({mode}): {res}")
@@ -612,6 +618,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-active", type=int, default=4, help="most active requests of the batched serving arms")
     ap.add_argument("--saturated-only", action="store_true", help="serving form: only the saturated load")
+    ap.add_argument("--poisson-only", action="store_true", help="serving form: only the Poisson load")
+    ap.add_argument("--serve-arms", default="", help="serving form: comma-separated arms to run (default all)")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving form (its own JSON line), e.g. with --serve-requests 120")
     ap.add_argument("--counters", action="store_true", help="only the run the counter profiles measure")
@@ -654,8 +662,10 @@ def main():
     if args.serving_only:
         del stack_d
         torch.cuda.empty_cache()
+        arms_only = [a for a in args.serve_arms.split(",") if a] or None
         print(json.dumps({"serving": serving_form(trec, torch, args.serve_requests, 2.0, active=args.serve_active,
-                                                  saturated_only=args.saturated_only)}), flush=True)
+                                                  saturated_only=args.saturated_only, poisson_only=args.poisson_only,
+                                                  arms_only=arms_only)}), flush=True)
         return
     if args.methods_only:
         print(json.dumps({"methods": methods_form(trec, ctx, torch, geom, ang, stack_d)}), flush=True)
