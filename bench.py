@@ -509,7 +509,7 @@ def profiled(kernel, column, summary="sirt1024_f32_counters.md", prefix=False):
 
 # MI355X: 256 CUs at 2.4 GHz, each issuing one 64-lane FP32 multiply-add per
 # cycle without packed instructions (half the published 157.3 TFLOP/s, which
-# counts packed FP32; this build emits none).  Mixed instruction streams can
+# counts packed FP32; the kernels here emit none).  Mixed instruction streams can
 # issue somewhat faster, so the fraction is a rate, not a hard ceiling.
 VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
 
@@ -569,9 +569,9 @@ def live_kernels(trec, ctx, torch, geom, ang, stack_d, method, vol, runs=2):
     return kern
 
 
-def north_star_form(trec, torch, iters=20, steps=2):
-    """The north-star shape: SIRT x20 of 61 x 2048^2 -> 2048^2 x 512 (the
-    precision configuration's series, scale x8 phantom), FP32 and FP16
+def sirt2048_form(trec, torch, iters=20, steps=2):
+    """SIRT x20 of 61 x 2048^2 -> 2048^2 x 512 (the precision
+    configuration's series, scale x8 phantom), FP32 and FP16
     storage, each with its step time and its dominant kernel's roofline."""
     from autom2_amd import synth
     c = synth.CONFIGS["sirt2048"]
@@ -595,7 +595,7 @@ def north_star_form(trec, torch, iters=20, steps=2):
         res[name] = {"ms_per_step": round(msm, 2), "value": updates / (msm / 1e3), "plan": plan,
                      "roofline": kernel_roofline(kern, plan, X, Y, T, P, sb,
                                                  "sirt2048_counters.md")}
-        log(f"[bench] north star {name}: {msm:.1f} ms/step, {updates / (msm / 1e3):.4e} updates/s, "
+        log(f"[bench] 2048 series {name}: {msm:.1f} ms/step, {updates / (msm / 1e3):.4e} updates/s, "
             f"forward {kern['forward']['ms_avg']:.2f} ms, back {kern['back']['ms_avg']:.2f} ms")
     res["f16_speedup_vs_f32"] = round(res["f32"]["ms_per_step"] / res["f16"]["ms_per_step"], 3)
     ctx.close()
@@ -612,7 +612,7 @@ def main():
     ap.add_argument("--config", default="sirt1024")
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-north", action="store_true", help="skip the 61 x 2048^2 -> 2048^2 x 512 form")
+    ap.add_argument("--no-2048", action="store_true", help="skip the 61 x 2048^2 -> 2048^2 x 512 form")
     ap.add_argument("--no-e2e", action="store_true", help="skip the end-to-end CLI run with MRC files")
     ap.add_argument("--no-forms", action="store_true", help="skip the FP16/BF16 and serving forms")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -724,11 +724,11 @@ def main():
         log(f"[bench] GeoParGen: {gp_res['ms_per_call']:.3f} ms/call, search {gp_res['kernels_ms']['gp_search']:.3f} ms, "
             f"exhaustive {gp_res['exhaustive_search_ms_per_call']:.1f} ms")
 
-    north = None
-    if not args.no_north:
+    s2048 = None
+    if not args.no_2048:
         vol = None
         torch.cuda.empty_cache()
-        north = north_star_form(trec, torch)
+        s2048 = sirt2048_form(trec, torch)
 
     e2e = None
     if not args.no_e2e:
@@ -770,7 +770,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "end_to_end": e2e,
-        "north_star": north,
+        "sirt2048": s2048,
         "forms": forms,
         "xf_apply": xf_res,
         "geopar": gp_res,

@@ -2,7 +2,7 @@
  * geopar_oracle.c -- CPU restatement of GeoParGen, the geometry estimate
  * AuTom2 runs on a trial reconstruction.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker, never the product:
+ * TEST INFRASTRUCTURE ONLY.  This file checks the product's results and is never part of it:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load the library built from it.  The HIP library in autom2_amd/ never
  * links, loads or falls back to it.

@@ -1,8 +1,8 @@
 """ctypes loader for the CPU oracle (oracle/lib/liborc.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, where it is the checker.  The product
-(autom2_amd) never imports this module.
+cpu_baseline leg of bench.py, where it checks results or is timed as the CPU
+baseline.  The product (autom2_amd) never imports this module.
 
 Array conventions follow TiltRec-mpi after RotateX
 (src/TiltRec/src/mrcmx/mrcstack.cpp:566-579): projections [P][rows][X],
