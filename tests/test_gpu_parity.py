@@ -184,18 +184,22 @@ def test_pitch_general_path(torch_cuda, trec, ctx, orc, method):
 
 
 # Offset 0 with a pitch: v does not depend on x, so the forward takes the
-# row-window kernel (16 detector rows per thread up to ~20 degrees of pitch,
-# 8 up to ~29, the per-pixel gather beyond) and the back-projection the
-# per-voxel row kernel.  X > 256 and Y not a multiple of the row block
-# exercise the partial work-groups; a 70 degree span needs 6 candidate
-# columns per row.
+# row-window kernels (three rows per slot: 16 detector rows per thread up to
+# ~19.2 degrees of pitch, 8 up to ~25.8, 4 up to ~33.6; the splat beyond) and
+# the back-projection the per-voxel row kernel.  X > 256 and Y not a multiple
+# of the row block exercise the partial work-groups; a 70 degree span needs
+# 6 candidate columns per row.  19.1/19.3, 25.8/25.9 and 33.5 sit on either
+# side of the rows-per-thread limits, (1 - cos pitch)(RJ + 2) <= 1.
 # A volume thicker than it is wide (T = 40, X = 48) projects its corners
 # past the detector's columns and, with the pitch, its end rows past its rows.
 @pytest.mark.parametrize("method,pitch,span,layout,shape", [
     ("SIRT,3,1.0", 0.5, 60.0, 0, (9, 37, 14, 300)), ("SIRT,3,1.0", 20.0, 70.0, 0, (9, 37, 14, 300)),
     ("SIRT,3,1.0", 25.0, 60.0, 1, (9, 37, 14, 300)), ("SIRT,3,1.0", 35.0, 60.0, 0, (9, 37, 14, 300)),
     ("SART,1,0.8", 4.0, 60.0, 0, (9, 37, 14, 300)), ("BPT", 12.0, 66.0, 1, (9, 37, 14, 300)),
-    ("SIRT,3,1.0", 20.0, 60.0, 0, (9, 21, 40, 48)), ("BPT", 8.0, 60.0, 1, (9, 21, 40, 48))])
+    ("SIRT,3,1.0", 20.0, 60.0, 0, (9, 21, 40, 48)), ("BPT", 8.0, 60.0, 1, (9, 21, 40, 48)),
+    ("SIRT,3,1.0", 19.1, 60.0, 0, (9, 37, 14, 300)), ("SIRT,3,1.0", 19.3, 60.0, 0, (9, 37, 14, 300)),
+    ("SIRT,3,1.0", 25.8, 60.0, 0, (9, 37, 14, 300)), ("SIRT,3,1.0", 25.9, 60.0, 0, (9, 37, 14, 300)),
+    ("SIRT,3,1.0", 33.5, 60.0, 0, (9, 37, 14, 300))])
 def test_pitch_rows_path(torch_cuda, trec, ctx, orc, method, pitch, span, layout, shape):
     P, Y, T, X = shape
     ang, coef, org, stack = case(orc, P, Y, T, X, span=span, seed=21, pitch=pitch)
