@@ -12,7 +12,11 @@
 //
 //   k_general_rows        forward projection when v does not depend on x
 //                         (offset 0 or pitch 0): the splat factors into an
-//                         x pass per volume row and a y pass over rows.
+//                         x pass per volume row and a y pass over rows
+//                         (k_general_rows3: three rows per slot).
+//   k_general_rows3b      the same row windows when v moves slowly along x
+//                         (a pitch with an offset), every voxel weighing its
+//                         rows by its own v.
 //   k_general_splat       forward projection of any pitch and offset (v may
 //                         depend on x): the reference's splat into a tile of
 //                         detector cells in LDS, owned by one wave.
@@ -573,6 +577,235 @@ __global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int th
         };
         if (mode == 2) slots(std::false_type{});
         else slots(std::true_type{});
+    }
+    if (i >= g.X) return;
+    float *out = nzc > 1 ? g.part + ((size_t)zc * ntheta + (th - theta0)) * g.Y * g.X
+                         : (ONES ? g.rwt : g.rval) + (size_t)th * g.Y * g.X;
+#pragma unroll
+    for (int r = 0; r < RJ; ++r)
+        if (j0 + r < g.Y) out[(size_t)(j0 + r) * g.X + i] = acc[r];
+}
+
+// The three-row windows of k_general_rows3 for a pitch with an offset.  Then
+// v = cv + b1 X + b2 Y moves along x (b1 = sin(pitch) sin(offset), the same
+// in every tilt), and each voxel weighs its detector rows by its own v:
+//
+//  - ylo is chosen at the wave's reference column xr (near the middle of its
+//    candidate columns): g0 = v(xr, ylo) - j0 lies in [lo, lo + b2), lo =
+//    -1 + e RJ + D (the launcher's dlo), where D bounds |b1 (x - xr)| over
+//    every candidate x of the wave.  The voxel at column x of slot k then has
+//    g = g0 + b1 (x - xr) - e k in [-1, 1] for k = -1 .. RJ while
+//    e RJ + 2 D <= 1, and reaches rows k - 1, k, k + 1 with weights
+//    max(0, -g), 1 - |g| and max(0, g).
+//  - With p_c the column-weighted voxel of candidate c and g_c = g_k + b1 c,
+//    rows k - 1, k, k + 1 take (A - G) / 2, S - A and (A + G) / 2 for S =
+//    sum p_c, A = sum p_c |g_c| and G = sum p_c g_c = g_k S + b1 S1,
+//    S1 = sum c p_c.  The candidates are summed last to first, so the partial
+//    sums of that chain add up to S1.  When g keeps its sign over a lane's
+//    candidates, A = +-G; a slot where some lane's candidates straddle g = 0
+//    forms A and G voxel by voxel in every lane of the wave.
+//  - A tilt takes 2 / a1 candidate columns rounded up (at least 3), chosen
+//    per work-group, so the tilts near 0 degrees do not pay for the steep
+//    ones.
+//  - Slots go in groups of 6, 5, 4 or 3 for 3, 4, 5 or 6 candidates (at
+//    most 20 loaded values, which keeps four waves per SIMD), each group's
+//    loads issued before any of its branches; a slot whose volume row lies
+//    outside the volume loads a clamped row and adds nothing.
+//
+// Candidate columns, their weights, the edge handling and the per-layer
+// constants are those of k_general_rows3.  Sums run over z, then slots in y
+// order.
+template <int RJ, bool ONES>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_general_rows3b(const GeneralJob g, int theta0, int ntheta, int nzc,
+                                                       double dlo)
+{
+    constexpr int NS = RJ + 2;  // slots k = -1 .. RJ at s = k + 1
+    constexpr int NCM = 6;      // most candidate columns (a1 >= 1/3)
+    const unsigned nib = (g.X + WG - 1) / WG;
+    const unsigned lin = xcd_order(blockIdx.x, gridDim.x);
+    const int th = theta0 + (int)(lin % (unsigned)ntheta);
+    const unsigned rest = lin / (unsigned)ntheta;
+    // pixels beyond the row repeat its last one and are not stored
+    const int i = (int)(rest % nib) * WG + threadIdx.x;
+    const int ic = min(i, g.X - 1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int iw = __builtin_amdgcn_readfirstlane(i - lane);  // the wave's first pixel
+    const int ia = min(iw, g.X - 1), ib = min(iw + 63, g.X - 1);
+    const unsigned njb = (g.Y + RJ - 1) / RJ;
+    const int j0 = (int)((rest / nib) % njb) * RJ;
+    const int zc = (int)(rest / nib / njb);
+    const int zlen = (g.T + nzc - 1) / nzc;
+    const int z0 = zc * zlen, z1 = min(g.T, z0 + zlen);
+    const TiltGeneral &t = g.tg[th];
+    const double a1 = t.a[1], a2 = t.a[2], b1 = t.b[1], b2 = t.b[2];
+    const double ra1 = 1.0 / a1, rb2 = 1.0 / b2;
+    const double ox = (float)t.ox, oy = (float)t.oy;
+    const float a1f = (float)a1, rkf = (float)(a2 * ra1), ef = (float)(1.0 - b2), b1f = (float)b1;
+    // the tilt's candidate count
+    const int nct = __builtin_amdgcn_readfirstlane(min(NCM, max(3, (int)ceil(2.0 / a1 - 1e-6))));
+    // how far the first candidate moves over the slots, plus rounding slack
+    const double drift = fabs(a2 * ra1) * (double)RJ + 0.01;
+    const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
+    const size_t sz = g.layout == TREC_LAYOUT_Y ? (size_t)g.X : (size_t)g.Y * g.X;
+    // constants of layers zb .. zb + 63, lane l holding layer zb + l: u - i
+    // at column x of slot k is W - i + a1 x + a2 k; mode 0 skips the layer,
+    // 1 takes the clamped windows, 2 the inner ones
+    int ylo_v = 0, mode_v = 0, xr_v = 0;
+    float g0_v = 0.f;
+    double w_v = 0.0;
+    auto layer_block = [&](int zb) {
+        const int z = min(zb + lane, z1 - 1);
+        const double Zc = (double)((float)z - (float)t.oz);
+        const double cu = t.a[0] + t.a[3] * Zc + t.ox;  // u = cu + a1 X + a2 Y
+        const double cv = t.b[0] + t.b[3] * Zc + t.oy;  // v = cv + b1 X + b2 Y
+        // the reference column: the wave's middle pixel on the rows near j0
+        const double xm = (0.5 * (double)(ia + ib) - cu - a2 * (((double)j0 - cv) * rb2)) * ra1 + ox;
+        xr_v = (int)fmin(fmax(floor(xm), -1e9), 1e9);
+        const double Xr = (double)((float)xr_v - (float)t.ox);
+        ylo_v = (int)fmin(fmax(ceil(((double)j0 + dlo - cv - b1 * Xr) * rb2 + oy), -1e9), 1e9);
+        const double Yc0 = (double)((float)ylo_v - (float)t.oy);
+        g0_v = (float)(cv + b1 * Xr + b2 * Yc0 - (double)j0);
+        w_v = cu + a2 * Yc0 - a1 * ox;
+        const double la = floor(((double)ia - 1.0 - w_v) * ra1 - drift) + 1.0;  // least first column
+        const double lb = floor(((double)ib - 1.0 - w_v) * ra1 + drift) + 1.0;  // greatest first column
+        const bool reach = lb + (double)nct >= 1.0 && la <= (double)(g.X - 1) && ylo_v + RJ >= 0 && ylo_v - 1 < g.Y;
+        const bool inner = la >= 0.0 && lb + (double)nct <= (double)g.X;
+        mode_v = reach ? (inner ? 2 : 1) : 0;
+    };
+    float acc[RJ];
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) acc[r] = 0.f;
+    int zb = z0;
+    layer_block(zb);
+    for (int z = z0; z < z1; ++z) {
+        if (z - zb == 64) {
+            zb = z;
+            layer_block(zb);
+        }
+        const int zl = z - zb;
+        const int mode = __builtin_amdgcn_readlane(mode_v, zl);
+        if (mode == 0) continue;
+        const int ylo = __builtin_amdgcn_readlane(ylo_v, zl);
+        const int xr = __builtin_amdgcn_readlane(xr_v, zl);
+        const float g0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(g0_v), zl));
+        const unsigned long long wb = (unsigned long long)__double_as_longlong(w_v);
+        const double W = __longlong_as_double(
+            (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(wb >> 32), zl) << 32) |
+                        (unsigned)__builtin_amdgcn_readlane((int)(unsigned)wb, zl)));
+        const double F0 = ((double)ic - 1.0 - W) * ra1;
+        const double X0d = floor(F0);
+        const int X1 = (int)X0d + 1;
+        const float f0 = (float)(F0 - X0d);
+        // g at column X1 of slot k = 0
+        const float gl = fmaf(b1f, (float)(X1 - xr), g0);
+        const float *vlayer = g.vol + (size_t)z * sz;
+        auto slots = [&](auto edge_tag, auto nc_tag) {
+            constexpr bool EDGE = decltype(edge_tag)::value;
+            constexpr int NC = decltype(nc_tag)::value;
+            // a1 (1 + c) - 1, 2 - a1 (1 + c), b1 c (formed here, so that only
+            // the variant running holds them)
+            float kp[NC], kn[NC], bc[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                kp[c] = fmaf(a1f, (float)(1 + c), -1.f);
+                kn[c] = fmaf(-a1f, (float)(1 + c), 2.f);
+                bc[c] = b1f * (float)c;
+            }
+            // groups of Q slots: every load of a group is issued before any
+            // of its branches (a slot outside the volume loads a clamped row)
+            constexpr int Q = NC <= 3 ? 6 : (NC == 4 ? 5 : (NC == 5 ? 4 : 3));  // at most 20 loaded values
+#pragma unroll
+            for (int s0 = 0; s0 < NS; s0 += Q) {
+                float v[Q][NC], fr[Q], gk[Q];
+                int xs[Q], xf[Q];
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int s = s0 + q, k = s - 1;
+                    if (s >= NS) break;
+                    const float tk = fmaf(-rkf, (float)k, f0);
+                    const float flt = floorf(tk);
+                    xf[q] = X1 + (int)flt;
+                    fr[q] = __builtin_amdgcn_fractf(tk);
+                    xs[q] = EDGE ? min(max(xf[q], 0), g.X - NC) : xf[q];
+                    // g of the slot's first column
+                    gk[q] = EDGE ? fmaf(b1f, (float)(xs[q] - X1), fmaf(-ef, (float)k, gl))
+                                 : fmaf(b1f, flt, fmaf(-ef, (float)k, gl));
+                    if constexpr (ONES) {
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) v[q][c] = 1.f;
+                    } else {
+                        const int yc = min(max(ylo + k, 0), g.Y - 1);
+                        const float *p = reinterpret_cast<const float *>(
+                            reinterpret_cast<const char *>(vlayer + (size_t)yc * sy) + (unsigned)xs[q] * 4u);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) v[q][c] = p[c];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int s = s0 + q, k = s - 1;
+                    if (s >= NS) break;
+                    if ((unsigned)(ylo + k) >= (unsigned)g.Y) continue;
+                    float w[NC];
+                    if constexpr (EDGE) {
+                        const float d0 = fmaf(a1f, (float)(xs[q] - xf[q]), fmaf(-a1f, fr[q], a1f - 1.f));
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) w[c] = clamp01(1.f - fabsf(fmaf(a1f, (float)c, d0)));
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            if (c == 0) w[c] = fmaf(-a1f, fr[q], a1f);
+                            else if (c == 1) w[c] = 1.f - fabsf(fmaf(-a1f, fr[q], kp[1]));
+                            else if (2 * c >= NC) w[c] = clamp01(fmaf(a1f, fr[q], kn[c]));
+                            else w[c] = clamp01(1.f - fabsf(fmaf(-a1f, fr[q], kp[c])));
+                        }
+                    }
+                    // S and S1 from the chain last candidate to first
+                    float S = 0.f, S1 = 0.f;
+#pragma unroll
+                    for (int c = NC - 1; c >= 0; --c) {
+                        S = c == NC - 1 ? w[c] * v[q][c] : fmaf(w[c], v[q][c], S);
+                        if (c >= 1) S1 = c == NC - 1 ? S : S1 + S;
+                    }
+                    const float G = fmaf(gk[q], S, b1f * S1);
+                    const bool pos = gk[q] >= 0.f;
+                    const bool straddle = pos != (gk[q] + bc[NC - 1] >= 0.f);
+                    float dn, dm, dp;  // rows k - 1, k, k + 1
+                    if (__builtin_amdgcn_ballot_w64(straddle) == 0) {
+                        const float tA = pos ? G : -G;
+                        dn = pos ? 0.f : tA;
+                        dm = S - tA;
+                        dp = pos ? G : 0.f;
+                    } else {
+                        float A = 0.f, Gx = 0.f;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            const float gc = gk[q] + bc[c];
+                            const float pc = w[c] * v[q][c];
+                            A = fmaf(pc, fabsf(gc), A);
+                            Gx = fmaf(pc, gc, Gx);
+                        }
+                        dn = 0.5f * (A - Gx);
+                        dm = S - A;
+                        dp = 0.5f * (A + Gx);
+                    }
+                    if (s >= 2 && s - 2 < RJ) acc[s - 2] += dn;
+                    if (s >= 1 && s - 1 < RJ) acc[s - 1] += dm;
+                    if (s < RJ) acc[s] += dp;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        };
+        auto by_nc = [&](auto edge_tag) {
+            switch (nct) {
+            case 3: slots(edge_tag, std::integral_constant<int, 3>{}); break;
+            case 4: slots(edge_tag, std::integral_constant<int, 4>{}); break;
+            case 5: slots(edge_tag, std::integral_constant<int, 5>{}); break;
+            default: slots(edge_tag, std::integral_constant<int, 6>{}); break;
+            }
+        };
+        if (mode == 2) by_nc(std::false_type{});
+        else by_nc(std::true_type{});
     }
     if (i >= g.X) return;
     float *out = nzc > 1 ? g.part + ((size_t)zc * ntheta + (th - theta0)) * g.Y * g.X
@@ -1165,6 +1398,35 @@ void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, bool
     }
 }
 
+template <int RJ>
+void launch_rows3b(const GeneralJob &g, int theta0, int ntheta, bool ones, int nzc, unsigned nblk, double dlo,
+                   hipStream_t st)
+{
+    if (ones)
+        hipLaunchKernelGGL((k_general_rows3b<RJ, true>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc, dlo);
+    else
+        hipLaunchKernelGGL((k_general_rows3b<RJ, false>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc, dlo);
+}
+
+// candidate columns a row window needs at a1: 2 / a1 rounded up (a1 just
+// under 1/2, as at 60 degrees with an offset, needs 5)
+int rows_nc(double aa1) { return std::max(2, (int)std::ceil(2.0 / aa1 - 1e-6)); }
+
+// A bound on |b1 (x - xr)| over the candidate columns x of one wave of
+// k_general_rows3b with RJ rows and NC candidates in tilt t (xr its reference
+// column).  The candidates span at most 63 / a1 + 2 drift + NC columns; xr,
+// taken at the rows near j0, sits within |a2 / a1| |Ye - Yc0| + 1 / a1 +
+// (NC + 3) / 2 of their middle, and |Ye - Yc0| <= (1 + |b1 Xr|) / b2 + 1.
+double rows3b_dbound(const TiltGeneral &t, int RJ, int NC, int X)
+{
+    const double a1 = t.a[1], ar = std::fabs(t.a[2] / a1), b1 = std::fabs(t.b[1]), b2 = t.b[2];
+    const double drift = ar * RJ + 0.01;
+    const double span = 63.0 / a1 + 2.0 * drift + NC;
+    const double xr_max = (double)X + std::fabs((double)t.ox) + span + 10.0;  // |Xr| of a wave that is used
+    const double off = ar * ((1.0 + b1 * xr_max) / b2 + 1.0) + 1.0 / a1 + 0.5 * (NC + 3);
+    return b1 * (0.5 * span + off + 1.0);
+}
+
 template <int NR, bool SH>
 void launch_back_lds_sh(const GeneralJob &g, int mode, int theta0, int ntheta, const float *img, dim3 grid,
                         hipStream_t st)
@@ -1227,7 +1489,8 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     if (nzc_used) *nzc_used = 1;
     // TREC_GENERAL_FORWARD=rows|rows5|splat|gather puts that kernel first where it
     // applies (tests and timing).  By default: the row windows when v does
-    // not depend on x, else the splat, else the per-pixel gather.
+    // not depend on x or moves slowly along it, else the splat, else the
+    // per-pixel gather.
     const char *force = getenv("TREC_GENERAL_FORWARD");
     const bool no_rows = force && !strcmp(force, "gather");
     const bool no_splat = force && (!strcmp(force, "gather") || !strcmp(force, "rows") || !strcmp(force, "rows5"));
@@ -1268,6 +1531,42 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     }
     rows = rows && rj > 0 && g.X >= nc;
     const size_t nib = cdiv((size_t)g.X, WG);
+    // row windows for a pitch with an offset (b1 != 0, the same b in every
+    // tilt, a1 >= 1/3): k_general_rows3b with 16, 8 or 4 rows per thread (4
+    // for SART's few-tilt launches) while e (RJ + 2) + 2 D <= 1, each tilt
+    // at its own candidate count (at least 3)
+    bool rowsb = !rows && ntheta > 0 && !no_rows && !(force && !strcmp(force, "rows5"));
+    int ncb = 2, rjb = 0;
+    double dlo = 0.0;
+    for (int th = theta0; th < theta0 + ntheta && rowsb; ++th) {
+        const TiltGeneral &t = host_tg[th];
+        rowsb = t.b[1] != 0.0 && t.b[2] <= 1.0 && t.b[2] >= 0.5 && same_rows(t, host_tg[theta0]) &&
+                t.a[1] >= 1.0 / 3.0;
+        if (rowsb) ncb = std::max(ncb, rows_nc(t.a[1]));
+    }
+    if (rowsb) {
+        const double e = 1.0 - host_tg[theta0].b[2];
+        for (int r = ntheta <= 4 ? 4 : 16; r >= 4 && !rjb; r /= 2) {
+            double d = 0.0;
+            for (int th = theta0; th < theta0 + ntheta; ++th)
+                d = std::max(d, rows3b_dbound(host_tg[th], r, std::max(3, rows_nc(host_tg[th].a[1])), g.X));
+            if (e * (r + 2) + 2.0 * d <= 1.0 - 1e-6) {
+                rjb = r;
+                dlo = -1.0 + e * r + d;
+            }
+        }
+        rowsb = rjb > 0 && g.X >= std::max(3, ncb);
+    }
+    if (rowsb && !(force && !strcmp(force, "splat"))) {
+        const size_t nblk = nib * cdiv((size_t)g.Y, rjb) * (size_t)ntheta * (size_t)nzc;
+        if (nblk <= 0x7fffffffu) {
+            if (rjb == 16) launch_rows3b<16>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, st);
+            else if (rjb == 8) launch_rows3b<8>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, st);
+            else launch_rows3b<4>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, st);
+            if (nzc_used) *nzc_used = nzc;
+            return hipGetLastError();
+        }
+    }
     // the splat: every tilt launched has its tiles
     bool splat = ntheta > 0 && !no_splat && (!rows || (force && !strcmp(force, "splat")));
     for (int th = theta0; th < theta0 + ntheta && splat; ++th) splat = host_tg[th].si > 0;

@@ -455,12 +455,17 @@ def test_back_shapes_equal(torch_cuda, trec, orc, monkeypatch, storage, shapes):
 
 # The general forward kernels one by one (TREC_GENERAL_FORWARD forces one
 # where it applies): the splat (any pitch and offset with a1 >= 1/3), the
-# row windows (offset 0) and the per-pixel gather (everything else), values
-# and ray weights against Reproject at 1e-6, each twice with equal bits.
+# row windows (offset 0, and a pitch with an offset while |b1| over a wave's
+# columns stays under about half a row) and the per-pixel gather (everything
+# else), values and ray weights against Reproject at 1e-6, each twice with
+# equal bits.  With a 0.5 degree pitch and a 1 degree offset the +60 degree
+# tilt has a1 = 0.485 (5 candidate columns); 5 and 4 degrees sit near the row
+# windows' limit on |b1|.
 @pytest.mark.parametrize("kind", ["splat", "rows", "gather"])
 @pytest.mark.parametrize("pitch,offset,span,shape", [
     (0.5, 0.0, 60.0, (7, 37, 10, 300)), (12.0, 0.0, 66.0, (7, 37, 10, 300)), (3.0, 2.0, 60.0, (7, 37, 12, 300)),
-    (6.0, -4.0, 70.0, (5, 40, 9, 130)), (35.0, 5.0, 60.0, (5, 21, 9, 70)), (4.0, 1.0, 60.0, (5, 9, 40, 20))])
+    (6.0, -4.0, 70.0, (5, 40, 9, 130)), (35.0, 5.0, 60.0, (5, 21, 9, 70)), (4.0, 1.0, 60.0, (5, 9, 40, 20)),
+    (0.5, 1.0, 60.0, (9, 37, 12, 300)), (5.0, 4.0, 60.0, (7, 37, 12, 300))])
 def test_general_forward_kernels(torch_cuda, trec, ctx, orc, monkeypatch, kind, pitch, offset, span, shape):
     monkeypatch.setenv("TREC_GENERAL_FORWARD", kind)
     P, Y, T, X = shape
