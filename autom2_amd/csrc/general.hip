@@ -1312,13 +1312,14 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
             constexpr bool EDGE = decltype(edge_tag)::value;
 #pragma unroll
             for (int q = 0; q < GL_Q; ++q) {
+                // tq >= 0 (f0 and fA are fractions), so truncation is its floor
                 const float tq = fmaf((float)q, fA, f0);
-                const float fl = floorf(tq);
                 const int base = I0 + q * IA;
-                const int li = EDGE ? min(max(base + (int)fl, 0), lhi) : base + (int)fl;
-                const float dx = EDGE ? tq - (float)(li - base) : tq - fl;
+                const int lr = q * IA + (int)tq;  // the tap's column past I0
+                const int li = EDGE ? min(max(I0 + lr, 0), lhi) : I0 + lr;
+                const float dx = EDGE ? tq - (float)(li - base) : __builtin_amdgcn_fractf(tq);
                 const float w0 = EDGE ? hat(dx) : 1.f - dx, w1 = EDGE ? hat(dx - 1.f) : dx;
-                const float *l0 = lds + ro[q] + li;
+                const float *l0 = EDGE ? lds + ro[q] + li : (lds + ro[q] + I0) + lr;
                 if constexpr (!PER_TILT) {
                     h0[q] = fmaf(w1, l0[1], fmaf(w0, l0[0], h0[q]));
                     h1[q] = fmaf(w1, l0[PITCH + 1], fmaf(w0, l0[PITCH], h1[q]));
