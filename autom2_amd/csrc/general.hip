@@ -1071,13 +1071,17 @@ __global__ __launch_bounds__(WG) void k_general_back_rows(const GeneralJob g, in
 // < 1) of at most GL_WS columns, double-buffered with one barrier per tilt,
 // and each wave takes its taps from its own part of it.  Every wave
 // evaluates the union itself, with the same expressions, so they agree.
+// With two staged rows and the shared window the kernel runs five waves per
+// SIMD (96 VGPRs, a few spilled), which hides more of the barrier per tilt
+// than four; with three rows four, and six waves or the per-wave window at
+// five spill more than they gain.
 constexpr int GL_Q = 8;        // voxels per lane, 64 apart
 constexpr int GL_BUF_DWORD3 = 0x00020000;  // raw buffer resource, gfx9 family
 constexpr int GL_W = 516;      // staged columns per row
 constexpr int GL_PITCH = 520;  // floats between two staged rows
 constexpr int GL_WS = 528;     // columns of a shared window: GL_W plus the shift of 3 layers, 3 |a3| <= 3
 template <int MODE, int NR, bool SH>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 : 3))) void k_general_back_lds(const GeneralJob g, int theta0, int ntheta,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 2 && SH ? 5 : (NR <= 3 ? 4 : 3)))) void k_general_back_lds(const GeneralJob g, int theta0, int ntheta,
                                                          const float *__restrict__ img)
 {
     constexpr int NRS = SH ? NR + 1 : NR;        // staged rows
@@ -1114,17 +1118,16 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
     const double X0 = (double)((float)xb0 - (float)t0.ox);                  // X of the wave's first voxel
     const double X1 = (double)((float)(xb0 + GL_Q * 64 - 1) - (float)t0.ox);  // and of its last
     const int rlo = min(max((int)fmin(vof(X0), vof(X1)), 0), Y - 2);
-    // the group's rows (SH): Yc and Zc of its wave k, and the least row of all
-    double Ycg[SH ? 4 : 1], Zcg[SH ? 4 : 1];
+    // the group's rows (SH): Yc and Zc of its wave k (formed where they are
+    // used, so that they hold no registers across the tilt loop), and the
+    // least row of all
+    auto ycg = [&](int k) { return (double)((float)(int)(min(row0 + k, rows - 1) / g.T) - (float)t0.oy); };
+    auto zcg = [&](int k) { return (double)((float)(int)(min(row0 + k, rows - 1) % g.T) - (float)t0.oz); };
     int rlo_g = rlo;
     if constexpr (SH) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const unsigned rk = min(row0 + k, rows - 1);
-            Ycg[k] = (double)((float)(int)(rk / g.T) - (float)t0.oy);
-            Zcg[k] = (double)((float)(int)(rk % g.T) - (float)t0.oz);
-            rlo_g = min(rlo_g, min(max((int)fmin(vofr(X0, Ycg[k], Zcg[k]), vofr(X1, Ycg[k], Zcg[k])), 0), Y - 2));
-        }
+        for (int k = 0; k < 4; ++k)
+            rlo_g = min(rlo_g, min(max((int)fmin(vofr(X0, ycg(k), zcg(k)), vofr(X1, ycg(k), zcg(k))), 0), Y - 2));
     }
     unsigned orow[NRS];  // staged rows rlo_g .. rlo_g + NRS - 1 (clamped into the projection)
 #pragma unroll
@@ -1197,7 +1200,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(NR <= 3 ? 4 
             int lo = X, hi = 0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const double ubk = t.a[0] + t.a[2] * Ycg[k] + t.a[3] * Zcg[k] + t.ox;
+                const double ubk = t.a[0] + t.a[2] * ycg(k) + t.a[3] * zcg(k) + t.ox;
                 const double v0 = __fma_rn(a1_v, X0, ubk), v1 = __fma_rn(a1_v, X1, ubk);
                 const int ck = min(max((int)fmin(v0, v1), 0), X - 2);
                 lo = min(lo, ck);
