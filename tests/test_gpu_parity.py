@@ -534,3 +534,31 @@ def test_general_back_shared_window(torch_cuda, trec, ctx, orc, monkeypatch, met
     monkeypatch.setenv("TREC_GENERAL_BACK", "wave")
     own = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset)
     np.testing.assert_array_equal(got, own)
+
+
+# The row windows with per-voxel row weights (a pitch with an offset): both
+# layouts, a negative offset with a z shift, SART's few-tilt launches (4 rows
+# per thread and layer chunks), partial pixel and row blocks (X = 300 or 600,
+# Y = 29 or 37), spans to +-66 degrees (up to 6 candidate columns), against
+# the FP32 oracle and bit for bit on a second run.
+@pytest.mark.parametrize("method,pitch,offset,span,layout,zshift,shape", [
+    ("SIRT,3,1.0", 0.5, 1.0, 60.0, 1, 0.0, (9, 37, 14, 300)),
+    ("SIRT,3,1.0", 3.0, -2.0, 66.0, 0, 2.0, (9, 29, 14, 300)),
+    ("SART,1,0.8", 2.0, 2.0, 60.0, 1, 0.0, (9, 37, 14, 300)),
+    ("SART,1,0.8", 5.0, 4.0, 60.0, 0, -1.5, (7, 21, 12, 600))])
+def test_offset_row_windows(torch_cuda, trec, ctx, orc, method, pitch, offset, span, layout, zshift, shape):
+    P, Y, T, X = shape
+    ang, coef, org, stack = case(orc, P, Y, T, X, span=span, seed=61, pitch=pitch, offset=offset, zshift=zshift)
+    got = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset, zshift=zshift, layout=layout)
+    if layout == 1:
+        got = got.transpose(1, 0, 2)
+    if method.startswith("SIRT"):
+        ref = orc.sirt_f32(coef, stack, T, org, 3, 1.0)
+    else:
+        ref = orc.sart_f32(coef, stack, T, org, 1, 0.8)
+    tag = f"offset rows {method} pitch {pitch} offset {offset} span {span} layout {layout} zshift {zshift}"
+    assert report(tag, got, ref) < TOL_F32
+    again = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset, zshift=zshift, layout=layout)
+    if layout == 1:
+        again = again.transpose(1, 0, 2)
+    np.testing.assert_array_equal(again, got)
