@@ -590,13 +590,19 @@ __global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int th
 // v = cv + b1 X + b2 Y moves along x (b1 = sin(pitch) sin(offset), the same
 // in every tilt), and each voxel weighs its detector rows by its own v:
 //
-//  - ylo is chosen at the wave's reference column xr (near the middle of its
-//    candidate columns): g0 = v(xr, ylo) - j0 lies in [lo, lo + b2), lo =
-//    -1 + e RJ + D (the launcher's dlo), where D bounds |b1 (x - xr)| over
-//    every candidate x of the wave.  The voxel at column x of slot k then has
-//    g = g0 + b1 (x - xr) - e k in [-1, 1] for k = -1 .. RJ while
-//    e RJ + 2 D <= 1, and reaches rows k - 1, k, k + 1 with weights
-//    max(0, -g), 1 - |g| and max(0, g).
+//  - Each lane takes its own slot rows.  The wave's ylo is chosen at its
+//    reference column xr (near the middle of its candidate columns), with
+//    g0 = v(xr, ylo) - j0 in [lo, lo + b2), lo = -1 + e RJ + D (the
+//    launcher's dlo); a lane whose first candidate X1 has g = g0 + b1 (X1 -
+//    xr) outside [lo, lo + b2) shifts its rows by dl = ceil((lo - g) / b2),
+//    |dl| <= dmax (a pixel's u then moves by a2 dl, and X1 with it).  D
+//    bounds |b1 (x - X1)| over a lane's candidates x and the shift of X1,
+//    so the voxel at column x of the lane's slot k has g = g_l + b1 (x - X1)
+//    - e k in [-1, 1] for k = -1 .. RJ while e RJ + 2 D <= 1, and reaches
+//    rows k - 1, k, k + 1 with weights max(0, -g), 1 - |g| and max(0, g).
+//    The per-layer tests (whether the wave's windows reach the row and the
+//    volume, and whether they lie inside both) allow for dmax rows and
+//    a2 / a1 dmax columns of shift.
 //  - With p_c the column-weighted voxel of candidate c and g_c = g_k + b1 c,
 //    rows k - 1, k, k + 1 take (A - G) / 2, S - A and (A + G) / 2 for S =
 //    sum p_c, A = sum p_c |g_c| and G = sum p_c g_c = g_k S + b1 S1,
@@ -609,16 +615,20 @@ __global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int th
 //    ones.
 //  - Slots go in groups of 6, 5, 4 or 3 for 3, 4, 5 or 6 candidates (at
 //    most 20 loaded values, which keeps four waves per SIMD), each group's
-//    loads issued before any of its branches; a slot whose volume row lies
-//    outside the volume loads a clamped row and adds nothing.
+//    loads issued before any of its branches.  Inside the volume a load is
+//    the layer's row ylo - dmax - 1 plus the lane's 32-bit offset (dl + dmax
+//    + k + 1) sy + xs; in a wave at the volume's or the row's edges a lane
+//    whose row lies outside the volume loads a clamped row and adds
+//    nothing.
 //
 // Candidate columns, their weights, the edge handling and the per-layer
 // constants are those of k_general_rows3.  Sums run over z, then slots in y
 // order.
-template <int RJ, bool ONES>
+template <int RJ, bool ONES, bool LANE>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_general_rows3b(const GeneralJob g, int theta0, int ntheta, int nzc,
-                                                       double dlo)
+                                                       double dlo, int dmax_arg)
 {
+    const int dmax = LANE ? dmax_arg : 0;  // without LANE every lane takes the wave's rows
     constexpr int NS = RJ + 2;  // slots k = -1 .. RJ at s = k + 1
     constexpr int NCM = 6;      // most candidate columns (a1 >= 1/3)
     const unsigned nib = (g.X + WG - 1) / WG;
@@ -641,10 +651,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const double ra1 = 1.0 / a1, rb2 = 1.0 / b2;
     const double ox = (float)t.ox, oy = (float)t.oy;
     const float a1f = (float)a1, rkf = (float)(a2 * ra1), ef = (float)(1.0 - b2), b1f = (float)b1;
+    const float b2f = (float)b2, rb2f = (float)rb2, lof = (float)dlo;
+    const double rk = a2 * ra1;
     // the tilt's candidate count
     const int nct = __builtin_amdgcn_readfirstlane(min(NCM, max(3, (int)ceil(2.0 / a1 - 1e-6))));
-    // how far the first candidate moves over the slots, plus rounding slack
-    const double drift = fabs(a2 * ra1) * (double)RJ + 0.01;
+    // how far the first candidate moves over the slots and a lane's row
+    // shift, plus rounding slack
+    const double drift = fabs(rk) * (double)(RJ + dmax) + 0.01;
     const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
     const size_t sz = g.layout == TREC_LAYOUT_Y ? (size_t)g.X : (size_t)g.Y * g.X;
     // constants of layers zb .. zb + 63, lane l holding layer zb + l: u - i
@@ -668,10 +681,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         w_v = cu + a2 * Yc0 - a1 * ox;
         const double la = floor(((double)ia - 1.0 - w_v) * ra1 - drift) + 1.0;  // least first column
         const double lb = floor(((double)ib - 1.0 - w_v) * ra1 + drift) + 1.0;  // greatest first column
-        const bool reach = lb + (double)nct >= 1.0 && la <= (double)(g.X - 1) && ylo_v + RJ >= 0 && ylo_v - 1 < g.Y;
-        const bool inner = la >= 0.0 && lb + (double)nct <= (double)g.X;
+        const bool reach =
+            lb + (double)nct >= 1.0 && la <= (double)(g.X - 1) && ylo_v + dmax + RJ >= 0 && ylo_v - dmax - 1 < g.Y;
+        // with each lane's rows, also every lane's rows inside the volume
+        // (the wave's own rows outside it are skipped slot by slot)
+        const bool inner = la >= 0.0 && lb + (double)nct <= (double)g.X &&
+                           (!LANE || (ylo_v - dmax - 1 >= 0 && ylo_v + dmax + RJ <= g.Y - 1));
         mode_v = reach ? (inner ? 2 : 1) : 0;
     };
+    const unsigned sy4 = (unsigned)(sy * 4u);  // the launcher keeps (2 dmax + RJ + 2) sy + X floats under 4 GiB
     float acc[RJ];
 #pragma unroll
     for (int r = 0; r < RJ; ++r) acc[r] = 0.f;
@@ -692,13 +710,20 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const double W = __longlong_as_double(
             (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(wb >> 32), zl) << 32) |
                         (unsigned)__builtin_amdgcn_readlane((int)(unsigned)wb, zl)));
-        const double F0 = ((double)ic - 1.0 - W) * ra1;
+        // the lane's row shift dl from g at its first candidate, then that
+        // candidate with the rows shifted (u moves by a2 dl) and g there
+        const double F0w = ((double)ic - 1.0 - W) * ra1;
+        const int dl = LANE ? (int)ceilf((lof - fmaf(b1f, (float)((int)floor(F0w) + 1 - xr), g0)) * rb2f) : 0;
+        const double F0 = F0w - rk * (double)dl;
         const double X0d = floor(F0);
         const int X1 = (int)X0d + 1;
         const float f0 = (float)(F0 - X0d);
         // g at column X1 of slot k = 0
-        const float gl = fmaf(b1f, (float)(X1 - xr), g0);
+        const float gl = fmaf(b1f, (float)(X1 - xr), fmaf(b2f, (float)dl, g0));
+        const unsigned roff = (unsigned)(dl + dmax) * sy4;  // the lane's rows past ylo - dmax - 1
         const float *vlayer = g.vol + (size_t)z * sz;
+        const char *vbase =
+            reinterpret_cast<const char *>(vlayer + (size_t)max(ylo - dmax - 1, 0) * sy);  // used when inside
         auto slots = [&](auto edge_tag, auto nc_tag) {
             constexpr bool EDGE = decltype(edge_tag)::value;
             constexpr int NC = decltype(nc_tag)::value;
@@ -730,13 +755,34 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     // g of the slot's first column
                     gk[q] = EDGE ? fmaf(b1f, (float)(xs[q] - X1), fmaf(-ef, (float)k, gl))
                                  : fmaf(b1f, flt, fmaf(-ef, (float)k, gl));
-                    if constexpr (ONES) {
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) v[q][c] = 1.f;
-                    } else {
+                    if constexpr (!LANE) {
+                        // the wave's row (clamped; a slot outside the volume is skipped)
                         const int yc = min(max(ylo + k, 0), g.Y - 1);
                         const float *p = reinterpret_cast<const float *>(
                             reinterpret_cast<const char *>(vlayer + (size_t)yc * sy) + (unsigned)xs[q] * 4u);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) v[q][c] = ONES ? 1.f : p[c];
+                    } else if constexpr (EDGE) {
+                        // the lane's row, clamped into the volume (outside it adds
+                        // nothing), as an offset of at most 2 dmax rows past the
+                        // slot's least row in the volume
+                        const int row = ylo + dl + k;
+                        const bool in = (unsigned)row < (unsigned)g.Y;
+                        const int rb = __builtin_amdgcn_readfirstlane(min(max(ylo - dmax + k, 0), g.Y - 1));
+                        const unsigned off = (unsigned)(min(max(row, rb), g.Y - 1) - rb) * sy4 + (unsigned)xs[q] * 4u;
+                        const float *p = reinterpret_cast<const float *>(
+                            reinterpret_cast<const char *>(vlayer + (size_t)rb * sy) + off);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) v[q][c] = in ? (ONES ? 1.f : p[c]) : 0.f;
+                    } else if constexpr (ONES) {
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) v[q][c] = 1.f;
+                    } else {
+                        // every lane's rows lie inside the volume: slot s of the
+                        // lane is s rows past its first, dl + dmax rows past
+                        // the layer's base row ylo - dmax - 1
+                        const float *p = reinterpret_cast<const float *>(
+                            vbase + (roff + (unsigned)s * sy4 + (unsigned)xs[q] * 4u));
 #pragma unroll
                         for (int c = 0; c < NC; ++c) v[q][c] = p[c];
                     }
@@ -745,7 +791,9 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 for (int q = 0; q < Q; ++q) {
                     const int s = s0 + q, k = s - 1;
                     if (s >= NS) break;
-                    if ((unsigned)(ylo + k) >= (unsigned)g.Y) continue;
+                    // no lane's row of this slot, ylo + k - dmax .. ylo + k +
+                    // dmax, lies inside the volume (one unsigned comparison)
+                    if ((unsigned)(ylo + dmax + k) >= (unsigned)(g.Y + 2 * dmax)) continue;
                     float w[NC];
                     if constexpr (EDGE) {
                         const float d0 = fmaf(a1f, (float)(xs[q] - xf[q]), fmaf(-a1f, fr[q], a1f - 1.f));
@@ -793,7 +841,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     if (s >= 1 && s - 1 < RJ) acc[s - 1] += dm;
                     if (s < RJ) acc[s] += dp;
                 }
-                __builtin_amdgcn_sched_barrier(0);
             }
         };
         auto by_nc = [&](auto edge_tag) {
@@ -1402,14 +1449,24 @@ void launch_rows_nc(const GeneralJob &g, int theta0, int ntheta, bool ones, bool
     }
 }
 
-template <int RJ>
+template <int RJ, bool LANE>
 void launch_rows3b(const GeneralJob &g, int theta0, int ntheta, bool ones, int nzc, unsigned nblk, double dlo,
-                   hipStream_t st)
+                   int dmax, hipStream_t st)
 {
     if (ones)
-        hipLaunchKernelGGL((k_general_rows3b<RJ, true>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc, dlo);
+        hipLaunchKernelGGL((k_general_rows3b<RJ, true, LANE>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc,
+                           dlo, dmax);
     else
-        hipLaunchKernelGGL((k_general_rows3b<RJ, false>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc, dlo);
+        hipLaunchKernelGGL((k_general_rows3b<RJ, false, LANE>), dim3(nblk), dim3(WG), 0, st, g, theta0, ntheta, nzc,
+                           dlo, dmax);
+}
+
+template <int RJ>
+void launch_rows3b_lane(const GeneralJob &g, int theta0, int ntheta, bool ones, int nzc, unsigned nblk, double dlo,
+                        int dmax, bool lane, hipStream_t st)
+{
+    if (lane) launch_rows3b<RJ, true>(g, theta0, ntheta, ones, nzc, nblk, dlo, dmax, st);
+    else launch_rows3b<RJ, false>(g, theta0, ntheta, ones, nzc, nblk, dlo, dmax, st);
 }
 
 // candidate columns a row window needs at a1: 2 / a1 rounded up (a1 just
@@ -1417,11 +1474,11 @@ void launch_rows3b(const GeneralJob &g, int theta0, int ntheta, bool ones, int n
 int rows_nc(double aa1) { return std::max(2, (int)std::ceil(2.0 / aa1 - 1e-6)); }
 
 // A bound on |b1 (x - xr)| over the candidate columns x of one wave of
-// k_general_rows3b with RJ rows and NC candidates in tilt t (xr its reference
+// k_general_rows3b when its lanes take the wave's rows (xr its reference
 // column).  The candidates span at most 63 / a1 + 2 drift + NC columns; xr,
 // taken at the rows near j0, sits within |a2 / a1| |Ye - Yc0| + 1 / a1 +
 // (NC + 3) / 2 of their middle, and |Ye - Yc0| <= (1 + |b1 Xr|) / b2 + 1.
-double rows3b_dbound(const TiltGeneral &t, int RJ, int NC, int X)
+double rows3b_wave_bound(const TiltGeneral &t, int RJ, int NC, int X)
 {
     const double a1 = t.a[1], ar = std::fabs(t.a[2] / a1), b1 = std::fabs(t.b[1]), b2 = t.b[2];
     const double drift = ar * RJ + 0.01;
@@ -1429,6 +1486,27 @@ double rows3b_dbound(const TiltGeneral &t, int RJ, int NC, int X)
     const double xr_max = (double)X + std::fabs((double)t.ox) + span + 10.0;  // |Xr| of a wave that is used
     const double off = ar * ((1.0 + b1 * xr_max) / b2 + 1.0) + 1.0 / a1 + 0.5 * (NC + 3);
     return b1 * (0.5 * span + off + 1.0);
+}
+
+// The bounds of k_general_rows3b<..., LANE> with RJ rows and NC candidates in tilt t:
+// dmax, the most a lane shifts its rows, and D, the most |b1 (x - X1)| over
+// a lane's candidates x.  A wave's candidates span at most 63 / a1 + 2 drift
+// + NC columns; its reference column xr, taken at the rows near j0, sits
+// within |a2 / a1| |Ye - Yc0| + 1 / a1 + (NC + 3) / 2 of their middle, and
+// |Ye - Yc0| <= (1 + |b1 Xr|) / b2 + 1.  So a lane's first candidate has g
+// within |b1| (span / 2 + that) of the wave's g0, and |dl| is at most that
+// over b2, plus one.  A lane's candidates lie within |a2 / a1| (RJ + 1) + NC
+// of X1, which the row shift moves by at most |a2 / a1| dmax + 1 columns.
+void rows3b_bounds(const TiltGeneral &t, int RJ, int NC, int X, double &d, int &dmax)
+{
+    const double a1 = t.a[1], ar = std::fabs(t.a[2] / a1), b1 = std::fabs(t.b[1]), b2 = t.b[2];
+    const double drift = ar * RJ + 0.01;
+    const double span = 63.0 / a1 + 2.0 * drift + NC;
+    const double xr_max = (double)X + std::fabs((double)t.ox) + span + 10.0;  // |Xr| of a wave that is used
+    const double off = ar * ((1.0 + b1 * xr_max) / b2 + 1.0) + 1.0 / a1 + 0.5 * (NC + 3);
+    const double dl = b1 * (0.5 * span + off) / b2 + 1.0;
+    dmax = (int)std::ceil(dl);
+    d = b1 * (ar * (RJ + 1 + dmax) + NC + 2) + 1e-4;
 }
 
 template <int NR, bool SH>
@@ -1548,15 +1626,38 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
                 t.a[1] >= 1.0 / 3.0;
         if (rowsb) ncb = std::max(ncb, rows_nc(t.a[1]));
     }
+    int dmaxb = 0;
+    bool laneb = false;
     if (rowsb) {
+        // per RJ, the wave's rows where their bound holds, else each lane's
+        // (a few more instructions per layer, but |b1| up to about 0.05)
         const double e = 1.0 - host_tg[theta0].b[2];
+        const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
+        const char *flane = getenv("TREC_GENERAL_ROWS_LANE");  // "1": each lane's rows wherever they apply
+        const bool force_lane = flane && !strcmp(flane, "1");
         for (int r = ntheta <= 4 ? 4 : 16; r >= 4 && !rjb; r /= 2) {
-            double d = 0.0;
-            for (int th = theta0; th < theta0 + ntheta; ++th)
-                d = std::max(d, rows3b_dbound(host_tg[th], r, std::max(3, rows_nc(host_tg[th].a[1])), g.X));
-            if (e * (r + 2) + 2.0 * d <= 1.0 - 1e-6) {
+            double dw = 0.0, d = 0.0;
+            int dm = 0;
+            for (int th = theta0; th < theta0 + ntheta; ++th) {
+                const int nct = std::max(3, rows_nc(host_tg[th].a[1]));
+                double dt;
+                int dmt;
+                rows3b_bounds(host_tg[th], r, nct, g.X, dt, dmt);
+                d = std::max(d, dt);
+                dm = std::max(dm, dmt);
+                dw = std::max(dw, rows3b_wave_bound(host_tg[th], r, nct, g.X));
+            }
+            // a lane's load offset, (dl + dmax + s) sy + xs floats, fits 32 bits
+            const bool off32 = ((size_t)(2 * dm + r + 2) * sy + (size_t)g.X) * 4u <= 0xffffffffu;
+            const bool off32w = ((size_t)(r + 2) * sy + (size_t)g.X) * 4u <= 0xffffffffu;
+            if (!force_lane && e * (r + 2) + 2.0 * dw <= 1.0 - 1e-6 && off32w) {
+                rjb = r;
+                dlo = -1.0 + e * r + dw;
+            } else if (e * (r + 2) + 2.0 * d <= 1.0 - 1e-6 && off32) {
                 rjb = r;
                 dlo = -1.0 + e * r + d;
+                dmaxb = dm;
+                laneb = true;
             }
         }
         rowsb = rjb > 0 && g.X >= std::max(3, ncb);
@@ -1564,9 +1665,9 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     if (rowsb && !(force && !strcmp(force, "splat"))) {
         const size_t nblk = nib * cdiv((size_t)g.Y, rjb) * (size_t)ntheta * (size_t)nzc;
         if (nblk <= 0x7fffffffu) {
-            if (rjb == 16) launch_rows3b<16>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, st);
-            else if (rjb == 8) launch_rows3b<8>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, st);
-            else launch_rows3b<4>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, st);
+            if (rjb == 16) launch_rows3b_lane<16>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, dmaxb, laneb, st);
+            else if (rjb == 8) launch_rows3b_lane<8>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, dmaxb, laneb, st);
+            else launch_rows3b_lane<4>(g, theta0, ntheta, ones, nzc, (unsigned)nblk, dlo, dmaxb, laneb, st);
             if (nzc_used) *nzc_used = nzc;
             return hipGetLastError();
         }

@@ -540,13 +540,23 @@ def test_general_back_shared_window(torch_cuda, trec, ctx, orc, monkeypatch, met
 # layouts, a negative offset with a z shift, SART's few-tilt launches (4 rows
 # per thread and layer chunks), partial pixel and row blocks (X = 300 or 600,
 # Y = 29 or 37), spans to +-66 degrees (up to 6 candidate columns), against
-# the FP32 oracle and bit for bit on a second run.
+# the FP32 oracle and bit for bit on a second run.  Small offsets take the
+# wave's rows; 8 with -6, 10 with 5 and 12 with 8 degrees each lane's (a
+# shift of up to 4 rows); lane=1 forces each lane's rows everywhere
+# (TREC_GENERAL_ROWS_LANE=1).
+@pytest.mark.parametrize("lane", ["auto", "1"])
 @pytest.mark.parametrize("method,pitch,offset,span,layout,zshift,shape", [
     ("SIRT,3,1.0", 0.5, 1.0, 60.0, 1, 0.0, (9, 37, 14, 300)),
     ("SIRT,3,1.0", 3.0, -2.0, 66.0, 0, 2.0, (9, 29, 14, 300)),
     ("SART,1,0.8", 2.0, 2.0, 60.0, 1, 0.0, (9, 37, 14, 300)),
-    ("SART,1,0.8", 5.0, 4.0, 60.0, 0, -1.5, (7, 21, 12, 600))])
-def test_offset_row_windows(torch_cuda, trec, ctx, orc, method, pitch, offset, span, layout, zshift, shape):
+    ("SART,1,0.8", 5.0, 4.0, 60.0, 0, -1.5, (7, 21, 12, 600)),
+    ("SIRT,3,1.0", 8.0, -6.0, 60.0, 0, 0.0, (9, 37, 14, 300)),
+    ("SIRT,3,1.0", 10.0, 5.0, 60.0, 1, 1.0, (9, 37, 14, 300)),
+    ("SART,1,0.8", 12.0, 8.0, 50.0, 0, 0.0, (7, 29, 12, 300))])
+def test_offset_row_windows(torch_cuda, trec, ctx, orc, monkeypatch, lane, method, pitch, offset, span, layout, zshift,
+                            shape):
+    if lane == "1":
+        monkeypatch.setenv("TREC_GENERAL_ROWS_LANE", "1")
     P, Y, T, X = shape
     ang, coef, org, stack = case(orc, P, Y, T, X, span=span, seed=61, pitch=pitch, offset=offset, zshift=zshift)
     got = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset, zshift=zshift, layout=layout)
@@ -556,7 +566,7 @@ def test_offset_row_windows(torch_cuda, trec, ctx, orc, method, pitch, offset, s
         ref = orc.sirt_f32(coef, stack, T, org, 3, 1.0)
     else:
         ref = orc.sart_f32(coef, stack, T, org, 1, 0.8)
-    tag = f"offset rows {method} pitch {pitch} offset {offset} span {span} layout {layout} zshift {zshift}"
+    tag = f"offset rows lane={lane} {method} pitch {pitch} offset {offset} span {span} layout {layout} zshift {zshift}"
     assert report(tag, got, ref) < TOL_F32
     again = run(torch_cuda, trec, ctx, stack, ang, T, method, pitch=pitch, offset=offset, zshift=zshift, layout=layout)
     if layout == 1:
