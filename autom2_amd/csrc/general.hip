@@ -590,19 +590,21 @@ __global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int th
 // v = cv + b1 X + b2 Y moves along x (b1 = sin(pitch) sin(offset), the same
 // in every tilt), and each voxel weighs its detector rows by its own v:
 //
-//  - Each lane takes its own slot rows.  The wave's ylo is chosen at its
-//    reference column xr (near the middle of its candidate columns), with
-//    g0 = v(xr, ylo) - j0 in [lo, lo + b2), lo = -1 + e RJ + D (the
-//    launcher's dlo); a lane whose first candidate X1 has g = g0 + b1 (X1 -
-//    xr) outside [lo, lo + b2) shifts its rows by dl = ceil((lo - g) / b2),
-//    |dl| <= dmax (a pixel's u then moves by a2 dl, and X1 with it).  D
-//    bounds |b1 (x - X1)| over a lane's candidates x and the shift of X1,
-//    so the voxel at column x of the lane's slot k has g = g_l + b1 (x - X1)
-//    - e k in [-1, 1] for k = -1 .. RJ while e RJ + 2 D <= 1, and reaches
-//    rows k - 1, k, k + 1 with weights max(0, -g), 1 - |g| and max(0, g).
-//    The per-layer tests (whether the wave's windows reach the row and the
-//    volume, and whether they lie inside both) allow for dmax rows and
-//    a2 / a1 dmax columns of shift.
+//  - The wave's ylo is chosen at its reference column xr (near the middle
+//    of its candidate columns), with g0 = v(xr, ylo) - j0 in [lo, lo + b2),
+//    lo = -1 + e RJ + D (the launcher's dlo).  Without LANE every lane takes
+//    those rows, and D bounds |b1 (x - xr)| over the wave's candidates x.
+//    With LANE a lane whose first candidate X1 has g = g0 + b1 (X1 - xr)
+//    outside [lo, lo + b2) shifts its rows by dl = ceil((lo - g) / b2),
+//    |dl| <= dmax (a pixel's u then moves by a2 dl, and X1 with it), and D
+//    bounds |b1 (x - X1)| over a lane's candidates and the shift of X1:
+//    some instructions more per layer, but |b1| up to about 0.05 instead of
+//    about half a row over the wave's columns.  Either way the voxel at
+//    column x of slot k has g in [-1, 1] for k = -1 .. RJ while
+//    e RJ + 2 D <= 1, and reaches rows k - 1, k, k + 1 with weights
+//    max(0, -g), 1 - |g| and max(0, g).  The per-layer tests (whether the
+//    wave's windows reach the row and the volume, and whether they lie
+//    inside them) allow for dmax rows and a2 / a1 dmax columns of shift.
 //  - With p_c the column-weighted voxel of candidate c and g_c = g_k + b1 c,
 //    rows k - 1, k, k + 1 take (A - G) / 2, S - A and (A + G) / 2 for S =
 //    sum p_c, A = sum p_c |g_c| and G = sum p_c g_c = g_k S + b1 S1,
@@ -615,10 +617,12 @@ __global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int th
 //    ones.
 //  - Slots go in groups of 6, 5, 4 or 3 for 3, 4, 5 or 6 candidates (at
 //    most 20 loaded values, which keeps four waves per SIMD), each group's
-//    loads issued before any of its branches.  Inside the volume a load is
-//    the layer's row ylo - dmax - 1 plus the lane's 32-bit offset (dl + dmax
-//    + k + 1) sy + xs; in a wave at the volume's or the row's edges a lane
-//    whose row lies outside the volume loads a clamped row and adds
+//    loads issued before any of its branches.  A slot none of whose rows
+//    lies inside the volume is skipped.  Without LANE a load is the slot's
+//    row (clamped) plus the lane's column.  With LANE, inside the volume it
+//    is the layer's row ylo - dmax - 1 plus the lane's 32-bit offset (dl +
+//    dmax + k + 1) sy + xs; in a wave at the volume's or the row's edges a
+//    lane whose row lies outside the volume loads a clamped row and adds
 //    nothing.
 //
 // Candidate columns, their weights, the edge handling and the per-layer
@@ -1488,7 +1492,8 @@ double rows3b_wave_bound(const TiltGeneral &t, int RJ, int NC, int X)
     return b1 * (0.5 * span + off + 1.0);
 }
 
-// The bounds of k_general_rows3b<..., LANE> with RJ rows and NC candidates in tilt t:
+// The bounds of k_general_rows3b<..., true> (each lane's rows) with RJ rows
+// and NC candidates in tilt t:
 // dmax, the most a lane shifts its rows, and D, the most |b1 (x - X1)| over
 // a lane's candidates x.  A wave's candidates span at most 63 / a1 + 2 drift
 // + NC columns; its reference column xr, taken at the rows near j0, sits
