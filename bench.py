@@ -64,15 +64,18 @@ def methods_form(trec, ctx, torch, geom, ang, stack_d, reps=3):
     with a 0.5 degree pitch, alone and with a 1 degree tilt-angle offset
     (the geometries a GeoParGen estimate and FindOffset feed back, which run
     the general path, bilinear in both detector axes; with both, the
-    detector row also moves along x), all on the same series (stack
-    resident, volume in layout Y)."""
+    detector row also moves along x), and with a 5 degree pitch and a 4
+    degree offset (the detector row moves by 3 rows over 512 voxels), all on
+    the same series (stack resident, volume in layout Y)."""
     vol = torch.empty((geom.ny, geom.thickness, geom.nx), dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     pitched = trec.make_geom(geom.nx, geom.ny, geom.nproj, geom.thickness, pitch=0.5)
     pitched_off = trec.make_geom(geom.nx, geom.ny, geom.nproj, geom.thickness, offset=1.0, pitch=0.5)
+    pitched_off54 = trec.make_geom(geom.nx, geom.ny, geom.nproj, geom.thickness, offset=4.0, pitch=5.0)
     cases = [("BPT", "BPT", geom), ("WBP", "WBP", geom), ("FBP", "FBP", geom),
              ("ADMM,2,5,0.5,0.001", "ADMM,2,5,0.5,0.001", geom), ("SIRT,20,1.0 pitch 0.5", "SIRT,20,1.0", pitched),
-             ("SIRT,20,1.0 pitch 0.5 offset 1.0", "SIRT,20,1.0", pitched_off)]
+             ("SIRT,20,1.0 pitch 0.5 offset 1.0", "SIRT,20,1.0", pitched_off),
+             ("SIRT,20,1.0 pitch 5.0 offset 4.0", "SIRT,20,1.0", pitched_off54)]
     res = {}
     for label, text, g in cases:
         m = trec.make_method(text)
