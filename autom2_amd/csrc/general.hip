@@ -717,7 +717,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // the lane's row shift dl from g at its first candidate, then that
         // candidate with the rows shifted (u moves by a2 dl) and g there
         const double F0w = ((double)ic - 1.0 - W) * ra1;
-        const int dl = LANE ? (int)ceilf((lof - fmaf(b1f, (float)((int)floor(F0w) + 1 - xr), g0)) * rb2f) : 0;
+        // (the launcher's dmax bounds |dl|; the clamp keeps every load inside
+        // the volume even so)
+        const int dl =
+            LANE ? min(max((int)ceilf((lof - fmaf(b1f, (float)((int)floor(F0w) + 1 - xr), g0)) * rb2f), -dmax), dmax)
+                 : 0;
         const double F0 = F0w - rk * (double)dl;
         const double X0d = floor(F0);
         const int X1 = (int)X0d + 1;
