@@ -520,13 +520,21 @@ trec_status plan_job(trec_context *ctx, JobPlan &p, const trec_geom &g, const tr
             t.oy = oy;
             t.oz = oz;
         }
-        // SIRT's splat forward (a pitch with an offset) sums chunks of about
-        // 32 layers into partials, so that the waves in flight on one XCD
-        // share their volume rows in its L2; the partials take at most 4 GB
+        // SIRT's splat forward (a pitch with an offset the row windows do not
+        // take) sums chunks of about 32 layers into partials, so that the
+        // waves in flight on one XCD share their volume rows in its L2; the
+        // partials take at most 4 GB.  The row windows need none, and the
+        // forward takes them where rows3b_plan says so (as its launcher does).
         q.nzc_all = 1;
         bool flat = true;
         for (const TiltGeneral &t : p.tg) flat = flat && t.b[1] == 0.0;
-        if (general_tiles(p.tg.data(), g.nproj, g.nx) && !flat && m.method == TREC_SIRT) {
+        const char *force = getenv("TREC_GENERAL_FORWARD");
+        const bool forced = force && (!strcmp(force, "splat") || !strcmp(force, "rows") ||
+                                      !strcmp(force, "rows5") || !strcmp(force, "gather"));
+        Rows3bPlan rp;
+        const bool splat_runs = forced ? !strcmp(force, "splat")
+                                       : !rows3b_plan(p.tg.data(), 0, g.nproj, g.nx, g.thickness, m.layout, rp);
+        if (general_tiles(p.tg.data(), g.nproj, g.nx) && !flat && m.method == TREC_SIRT && splat_runs) {
             const size_t one = 4 * (size_t)g.nproj * g.ny * g.nx;
             q.nzc_all = (int)std::min<size_t>({8, (size_t)(g.thickness + 31) / 32, ((size_t)4 << 30) / one});
             if (q.nzc_all < 2) q.nzc_all = 1;

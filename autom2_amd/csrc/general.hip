@@ -1574,6 +1574,60 @@ bool general_tiles(TiltGeneral *tg, int P, int X)
     return all;
 }
 
+// The row windows for a pitch with an offset (b1 != 0, the same b in every
+// tilt, a1 >= 1/3, 1/2 <= b2 <= 1): k_general_rows3b with 16, 8 or 4 rows
+// per thread (4 for SART's few-tilt launches) while e (RJ + 2) + 2 D <= 1,
+// each tilt at its own candidate count (at least 3).  Per RJ the wave's
+// rows where their bound holds, else each lane's (a few more instructions
+// per layer, but |b1| up to about 0.05); TREC_GENERAL_ROWS_LANE=1 takes each
+// lane's rows wherever they apply.
+bool rows3b_plan(const TiltGeneral *host_tg, int theta0, int ntheta, int X, int T, int layout, Rows3bPlan &rp)
+{
+    rp = Rows3bPlan{};
+    if (ntheta <= 0) return false;
+    int ncb = 2;
+    for (int th = theta0; th < theta0 + ntheta; ++th) {
+        const TiltGeneral &t = host_tg[th];
+        if (!(t.b[1] != 0.0 && t.b[2] <= 1.0 && t.b[2] >= 0.5 && same_rows(t, host_tg[theta0]) && t.a[1] >= 1.0 / 3.0))
+            return false;
+        ncb = std::max(ncb, rows_nc(t.a[1]));
+    }
+    if (X < std::max(3, ncb)) return false;
+    const double e = 1.0 - host_tg[theta0].b[2];
+    const size_t sy = layout == TREC_LAYOUT_Y ? (size_t)T * X : (size_t)X;
+    const char *flane = getenv("TREC_GENERAL_ROWS_LANE");
+    const bool force_lane = flane && !strcmp(flane, "1");
+    for (int r = ntheta <= 4 ? 4 : 16; r >= 4; r /= 2) {
+        double dw = 0.0, d = 0.0;
+        int dm = 0;
+        for (int th = theta0; th < theta0 + ntheta; ++th) {
+            const int nct = std::max(3, rows_nc(host_tg[th].a[1]));
+            double dt;
+            int dmt;
+            rows3b_bounds(host_tg[th], r, nct, X, dt, dmt);
+            d = std::max(d, dt);
+            dm = std::max(dm, dmt);
+            dw = std::max(dw, rows3b_wave_bound(host_tg[th], r, nct, X));
+        }
+        // a lane's load offset, (dl + dmax + s) sy + xs floats, fits 32 bits
+        const bool off32 = ((size_t)(2 * dm + r + 2) * sy + (size_t)X) * 4u <= 0xffffffffu;
+        const bool off32w = ((size_t)(r + 2) * sy + (size_t)X) * 4u <= 0xffffffffu;
+        if (!force_lane && e * (r + 2) + 2.0 * dw <= 1.0 - 1e-6 && off32w) {
+            rp.rj = r;
+            rp.dlo = -1.0 + e * r + dw;
+            return true;
+        }
+        if (e * (r + 2) + 2.0 * d <= 1.0 - 1e-6 && off32) {
+            rp.rj = r;
+            rp.dlo = -1.0 + e * r + d;
+            rp.dmax = dm;
+            rp.lane = true;
+            return true;
+        }
+    }
+    return false;
+}
+
 hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
                                   hipStream_t st, int *nzc_used)
 {
@@ -1622,55 +1676,13 @@ hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_t
     }
     rows = rows && rj > 0 && g.X >= nc;
     const size_t nib = cdiv((size_t)g.X, WG);
-    // row windows for a pitch with an offset (b1 != 0, the same b in every
-    // tilt, a1 >= 1/3): k_general_rows3b with 16, 8 or 4 rows per thread (4
-    // for SART's few-tilt launches) while e (RJ + 2) + 2 D <= 1, each tilt
-    // at its own candidate count (at least 3)
-    bool rowsb = !rows && ntheta > 0 && !no_rows && !(force && !strcmp(force, "rows5"));
-    int ncb = 2, rjb = 0;
-    double dlo = 0.0;
-    for (int th = theta0; th < theta0 + ntheta && rowsb; ++th) {
-        const TiltGeneral &t = host_tg[th];
-        rowsb = t.b[1] != 0.0 && t.b[2] <= 1.0 && t.b[2] >= 0.5 && same_rows(t, host_tg[theta0]) &&
-                t.a[1] >= 1.0 / 3.0;
-        if (rowsb) ncb = std::max(ncb, rows_nc(t.a[1]));
-    }
-    int dmaxb = 0;
-    bool laneb = false;
-    if (rowsb) {
-        // per RJ, the wave's rows where their bound holds, else each lane's
-        // (a few more instructions per layer, but |b1| up to about 0.05)
-        const double e = 1.0 - host_tg[theta0].b[2];
-        const size_t sy = g.layout == TREC_LAYOUT_Y ? (size_t)g.T * g.X : (size_t)g.X;
-        const char *flane = getenv("TREC_GENERAL_ROWS_LANE");  // "1": each lane's rows wherever they apply
-        const bool force_lane = flane && !strcmp(flane, "1");
-        for (int r = ntheta <= 4 ? 4 : 16; r >= 4 && !rjb; r /= 2) {
-            double dw = 0.0, d = 0.0;
-            int dm = 0;
-            for (int th = theta0; th < theta0 + ntheta; ++th) {
-                const int nct = std::max(3, rows_nc(host_tg[th].a[1]));
-                double dt;
-                int dmt;
-                rows3b_bounds(host_tg[th], r, nct, g.X, dt, dmt);
-                d = std::max(d, dt);
-                dm = std::max(dm, dmt);
-                dw = std::max(dw, rows3b_wave_bound(host_tg[th], r, nct, g.X));
-            }
-            // a lane's load offset, (dl + dmax + s) sy + xs floats, fits 32 bits
-            const bool off32 = ((size_t)(2 * dm + r + 2) * sy + (size_t)g.X) * 4u <= 0xffffffffu;
-            const bool off32w = ((size_t)(r + 2) * sy + (size_t)g.X) * 4u <= 0xffffffffu;
-            if (!force_lane && e * (r + 2) + 2.0 * dw <= 1.0 - 1e-6 && off32w) {
-                rjb = r;
-                dlo = -1.0 + e * r + dw;
-            } else if (e * (r + 2) + 2.0 * d <= 1.0 - 1e-6 && off32) {
-                rjb = r;
-                dlo = -1.0 + e * r + d;
-                dmaxb = dm;
-                laneb = true;
-            }
-        }
-        rowsb = rjb > 0 && g.X >= std::max(3, ncb);
-    }
+    // row windows for a pitch with an offset (rows3b_plan)
+    Rows3bPlan rp;
+    const bool rowsb = !rows && ntheta > 0 && !no_rows && !(force && !strcmp(force, "rows5")) &&
+                       rows3b_plan(host_tg, theta0, ntheta, g.X, g.T, g.layout, rp);
+    const int rjb = rp.rj, dmaxb = rp.dmax;
+    const double dlo = rp.dlo;
+    const bool laneb = rp.lane;
     if (rowsb && !(force && !strcmp(force, "splat"))) {
         const size_t nblk = nib * cdiv((size_t)g.Y, rjb) * (size_t)ntheta * (size_t)nzc;
         if (nblk <= 0x7fffffffu) {

@@ -205,6 +205,17 @@ constexpr int GZ_CHUNKS = 4;
 // host_tg: the same tilt table on the host, for the choice of kernel
 // nzc_used (optional): the caller takes layer-chunk partials in g.part; the
 // number of chunks written there (1: the projection is in rval itself)
+// The row-window forward for a pitch with an offset (k_general_rows3b): whether
+// tilts theta0 .. theta0 + ntheta - 1 of a volume X wide and T thick take it,
+// and with which rows per thread (rj), slot offset (dlo), most row shift
+// (dmax) and row mode (lane: each lane's rows).
+struct Rows3bPlan {
+    int rj = 0;
+    double dlo = 0.0;
+    int dmax = 0;
+    bool lane = false;
+};
+bool rows3b_plan(const TiltGeneral *host_tg, int theta0, int ntheta, int X, int T, int layout, Rows3bPlan &rp);
 hipError_t launch_general_forward(const GeneralJob &g, const TiltGeneral *host_tg, int theta0, int ntheta, bool ones,
                                   hipStream_t st, int *nzc_used = nullptr);
 // quot: -rval / rwt above the threshold, 0 elsewhere (ADMM's operator on a
