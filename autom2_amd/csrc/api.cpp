@@ -1156,6 +1156,13 @@ std::string graph_key(trec_context *ctx, const std::vector<JobPlan *> &plans)
     char b[640];
     snprintf(b, sizeof b, "%p|", (void *)ctx->arena);
     k += b;
+    // the variables that force a general-path kernel choose at launch, so a
+    // graph captured under one setting must not replay under another
+    for (const char *v : {"TREC_GENERAL_FORWARD", "TREC_GENERAL_ROWS_LANE", "TREC_GENERAL_BACK"}) {
+        const char *e = getenv(v);
+        k += e ? e : "-";
+        k += '|';
+    }
     for (auto *pp : plans) {
         const JobPlan &p = *pp;
         const uint64_t hc = fnv1a(p.coef.data(), sizeof(double) * p.coef.size());
