@@ -489,6 +489,37 @@ def test_general_forward_kernels(torch_cuda, trec, ctx, orc, monkeypatch, kind, 
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+# A context replays the graph it captured while the call, its buffers and the
+# forcing variables are unchanged.  The same SIRT call on the same buffers
+# under the row windows, then under the splat, must give the splat's result
+# (that of a fresh context), not a replay of the row windows' graph.
+def test_graph_follows_forced_kernel(torch_cuda, trec, ctx, orc, monkeypatch):
+    torch = torch_cuda
+    P, Y, T, X = 9, 37, 12, 300
+    ang, coef, org, stack = case(orc, P, Y, T, X, seed=62, pitch=0.5, offset=1.0)
+    g = trec.make_geom(X, Y, P, T, 1.0, 0.5, 0.0)
+    m = trec.make_method("SIRT,3,1.0", layout=0)
+    s = dev(torch, stack)
+    out = torch.empty((Y, T, X), dtype=torch.float32, device="cuda")
+    res = {}
+    for kind in ("rows", "splat"):
+        monkeypatch.setenv("TREC_GENERAL_FORWARD", kind)
+        ctx.reconstruct(g, ang, s, m, out)
+        torch.cuda.synchronize()
+        res[kind] = out.cpu().numpy()
+    c2 = trec.Context(0)
+    try:
+        c2.reconstruct(g, ang, s, m, out)  # still the splat
+        torch.cuda.synchronize()
+        fresh = out.cpu().numpy()
+    finally:
+        c2.close()
+    print(f"row windows and splat differ in {int((res['rows'] != res['splat']).sum())} voxels")
+    ref = orc.sirt_f32(coef, stack, T, org, 3, 1.0)
+    assert report("splat after the row windows", res["splat"], ref) < TOL_F32
+    np.testing.assert_array_equal(res["splat"], fresh)
+
+
 # A pitch with an offset (b1 != 0): v moves along x, so the back-projection
 # stages 3 or 4 detector rows per wave (|b1| up to 2/511), or reads its taps
 # from memory beyond.  X = 600 spans two waves of 512 voxels per row.
