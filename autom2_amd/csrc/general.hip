@@ -443,7 +443,10 @@ template <int RJ, int NC, bool ONES>
 __global__ __launch_bounds__(WG) void k_general_rows3(const GeneralJob g, int theta0, int ntheta, int nzc)
 {
     constexpr int NS = RJ + 2;  // slots k = -1 .. RJ at s = k + 1
-    constexpr int SB = 6;       // slots whose loads may be in flight together
+    // slots whose loads may be in flight together: the 18 slots of 16 rows
+    // in two groups of 9 (2.4 % faster than groups of 6, at the same 4 waves
+    // per SIMD), else 6
+    constexpr int SB = RJ == 16 ? 9 : 6;
     static_assert(NS <= 64, "a lane per slot");
     __shared__ float4 ywl[WG / 64][NS];
     float4 *yw = ywl[threadIdx.x / 64];
